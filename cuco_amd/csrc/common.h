@@ -1,0 +1,153 @@
+// Internal definitions shared by the engine's HIP translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+namespace cuco {
+
+constexpr int kNTab = 1000;       // allvars.h:27
+constexpr int kTabPadded = 1024;  // FP32 table padded with zeros past rcut
+
+// Key layouts of the cell sort.
+enum KeyMode : int {
+    kKeyXMajor = 0,  // (ix*M+iy)*M+iz, the reference order (gravlist_kernel.cu:114)
+    kKeyMorton = 1,  // bit-interleaved (x,y,z): compact tiles for the FP32 PP kernel
+};
+
+// Scalars every kernel of one evaluation needs; passed by value.
+struct GridInfo {
+    int32_t M;          // mesh cells per dimension
+    int32_t key_mode;   // KeyMode
+    int32_t bits;       // bits per coordinate of the Morton key
+    int32_t key_bits;   // total key bits
+    uint32_t nkeys;     // size of the key space (table has nkeys+1 entries)
+    double L;           // box size
+    double fac;         // M/L (to_grid_fac, gravlist.cpp:61)
+    double cell;        // L/M
+};
+
+struct PairParams {
+    double L, boxhalf;
+    double rcut, rcut2;
+    double asmthfac;
+    double h, h_inv, h3_inv;
+    double G;
+    double fac;  // M/L
+    int32_t M;
+};
+
+// ---------------------------------------------------------------- keys
+__host__ __device__ inline uint32_t spread3(uint32_t v)
+{
+    // Inserts two zero bits between the low 10 bits of v.
+    v &= 0x3ffu;
+    v = (v | (v << 16)) & 0x030000ffu;
+    v = (v | (v << 8)) & 0x0300f00fu;
+    v = (v | (v << 4)) & 0x030c30c3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+__host__ __device__ inline uint32_t compact3(uint32_t v)
+{
+    v &= 0x09249249u;
+    v = (v | (v >> 2)) & 0x030c30c3u;
+    v = (v | (v >> 4)) & 0x0300f00fu;
+    v = (v | (v >> 8)) & 0x030000ffu;
+    v = (v | (v >> 16)) & 0x000003ffu;
+    return v;
+}
+
+__host__ __device__ inline uint32_t cell_key(const GridInfo &g, int ix, int iy,
+                                             int iz)
+{
+    if (g.key_mode == kKeyXMajor)
+        return (uint32_t)(((int64_t)ix * g.M + iy) * g.M + iz);
+    return (spread3((uint32_t)ix) << 2) | (spread3((uint32_t)iy) << 1) |
+           spread3((uint32_t)iz);
+}
+
+__host__ __device__ inline void key_cell(const GridInfo &g, uint32_t key,
+                                         int &ix, int &iy, int &iz)
+{
+    if (g.key_mode == kKeyXMajor) {
+        iz = (int)(key % (uint32_t)g.M);
+        uint32_t r = key / (uint32_t)g.M;
+        iy = (int)(r % (uint32_t)g.M);
+        ix = (int)(r / (uint32_t)g.M);
+    } else {
+        ix = (int)compact3(key >> 2);
+        iy = (int)compact3(key >> 1);
+        iz = (int)compact3(key);
+    }
+}
+
+// (int)(fac*x) clamped to [0, M-1]: gravlist_kernel.cu:111-113 with the
+// clamp of pm_periodic.c:100-101.
+//
+// The clamp happens in floating point, which gives the same index for every
+// position in [0, L) and keeps a non-finite value from becoming an
+// out-of-range integer (fmax(NaN, 0) is 0).
+__host__ __device__ inline int cell_coord(double fac, double x, int M)
+{
+    return (int)fmin(fmax(fac * x, 0.0), (double)(M - 1));
+}
+
+__host__ __device__ inline int wrap_index(int a, int M)
+{
+    a = a >= M ? a - M : a;
+    return a < 0 ? a + M : a;
+}
+
+// ---------------------------------------------------------------- complex
+template <typename T>
+struct Cx {
+    T x, y;
+};
+
+template <typename T>
+__device__ inline Cx<T> cmul(Cx<T> a, Cx<T> b)
+{
+    return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+
+template <typename T>
+__device__ inline Cx<T> cadd(Cx<T> a, Cx<T> b) { return {a.x + b.x, a.y + b.y}; }
+template <typename T>
+__device__ inline Cx<T> csub(Cx<T> a, Cx<T> b) { return {a.x - b.x, a.y - b.y}; }
+template <typename T>
+__device__ inline Cx<T> cconj(Cx<T> a) { return {a.x, -a.y}; }
+
+// ---------------------------------------------------------------- errors
+void set_error(const char *fmt, ...);
+
+#define CUCO_HIP(call)                                                        \
+    do {                                                                      \
+        hipError_t err_ = (call);                                             \
+        if (err_ != hipSuccess) {                                             \
+            ::cuco::set_error("%s failed: %s (%s:%d)", #call,                 \
+                              hipGetErrorString(err_), __FILE__, __LINE__);  \
+            return CUCO_ERR_HIP;                                              \
+        }                                                                     \
+    } while (0)
+
+#define CUCO_LAUNCH_CHECK()                                                   \
+    do {                                                                      \
+        hipError_t err_ = hipGetLastError();                                  \
+        if (err_ != hipSuccess) {                                             \
+            ::cuco::set_error("kernel launch failed: %s (%s:%d)",             \
+                              hipGetErrorString(err_), __FILE__, __LINE__);  \
+            return CUCO_ERR_HIP;                                              \
+        }                                                                     \
+    } while (0)
+
+inline unsigned grid_for(int64_t n, int block)
+{
+    return (unsigned)((n + block - 1) / block);
+}
+
+}  // namespace cuco

@@ -1,0 +1,616 @@
+// PM long-range force: CIC deposit, 3D r2c FFT, Green's function, c2r FFT,
+// four-point gradient and CIC interpolation (pm_periodic.c:67-314).
+//
+// Layout in HBM: real grids are M^3 x-major (index (x*M+y)*M+z), unpadded;
+// the complex grid is M x M x (M/2+1), z innermost.
+//
+//  deposit  one thread per mesh node gathers the particles of its eight
+//           neighbouring cells (sorted by cell, read through the CSR table)
+//           with the reference's CIC weights (pm_periodic.c:97-132).  Every
+//           node is written once, in a fixed order: no atomics.
+//  FFT      hand-written Stockham autosort FFT in LDS, radix 4/2/3/5:
+//           z rows: real length-M r2c through a length-M/2 complex FFT and the
+//                   split post-process; c2r the mirror image;
+//           y lines: strided batched complex FFT, lines contiguous in z so
+//                   every load/store is a coalesced row segment;
+//           x lines: forward FFT, multiply by the Green's function
+//                   (pm_periodic.c:146-197; DC zeroed, :197), inverse FFT,
+//                   all in one LDS residency.
+//  interp   one thread per particle: the gradient stencil of pm_periodic.c:
+//           260-267 evaluated at the 8 CIC nodes and summed with the CIC
+//           weights in the reference order (pm_periodic.c:297-306).
+//
+// This file is compiled with -ffp-contract=off so the FP64 instantiations
+// keep the reference's rounding sequence for the weights and stencil.
+#include "internal.h"
+
+namespace cuco {
+
+namespace {
+
+constexpr int kFftThreads = 256;
+constexpr int kFftBytes = 65536;  // LDS tile per block: 8192 FP32 / 4096 FP64 complex
+
+template <typename T>
+constexpr int fft_elems() { return kFftBytes / (int)sizeof(Cx<T>); }
+
+struct FftPlan {
+    int nstages;
+    int radix[16];
+};
+
+FftPlan make_plan(int n)
+{
+    FftPlan p{};
+    int m = n;
+    while (m % 4 == 0) { p.radix[p.nstages++] = 4; m /= 4; }
+    while (m % 2 == 0) { p.radix[p.nstages++] = 2; m /= 2; }
+    while (m % 3 == 0) { p.radix[p.nstages++] = 3; m /= 3; }
+    while (m % 5 == 0) { p.radix[p.nstages++] = 5; m /= 5; }
+    if (m != 1) p.nstages = -1;
+    return p;
+}
+
+// ------------------------------------------------------------- small DFTs
+template <typename T, int R, bool INV>
+__device__ inline void dft(Cx<T> *v)
+{
+    if (R == 2) {
+        Cx<T> a = v[0], b = v[1];
+        v[0] = cadd(a, b);
+        v[1] = csub(a, b);
+    } else if (R == 4) {
+        Cx<T> a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
+        Cx<T> a2 = cadd(v[1], v[3]), a3 = csub(v[1], v[3]);
+        // i*a3 = (-a3.y, a3.x)
+        Cx<T> ia3 = {-a3.y, a3.x};
+        v[0] = cadd(a0, a2);
+        v[2] = csub(a0, a2);
+        if (!INV) {
+            v[1] = csub(a1, ia3);
+            v[3] = cadd(a1, ia3);
+        } else {
+            v[1] = cadd(a1, ia3);
+            v[3] = csub(a1, ia3);
+        }
+    } else if (R == 3) {
+        const T s3 = (T)0.86602540378443864676;
+        Cx<T> sum = cadd(v[1], v[2]), dif = csub(v[1], v[2]);
+        Cx<T> t1 = {v[0].x - (T)0.5 * sum.x, v[0].y - (T)0.5 * sum.y};
+        // t2 = -i*s3*dif (forward), +i*s3*dif (inverse)
+        Cx<T> t2 = INV ? Cx<T>{-s3 * dif.y, s3 * dif.x} : Cx<T>{s3 * dif.y, -s3 * dif.x};
+        v[0] = cadd(v[0], sum);
+        v[1] = cadd(t1, t2);
+        v[2] = csub(t1, t2);
+    } else if (R == 5) {
+        const T c1 = (T)0.30901699437494742410, c2 = (T)-0.80901699437494742410;
+        const T s1 = (T)0.95105651629515357212, s2 = (T)0.58778525229247312917;
+        Cx<T> a1 = cadd(v[1], v[4]), b1 = csub(v[1], v[4]);
+        Cx<T> a2 = cadd(v[2], v[3]), b2 = csub(v[2], v[3]);
+        Cx<T> x0 = v[0];
+        Cx<T> r1 = {x0.x + c1 * a1.x + c2 * a2.x, x0.y + c1 * a1.y + c2 * a2.y};
+        Cx<T> r2 = {x0.x + c2 * a1.x + c1 * a2.x, x0.y + c2 * a1.y + c1 * a2.y};
+        // forward: y1 = r1 - i(s1 b1 + s2 b2), y4 = r1 + i(...)
+        //          y2 = r2 - i(s2 b1 - s1 b2), y3 = r2 + i(...)
+        Cx<T> q1 = {s1 * b1.x + s2 * b2.x, s1 * b1.y + s2 * b2.y};
+        Cx<T> q2 = {s2 * b1.x - s1 * b2.x, s2 * b1.y - s1 * b2.y};
+        Cx<T> iq1 = {-q1.y, q1.x}, iq2 = {-q2.y, q2.x};
+        v[0] = {x0.x + a1.x + a2.x, x0.y + a1.y + a2.y};
+        if (!INV) {
+            v[1] = csub(r1, iq1);
+            v[4] = cadd(r1, iq1);
+            v[2] = csub(r2, iq2);
+            v[3] = cadd(r2, iq2);
+        } else {
+            v[1] = cadd(r1, iq1);
+            v[4] = csub(r1, iq1);
+            v[2] = cadd(r2, iq2);
+            v[3] = csub(r2, iq2);
+        }
+    }
+}
+
+// Element (l, j) of the LDS tile: ROWS keeps a line contiguous, COLS keeps
+// the B lines of one position contiguous.
+template <bool ROWS>
+__device__ inline int lidx(int l, int j, int ld)
+{
+    return ROWS ? l * ld + j : j * ld + l;
+}
+
+// One Stockham stage over all B lines of length Nf held in LDS.
+template <typename T, int R, bool ROWS, bool INV>
+__device__ void stage(Cx<T> *lds, int Nf, int B, int ld, int Ns,
+                      const Cx<T> *__restrict__ tw, int twstride)
+{
+    constexpr int MAXB = (fft_elems<T>() / R + kFftThreads - 1) / kFftThreads;
+    const int span = Nf / R;
+    const int nb = span * B;
+    Cx<T> v[MAXB][R];
+    int jl[MAXB];
+#pragma unroll
+    for (int q = 0; q < MAXB; ++q) {
+        const int b = threadIdx.x + q * kFftThreads;
+        if (b < nb) {
+            int l, j;
+            if (ROWS) { j = b % span; l = b / span; } else { l = b % B; j = b / B; }
+            jl[q] = j;
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[q][r] = lds[lidx<ROWS>(l, j + r * span, ld)];
+            const int k = j % Ns;
+            if (Ns > 1) {
+                const int tstep = (Nf / (Ns * R)) * twstride;
+#pragma unroll
+                for (int r = 1; r < R; ++r) {
+                    Cx<T> w = tw[k * r * tstep];
+                    if (INV) w.y = -w.y;
+                    v[q][r] = cmul(v[q][r], w);
+                }
+            }
+            dft<T, R, INV>(v[q]);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < MAXB; ++q) {
+        const int b = threadIdx.x + q * kFftThreads;
+        if (b < nb) {
+            const int l = ROWS ? b / span : b % B;
+            const int j = jl[q];
+            const int k = j % Ns;
+            const int d = (j / Ns) * Ns * R + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) lds[lidx<ROWS>(l, d + r * Ns, ld)] = v[q][r];
+        }
+    }
+    __syncthreads();
+}
+
+template <typename T, bool ROWS, bool INV>
+__device__ void run_fft(Cx<T> *lds, int Nf, int B, int ld, const FftPlan &plan,
+                        const Cx<T> *__restrict__ tw, int twstride)
+{
+    int Ns = 1;
+    for (int s = 0; s < plan.nstages; ++s) {
+        const int R = plan.radix[s];
+        if (R == 4) stage<T, 4, ROWS, INV>(lds, Nf, B, ld, Ns, tw, twstride);
+        else if (R == 2) stage<T, 2, ROWS, INV>(lds, Nf, B, ld, Ns, tw, twstride);
+        else if (R == 3) stage<T, 3, ROWS, INV>(lds, Nf, B, ld, Ns, tw, twstride);
+        else stage<T, 5, ROWS, INV>(lds, Nf, B, ld, Ns, tw, twstride);
+        Ns *= R;
+    }
+}
+
+template <typename T>
+__device__ inline Cx<T> *lds_tile()
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    return reinterpret_cast<Cx<T> *>(smem);
+}
+
+// ------------------------------------------------------------- z rows
+// Real rows of length M -> M/2+1 complex (forward, unnormalized).
+template <typename T>
+__global__ __launch_bounds__(kFftThreads) void k_rows_r2c(
+    const T *__restrict__ in, Cx<T> *__restrict__ out, int M, int nrows,
+    FftPlan plan, const Cx<T> *__restrict__ tw)
+{
+    Cx<T> *lds = lds_tile<T>();
+    const int Nf = M / 2, Zc = M / 2 + 1;
+    const int B = fft_elems<T>() / Nf;
+    const int row0 = blockIdx.x * B;
+    const Cx<T> *inc = reinterpret_cast<const Cx<T> *>(in);
+    for (int e = threadIdx.x; e < Nf * B; e += kFftThreads) {
+        const int l = e / Nf, j = e % Nf;
+        const int row = row0 + l;
+        lds[l * Nf + j] = row < nrows ? inc[(size_t)row * Nf + j] : Cx<T>{0, 0};
+    }
+    __syncthreads();
+    run_fft<T, true, false>(lds, Nf, B, Nf, plan, tw, 2);
+    for (int e = threadIdx.x; e < Zc * B; e += kFftThreads) {
+        const int l = e / Zc, k = e % Zc;
+        const int row = row0 + l;
+        if (row >= nrows) continue;
+        const Cx<T> zk = lds[l * Nf + (k % Nf)];
+        const Cx<T> zm = cconj(lds[l * Nf + ((Nf - k) % Nf)]);
+        const Cx<T> s = cadd(zk, zm), d = csub(zk, zm);
+        const Cx<T> wd = cmul(tw[k], d);  // W_M^k (Z_k - conj Z_{N/2-k})
+        Cx<T> x;
+        x.x = (T)0.5 * s.x + (T)0.5 * wd.y;
+        x.y = (T)0.5 * s.y - (T)0.5 * wd.x;
+        out[(size_t)row * Zc + k] = x;
+    }
+}
+
+// M/2+1 complex -> real rows of length M (inverse, unnormalized).
+template <typename T>
+__global__ __launch_bounds__(kFftThreads) void k_rows_c2r(
+    const Cx<T> *__restrict__ in, T *__restrict__ out, int M, int nrows,
+    FftPlan plan, const Cx<T> *__restrict__ tw)
+{
+    Cx<T> *lds = lds_tile<T>();
+    const int Nf = M / 2, Zc = M / 2 + 1;
+    const int B = fft_elems<T>() / Nf;
+    const int row0 = blockIdx.x * B;
+    for (int e = threadIdx.x; e < Nf * B; e += kFftThreads) {
+        const int l = e / Nf, k = e % Nf;
+        const int row = row0 + l;
+        Cx<T> z = {0, 0};
+        if (row < nrows) {
+            Cx<T> xk = in[(size_t)row * Zc + k];
+            Cx<T> xm = in[(size_t)row * Zc + (Nf - k)];
+            if (k == 0) { xk.y = 0; xm.y = 0; }  // DC and Nyquist are real
+            const Cx<T> c = cconj(xm);
+            const Cx<T> s = cadd(xk, c), d = csub(xk, c);
+            const Cx<T> wd = cmul(cconj(tw[k]), d);
+            z.x = s.x - wd.y;
+            z.y = s.y + wd.x;
+        }
+        lds[l * Nf + k] = z;
+    }
+    __syncthreads();
+    run_fft<T, true, true>(lds, Nf, B, Nf, plan, tw, 2);
+    Cx<T> *outc = reinterpret_cast<Cx<T> *>(out);
+    for (int e = threadIdx.x; e < Nf * B; e += kFftThreads) {
+        const int l = e / Nf, j = e % Nf;
+        const int row = row0 + l;
+        if (row < nrows) outc[(size_t)row * Nf + j] = lds[l * Nf + j];
+    }
+}
+
+// ------------------------------------------------------------- strided lines
+// Line L starts at (L / inner) * slab + (L % inner); elements are `stride`
+// apart.  Consecutive lines are consecutive in memory inside a slab.
+template <typename T, bool INV>
+__global__ __launch_bounds__(kFftThreads) void k_lines(
+    Cx<T> *__restrict__ data, int Nf, int nlines, int inner, size_t slab,
+    size_t stride, FftPlan plan, const Cx<T> *__restrict__ tw)
+{
+    Cx<T> *lds = lds_tile<T>();
+    const int B = fft_elems<T>() / Nf;
+    const int line0 = blockIdx.x * B;
+    for (int e = threadIdx.x; e < Nf * B; e += kFftThreads) {
+        const int l = e % B, j = e / B;
+        const int L = line0 + l;
+        Cx<T> v = {0, 0};
+        if (L < nlines) v = data[(size_t)(L / inner) * slab + (L % inner) + j * stride];
+        lds[j * B + l] = v;
+    }
+    __syncthreads();
+    run_fft<T, false, INV>(lds, Nf, B, B, plan, tw, 1);
+    for (int e = threadIdx.x; e < Nf * B; e += kFftThreads) {
+        const int l = e % B, j = e / B;
+        const int L = line0 + l;
+        if (L < nlines) data[(size_t)(L / inner) * slab + (L % inner) + j * stride] = lds[j * B + l];
+    }
+}
+
+// x lines: forward FFT, Green's function, inverse FFT.  Line L = y*Zc + z.
+template <typename T>
+__global__ __launch_bounds__(kFftThreads) void k_lines_green(
+    Cx<T> *__restrict__ data, int M, FftPlan plan, const Cx<T> *__restrict__ tw,
+    const T *__restrict__ gexp, const T *__restrict__ gsinc)
+{
+    Cx<T> *lds = lds_tile<T>();
+    const int Nf = M, Zc = M / 2 + 1;
+    const int nlines = M * Zc;
+    const size_t stride = (size_t)M * Zc;
+    const int B = fft_elems<T>() / Nf;
+    const int line0 = blockIdx.x * B;
+    for (int e = threadIdx.x; e < Nf * B; e += kFftThreads) {
+        const int l = e % B, j = e / B;
+        const int L = line0 + l;
+        lds[j * B + l] = L < nlines ? data[(size_t)L + j * stride] : Cx<T>{0, 0};
+    }
+    __syncthreads();
+    run_fft<T, false, false>(lds, Nf, B, B, plan, tw, 1);
+    for (int e = threadIdx.x; e < Nf * B; e += kFftThreads) {
+        const int l = e % B, j = e / B;
+        const int L = line0 + l;
+        if (L >= nlines) continue;
+        const int y = L / Zc, z = L % Zc;
+        const int kx = j > M / 2 ? j - M : j;
+        const int ky = y > M / 2 ? y - M : y;
+        const int kz = z;
+        const int k2i = kx * kx + ky * ky + kz * kz;
+        T g = 0;
+        if (k2i > 0)
+            g = -(gexp[j] * gexp[y] * gexp[z]) * (gsinc[j] * gsinc[y] * gsinc[z]) / (T)k2i;
+        Cx<T> v = lds[j * B + l];
+        lds[j * B + l] = {v.x * g, v.y * g};
+    }
+    __syncthreads();
+    run_fft<T, false, true>(lds, Nf, B, B, plan, tw, 1);
+    for (int e = threadIdx.x; e < Nf * B; e += kFftThreads) {
+        const int l = e % B, j = e / B;
+        const int L = line0 + l;
+        if (L < nlines) data[(size_t)L + j * stride] = lds[j * B + l];
+    }
+}
+
+// ------------------------------------------------------------- deposit
+// Particle view: absolute FP64 position, or FP32 cell-relative position.
+struct View64 {
+    const double4 *p;
+    __device__ inline void weights(uint32_t j, const GridInfo &g, int, int, int,
+                                   double &dx, double &dy, double &dz,
+                                   double &m) const
+    {
+        const double4 q = p[j];
+        // pm_periodic.c:99-118: d = fac*x - slab, with the clamped slab.
+        dx = g.fac * q.x - cell_coord(g.fac, q.x, g.M);
+        dy = g.fac * q.y - cell_coord(g.fac, q.y, g.M);
+        dz = g.fac * q.z - cell_coord(g.fac, q.z, g.M);
+        m = q.w;
+    }
+};
+
+struct View32 {
+    const float4 *p;
+    float facf;
+    __device__ inline void weights(uint32_t j, const GridInfo &, int, int, int,
+                                   float &dx, float &dy, float &dz,
+                                   float &m) const
+    {
+        const float4 q = p[j];
+        dx = q.x * facf;
+        dy = q.y * facf;
+        dz = q.z * facf;
+        m = q.w;
+    }
+};
+
+template <typename T, typename V, typename ACC>
+__global__ __launch_bounds__(256) void k_deposit(V view, const uint32_t *__restrict__ off,
+                                                 GridInfo g, T *__restrict__ rho)
+{
+    const int M = g.M;
+    const size_t node = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    const size_t M3 = (size_t)M * M * M;
+    if (node >= M3) return;
+    const int z = (int)(node % M);
+    const int y = (int)((node / M) % M);
+    const int x = (int)(node / ((size_t)M * M));
+    ACC sum = 0;
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+        const int cx = wrap_index(x - a, M);
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            const int cy = wrap_index(y - b, M);
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                const int cz = wrap_index(z - c, M);
+                const uint32_t key = cell_key(g, cx, cy, cz);
+                const uint32_t s = off[key], e = off[key + 1];
+                for (uint32_t j = s; j < e; ++j) {
+                    T dx, dy, dz, m;
+                    view.weights(j, g, cx, cy, cz, dx, dy, dz, m);
+                    const T wx = a == 0 ? (T)1.0 - dx : dx;
+                    const T wy = b == 0 ? (T)1.0 - dy : dy;
+                    const T wz = c == 0 ? (T)1.0 - dz : dz;
+                    sum += (ACC)(m * wx * wy * wz);
+                }
+            }
+        }
+    }
+    rho[node] = (T)sum;
+}
+
+// ------------------------------------------------------------- interpolation
+template <typename T>
+__device__ inline T stencil(const T *__restrict__ phi, int M, int x, int y, int z,
+                            int dim, T fac)
+{
+    int xl = x, xr = x, xll = x, xrr = x, yl = y, yr = y, yll = y, yrr = y;
+    int zl = z, zr = z, zll = z, zrr = z;
+    if (dim == 0) { xr = wrap_index(x + 1, M); xrr = wrap_index(x + 2, M); xl = wrap_index(x - 1, M); xll = wrap_index(x - 2, M); }
+    if (dim == 1) { yr = wrap_index(y + 1, M); yrr = wrap_index(y + 2, M); yl = wrap_index(y - 1, M); yll = wrap_index(y - 2, M); }
+    if (dim == 2) { zr = wrap_index(z + 1, M); zrr = wrap_index(z + 2, M); zl = wrap_index(z - 1, M); zll = wrap_index(z - 2, M); }
+    const size_t MM = (size_t)M;
+    return fac * (((T)4.0 / (T)3) * (phi[((size_t)xl * MM + yl) * MM + zl] - phi[((size_t)xr * MM + yr) * MM + zr]) -
+                  ((T)1.0 / (T)6) * (phi[((size_t)xll * MM + yll) * MM + zll] - phi[((size_t)xrr * MM + yrr) * MM + zrr]));
+}
+
+template <typename T>
+__device__ inline T interp_dim(const T *__restrict__ phi, int M, int sx, int sxx,
+                               int sy, int syy, int sz, int szz, T dx, T dy,
+                               T dz, int dim, T fac)
+{
+    // pm_periodic.c:297-306, same term order.
+    T a = stencil(phi, M, sx, sy, sz, dim, fac) * ((T)1.0 - dx) * ((T)1.0 - dy) * ((T)1.0 - dz);
+    a += stencil(phi, M, sx, syy, sz, dim, fac) * ((T)1.0 - dx) * dy * ((T)1.0 - dz);
+    a += stencil(phi, M, sx, sy, szz, dim, fac) * ((T)1.0 - dx) * ((T)1.0 - dy) * dz;
+    a += stencil(phi, M, sx, syy, szz, dim, fac) * ((T)1.0 - dx) * dy * dz;
+    a += stencil(phi, M, sxx, sy, sz, dim, fac) * (dx) * ((T)1.0 - dy) * ((T)1.0 - dz);
+    a += stencil(phi, M, sxx, syy, sz, dim, fac) * (dx) * dy * ((T)1.0 - dz);
+    a += stencil(phi, M, sxx, sy, szz, dim, fac) * (dx) * ((T)1.0 - dy) * dz;
+    a += stencil(phi, M, sxx, syy, szz, dim, fac) * (dx) * dy * dz;
+    return a;
+}
+
+__global__ __launch_bounds__(256) void k_interp64(const double *__restrict__ phi,
+                                                  const double4 *__restrict__ spos,
+                                                  const uint32_t *__restrict__ perm,
+                                                  uint32_t n, GridInfo g, double fac,
+                                                  double4 *__restrict__ acc)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int M = g.M;
+    const double4 q = spos[i];
+    const int sx = cell_coord(g.fac, q.x, M), sy = cell_coord(g.fac, q.y, M),
+              sz = cell_coord(g.fac, q.z, M);
+    const double dx = g.fac * q.x - sx, dy = g.fac * q.y - sy, dz = g.fac * q.z - sz;
+    const int sxx = wrap_index(sx + 1, M), syy = wrap_index(sy + 1, M),
+              szz = wrap_index(sz + 1, M);
+    double4 a;
+    a.x = interp_dim(phi, M, sx, sxx, sy, syy, sz, szz, dx, dy, dz, 0, fac);
+    a.y = interp_dim(phi, M, sx, sxx, sy, syy, sz, szz, dx, dy, dz, 1, fac);
+    a.z = interp_dim(phi, M, sx, sxx, sy, syy, sz, szz, dx, dy, dz, 2, fac);
+    a.w = 0.0;
+    acc[perm[i]] = a;
+}
+
+__global__ __launch_bounds__(256) void k_interp32(const float *__restrict__ phi,
+                                                  const float4 *__restrict__ spos,
+                                                  const uint32_t *__restrict__ skeys,
+                                                  const uint32_t *__restrict__ perm,
+                                                  uint32_t n, GridInfo g, float facf,
+                                                  float fac, float4 *__restrict__ acc)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int M = g.M;
+    const float4 q = spos[i];
+    int sx, sy, sz;
+    key_cell(g, skeys[i], sx, sy, sz);
+    const float dx = q.x * facf, dy = q.y * facf, dz = q.z * facf;
+    const int sxx = wrap_index(sx + 1, M), syy = wrap_index(sy + 1, M),
+              szz = wrap_index(sz + 1, M);
+    float4 a;
+    a.x = interp_dim(phi, M, sx, sxx, sy, syy, sz, szz, dx, dy, dz, 0, fac);
+    a.y = interp_dim(phi, M, sx, sxx, sy, syy, sz, szz, dx, dy, dz, 1, fac);
+    a.z = interp_dim(phi, M, sx, sxx, sy, syy, sz, szz, dx, dy, dz, 2, fac);
+    a.w = 0.f;
+    acc[perm[i]] = a;
+}
+
+template <typename T>
+size_t lds_bytes() { return (size_t)kFftBytes; }
+
+template <typename T, typename K, typename... Args>
+int launch_fft_kernel(K kernel, unsigned grid, hipStream_t s, Args... args)
+{
+    kernel<<<grid, kFftThreads, lds_bytes<T>(), s>>>(args...);
+    CUCO_LAUNCH_CHECK();
+    return CUCO_OK;
+}
+
+}  // namespace
+
+bool fft_size_supported(int M)
+{
+    if (M < 4 || M % 2 != 0 || M > fft_elems<double>()) return false;
+    return make_plan(M).nstages >= 0 && make_plan(M / 2).nstages >= 0;
+}
+
+template <typename T>
+int pm_deposit(const void *spos, const uint32_t *off, uint32_t n,
+               const GridInfo &g, T *rho, hipStream_t s)
+{
+    const size_t M3 = (size_t)g.M * g.M * g.M;
+    const unsigned grid = (unsigned)((M3 + 255) / 256);
+    (void)n;
+    if (sizeof(T) == 8) {
+        View64 v{static_cast<const double4 *>(spos)};
+        k_deposit<double, View64, double><<<grid, 256, 0, s>>>(
+            v, off, g, reinterpret_cast<double *>(rho));
+    } else {
+        View32 v{static_cast<const float4 *>(spos), (float)g.fac};
+        k_deposit<float, View32, double><<<grid, 256, 0, s>>>(
+            v, off, g, reinterpret_cast<float *>(rho));
+    }
+    CUCO_LAUNCH_CHECK();
+    return CUCO_OK;
+}
+
+template <typename T>
+int pm_fft_forward(const T *rho, Cx<T> *c, int M, const PmTables &tb,
+                   hipStream_t s)
+{
+    const FftPlan ph = make_plan(M / 2), pf = make_plan(M);
+    const Cx<T> *tw = static_cast<const Cx<T> *>(tb.tw);
+    const int nrows = M * M, Zc = M / 2 + 1;
+    const int Brow = fft_elems<T>() / (M / 2);
+    int rc = launch_fft_kernel<T>(k_rows_r2c<T>, (unsigned)((nrows + Brow - 1) / Brow),
+                                  s, rho, c, M, nrows, ph, tw);
+    if (rc) return rc;
+    // y lines: L = x*Zc + z, slab M*Zc, stride Zc
+    const int nlines = M * Zc, B = fft_elems<T>() / M;
+    return launch_fft_kernel<T>(k_lines<T, false>, (unsigned)((nlines + B - 1) / B), s,
+                                c, M, nlines, Zc, (size_t)M * Zc, (size_t)Zc, pf, tw);
+}
+
+template <typename T>
+int pm_green_x(Cx<T> *c, int M, const PmTables &tb, hipStream_t s)
+{
+    const FftPlan pf = make_plan(M);
+    const int nlines = M * (M / 2 + 1), B = fft_elems<T>() / M;
+    return launch_fft_kernel<T>(k_lines_green<T>, (unsigned)((nlines + B - 1) / B), s,
+                                c, M, pf, static_cast<const Cx<T> *>(tb.tw),
+                                static_cast<const T *>(tb.gexp),
+                                static_cast<const T *>(tb.gsinc));
+}
+
+template <typename T>
+int pm_fft_inverse(Cx<T> *c, T *phi, int M, const PmTables &tb, hipStream_t s)
+{
+    const FftPlan ph = make_plan(M / 2), pf = make_plan(M);
+    const Cx<T> *tw = static_cast<const Cx<T> *>(tb.tw);
+    const int Zc = M / 2 + 1, nlines = M * Zc, B = fft_elems<T>() / M;
+    int rc = launch_fft_kernel<T>(k_lines<T, true>, (unsigned)((nlines + B - 1) / B), s,
+                                  c, M, nlines, Zc, (size_t)M * Zc, (size_t)Zc, pf, tw);
+    if (rc) return rc;
+    const int nrows = M * M, Brow = fft_elems<T>() / (M / 2);
+    return launch_fft_kernel<T>(k_rows_c2r<T>, (unsigned)((nrows + Brow - 1) / Brow), s,
+                                (const Cx<T> *)c, phi, M, nrows, ph, tw);
+}
+
+template <typename T>
+int pm_interp(const T *phi, const void *spos, const uint32_t *skeys,
+              const uint32_t *perm, uint32_t n, const GridInfo &g, double fac,
+              void *acc, hipStream_t s)
+{
+    if (n == 0) return CUCO_OK;
+    if (sizeof(T) == 8) {
+        k_interp64<<<grid_for(n, 256), 256, 0, s>>>(
+            reinterpret_cast<const double *>(phi), static_cast<const double4 *>(spos),
+            perm, n, g, fac, static_cast<double4 *>(acc));
+    } else {
+        k_interp32<<<grid_for(n, 256), 256, 0, s>>>(
+            reinterpret_cast<const float *>(phi), static_cast<const float4 *>(spos),
+            skeys, perm, n, g, (float)g.fac, (float)fac, static_cast<float4 *>(acc));
+    }
+    CUCO_LAUNCH_CHECK();
+    return CUCO_OK;
+}
+
+template <typename T>
+int pm_fft_x(Cx<T> *c, int M, bool inverse, hipStream_t s, const PmTables &tb)
+{
+    const FftPlan pf = make_plan(M);
+    const Cx<T> *tw = static_cast<const Cx<T> *>(tb.tw);
+    const int Zc = M / 2 + 1, nlines = M * Zc, B = fft_elems<T>() / M;
+    const size_t stride = (size_t)M * Zc;
+    if (inverse)
+        return launch_fft_kernel<T>(k_lines<T, true>, (unsigned)((nlines + B - 1) / B), s,
+                                    c, M, nlines, nlines, (size_t)0, stride, pf, tw);
+    return launch_fft_kernel<T>(k_lines<T, false>, (unsigned)((nlines + B - 1) / B), s,
+                                c, M, nlines, nlines, (size_t)0, stride, pf, tw);
+}
+
+template int pm_fft_x<double>(Cx<double> *, int, bool, hipStream_t, const PmTables &);
+template int pm_fft_x<float>(Cx<float> *, int, bool, hipStream_t, const PmTables &);
+template int pm_deposit<double>(const void *, const uint32_t *, uint32_t,
+                                const GridInfo &, double *, hipStream_t);
+template int pm_deposit<float>(const void *, const uint32_t *, uint32_t,
+                               const GridInfo &, float *, hipStream_t);
+template int pm_fft_forward<double>(const double *, Cx<double> *, int,
+                                    const PmTables &, hipStream_t);
+template int pm_fft_forward<float>(const float *, Cx<float> *, int,
+                                   const PmTables &, hipStream_t);
+template int pm_green_x<double>(Cx<double> *, int, const PmTables &, hipStream_t);
+template int pm_green_x<float>(Cx<float> *, int, const PmTables &, hipStream_t);
+template int pm_fft_inverse<double>(Cx<double> *, double *, int, const PmTables &,
+                                    hipStream_t);
+template int pm_fft_inverse<float>(Cx<float> *, float *, int, const PmTables &,
+                                   hipStream_t);
+template int pm_interp<double>(const double *, const void *, const uint32_t *,
+                               const uint32_t *, uint32_t, const GridInfo &,
+                               double, void *, hipStream_t);
+template int pm_interp<float>(const float *, const void *, const uint32_t *,
+                              const uint32_t *, uint32_t, const GridInfo &,
+                              double, void *, hipStream_t);
+
+}  // namespace cuco

@@ -1,0 +1,436 @@
+// FP32 short-range pair force -- the chosen path.
+//
+// Same pair law as the reference (gravlist_kernel.cu:20-88 / gravlist.cpp:
+// 279-327: minimum image via cell offsets, cut at rcut, spline softening
+// inside h, erfc table), re-organised for CDNA4:
+//
+//  * Particles are sorted by a Morton cell key, so consecutive particles are
+//    spatially compact at every density.  A tile is <= 64 consecutive
+//    particles inside one Morton block of 2^s cells per dimension; one
+//    wavefront owns one tile, one lane one target particle.
+//  * Positions are FP32 relative to their cell corner (the subtraction is
+//    done in FP64 by the reorder kernel).  A tile works in a frame anchored
+//    at the corner of its first particle's cell: every coordinate is the
+//    cell-relative value plus an integer number of cell sizes, so pair
+//    separations are small numbers with small absolute error.
+//  * For each tile the wave enumerates the cells within rcut of the tile's
+//    bounding box (64 cells per step), reads their ranges from the CSR cell
+//    table, and streams the j particles through a filter (distance to the
+//    bounding box <= rcut) into an LDS buffer.  Each 64-entry buffer is
+//    broadcast to all lanes and evaluated with the erfc table in LDS.
+//  * Each lane sums a 64-j chunk in plain FP32 and adds the chunk sum into
+//    its total with Kahan compensation.
+//  * A persistent grid pulls tiles from an atomic work counter, so dense
+//    halo tiles and void tiles balance across the chip.
+#include "internal.h"
+
+namespace cuco {
+
+namespace {
+
+constexpr int kWaves = 4;
+constexpr int kThreads = 64 * kWaves;
+constexpr int kRanges = 256;   // cell ranges per wave before the j stream
+constexpr int kJBuf = 128;     // staged j particles per wave
+
+struct P32 {
+    float rcut, rcut2, filter2;
+    float asmthfac, h, h_inv, h3_inv, G;
+    float cellf, inv_cellf;
+    int M;
+};
+
+__device__ inline void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ inline float wave_min(float v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+__device__ inline float wave_max(float v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Wave-uniform copies in scalar registers.  Every loop bound and branch
+// condition of the tile loop goes through these, so the compiler emits
+// uniform (scalar) control flow: a lane-divergent view of an in-fact
+// uniform loop exit lets it split the loop into per-lane nests around the
+// cross-lane operations, which then read lanes that are not active.
+__device__ inline uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ inline int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ inline float uni(float v)
+{
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+
+__device__ inline uint32_t wave_incl_scan(uint32_t v)
+{
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t t = __shfl_up(v, o, 64);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+
+__device__ inline uint32_t pack_q(int qx, int qy, int qz)
+{
+    return ((uint32_t)(qx + 512) << 20) | ((uint32_t)(qy + 512) << 10) |
+           (uint32_t)(qz + 512);
+}
+
+__device__ inline void unpack_q(uint32_t p, int &qx, int &qy, int &qz)
+{
+    qx = (int)(p >> 20) - 512;
+    qy = (int)((p >> 10) & 1023u) - 512;
+    qz = (int)(p & 1023u) - 512;
+}
+
+__device__ inline int unwrap_delta(int d, int M)
+{
+    if (d > M / 2) d -= M;
+    if (d < -M / 2) d += M;
+    return d;
+}
+
+// Pair sum of one lane against cnt staged j particles.
+template <bool MASK>
+__device__ inline void interact(const float4 *__restrict__ sj, int cnt,
+                                const P32 &pp, const float *__restrict__ tab,
+                                float xi, float yi, float zi, float &sx,
+                                float &sy, float &sz)
+{
+#pragma unroll 4
+    for (int k = 0; k < cnt; ++k) {
+        const float4 q = sj[k];
+        const float dx = q.x - xi;
+        const float dy = q.y - yi;
+        const float dz = q.z - zi;
+        float r2 = dx * dx + dy * dy + dz * dz;
+        const bool inside = r2 <= pp.rcut2;
+        // Self and coincident pairs: dx = dy = dz = 0, so the clamp only keeps
+        // rsq finite and the term is exactly zero (gravlist_kernel.cu:64-69).
+        r2 = fmaxf(r2, 1e-20f);
+        const float rinv = __builtin_amdgcn_rsqf(r2);
+        const float r = r2 * rinv;
+        const float rinv3 = rinv * rinv * rinv;
+        float fac = q.w * rinv3;
+        if (r < pp.h) {
+            const float u = r * pp.h_inv;
+            if (u < 0.5f)
+                fac = q.w * pp.h3_inv * (10.66667f + u * u * (32.0f * u - 38.4f));
+            else
+                fac = q.w * pp.h3_inv *
+                          (21.33333f - 48.0f * u + 38.4f * u * u -
+                           10.66667f * u * u * u) -
+                      0.06667f * fac;
+        }
+        const int idx = (int)fminf(pp.asmthfac * r, (float)(kTabPadded - 1));
+        float t = tab[idx];
+        if (MASK) t = inside ? t : 0.0f;
+        fac *= t;
+        sx += dx * fac;
+        sy += dy * fac;
+        sz += dz * fac;
+    }
+}
+
+__device__ inline void kahan_add(float &sum, float &comp, float v)
+{
+    const float y = v - comp;
+    const float t = sum + y;
+    comp = (t - sum) - y;
+    sum = t;
+}
+
+template <bool MASK, bool COUNT>
+__global__ __launch_bounds__(kThreads) void k_pp32(
+    const float4 *__restrict__ spos, const uint32_t *__restrict__ skeys,
+    const uint32_t *__restrict__ perm, const uint32_t *__restrict__ off,
+    GridInfo g, P32 pp, const float *__restrict__ table32,
+    const uint32_t *__restrict__ tstart, const uint32_t *__restrict__ tcount,
+    uint32_t *__restrict__ counters, float4 *__restrict__ acc,
+    unsigned long long *__restrict__ cand)
+{
+    __shared__ float s_tab[kTabPadded];
+    __shared__ uint32_t s_rs[kWaves][kRanges];
+    __shared__ uint32_t s_rp[kWaves][kRanges + 1];
+    __shared__ uint32_t s_rq[kWaves][kRanges];
+    __shared__ float4 s_j[kWaves][kJBuf];
+
+    for (int k = threadIdx.x; k < kTabPadded; k += kThreads) s_tab[k] = table32[k];
+    __syncthreads();
+
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t *rs = s_rs[w];
+    uint32_t *rp = s_rp[w];
+    uint32_t *rq = s_rq[w];
+    float4 *sj = s_j[w];
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const uint32_t ntiles = counters[0];
+    const int M = pp.M;
+    const float inf = __builtin_huge_valf();
+
+    for (;;) {
+        // Every lane takes part in the atomic (lane 0 adds 1, the rest 0), so
+        // the loop header carries no lane-divergent branch; lane 0's return
+        // value is the tile index whatever order the adds are applied in.
+        uint32_t t = atomicAdd(&counters[1], lane == 0 ? 1u : 0u);
+        t = __builtin_amdgcn_readlane(t, 0);
+        if (t >= ntiles) break;
+
+        const uint32_t i0 = tstart[t];
+        const uint32_t cnt = tcount[t];
+        const bool valid = (uint32_t)lane < cnt;
+        const uint32_t i = i0 + (valid ? (uint32_t)lane : 0u);
+
+        int c0x, c0y, c0z, cx, cy, cz;
+        key_cell(g, skeys[i0], c0x, c0y, c0z);
+        key_cell(g, skeys[i], cx, cy, cz);
+        const float4 pi = spos[i];
+        const float xi = pi.x + (float)unwrap_delta(cx - c0x, M) * pp.cellf;
+        const float yi = pi.y + (float)unwrap_delta(cy - c0y, M) * pp.cellf;
+        const float zi = pi.z + (float)unwrap_delta(cz - c0z, M) * pp.cellf;
+
+        const float bminx = uni(wave_min(valid ? xi : inf));
+        const float bminy = uni(wave_min(valid ? yi : inf));
+        const float bminz = uni(wave_min(valid ? zi : inf));
+        const float bmaxx = uni(wave_max(valid ? xi : -inf));
+        const float bmaxy = uni(wave_max(valid ? yi : -inf));
+        const float bmaxz = uni(wave_max(valid ? zi : -inf));
+
+        const int lqx = (int)floorf((bminx - pp.rcut) * pp.inv_cellf - 0.01f);
+        const int lqy = (int)floorf((bminy - pp.rcut) * pp.inv_cellf - 0.01f);
+        const int lqz = (int)floorf((bminz - pp.rcut) * pp.inv_cellf - 0.01f);
+        const int nqx = (int)floorf((bmaxx + pp.rcut) * pp.inv_cellf + 0.01f) - lqx + 1;
+        const int nqy = (int)floorf((bmaxy + pp.rcut) * pp.inv_cellf + 0.01f) - lqy + 1;
+        const int nqz = (int)floorf((bmaxz + pp.rcut) * pp.inv_cellf + 0.01f) - lqz + 1;
+        // A tile spans at most one Morton block plus the cut on each side;
+        // anything larger means non-finite input and the tile is skipped.
+        const bool sane = nqx > 0 && nqy > 0 && nqz > 0 && nqx <= 2 * M &&
+                          nqy <= 2 * M && nqz <= 2 * M;
+        const uint32_t total_q = uni(sane ? (uint32_t)nqx * nqy * nqz : 0u);
+
+        float ax = 0.f, ay = 0.f, az = 0.f, kx = 0.f, ky = 0.f, kz = 0.f;
+        uint32_t jc = 0;
+        uint32_t cursor = 0;
+        unsigned long long ncand = 0;
+
+        while (cursor < total_q) {
+            // Enumerate cells into the range list.
+            uint32_t nr = 0;
+            while (cursor < total_q && nr + 64 <= (uint32_t)kRanges) {
+                const uint32_t q = cursor + lane;
+                bool ok = q < total_q;
+                uint32_t s = 0, c = 0, packed = 0;
+                if (ok) {
+                    const int qz = (int)(q % (uint32_t)nqz) + lqz;
+                    const uint32_t tq = q / (uint32_t)nqz;
+                    const int qy = (int)(tq % (uint32_t)nqy) + lqy;
+                    const int qx = (int)(tq / (uint32_t)nqy) + lqx;
+                    const float gx = fmaxf(0.f, fmaxf(qx * pp.cellf - bmaxx, bminx - (qx + 1) * pp.cellf));
+                    const float gy = fmaxf(0.f, fmaxf(qy * pp.cellf - bmaxy, bminy - (qy + 1) * pp.cellf));
+                    const float gz = fmaxf(0.f, fmaxf(qz * pp.cellf - bmaxz, bminz - (qz + 1) * pp.cellf));
+                    ok = gx * gx + gy * gy + gz * gz <= pp.filter2;
+                    if (ok) {
+                        int ux = (c0x + qx) % M, uy = (c0y + qy) % M, uz = (c0z + qz) % M;
+                        ux += ux < 0 ? M : 0;
+                        uy += uy < 0 ? M : 0;
+                        uz += uz < 0 ? M : 0;
+                        const uint32_t key = cell_key(g, ux, uy, uz);
+                        s = off[key];
+                        c = off[key + 1] - s;
+                        ok = c > 0;
+                        packed = pack_q(qx, qy, qz);
+                    }
+                }
+                const uint64_t bal = __ballot(ok);
+                if (ok) {
+                    const uint32_t pos = nr + (uint32_t)__popcll(bal & lt);
+                    rs[pos] = s;
+                    rp[pos] = c;
+                    rq[pos] = packed;
+                }
+                nr = uni(nr + (uint32_t)__popcll(bal));
+                cursor += 64;
+            }
+            wave_sync();
+            // Exclusive prefix of the range lengths.
+            uint32_t carry = 0;
+            for (uint32_t b0 = 0; b0 < nr; b0 += 64) {
+                const uint32_t idx = b0 + lane;
+                const uint32_t v = idx < nr ? rp[idx] : 0u;
+                const uint32_t inc = wave_incl_scan(v);
+                if (idx < nr) rp[idx] = carry + inc - v;
+                carry = uni(carry + (uint32_t)__builtin_amdgcn_readlane(inc, 63));
+            }
+            if (lane == 0) rp[nr] = carry;
+            wave_sync();
+            const uint32_t total_j = carry;
+
+            // Stream, filter and evaluate.
+            for (uint32_t f0 = 0; f0 < total_j; f0 += 64) {
+                const uint32_t f = f0 + lane;
+                bool ok = f < total_j;
+                float4 qj = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (ok) {
+                    uint32_t lo = 0, hi = nr;
+                    while (hi - lo > 1) {
+                        const uint32_t mid = (lo + hi) >> 1;
+                        if (rp[mid] <= f) lo = mid; else hi = mid;
+                    }
+                    const uint32_t j = rs[lo] + (f - rp[lo]);
+                    int qx, qy, qz;
+                    unpack_q(rq[lo], qx, qy, qz);
+                    const float4 pj = spos[j];
+                    qj = make_float4(pj.x + qx * pp.cellf, pj.y + qy * pp.cellf,
+                                     pj.z + qz * pp.cellf, pj.w);
+                    const float gx = fmaxf(0.f, fmaxf(bminx - qj.x, qj.x - bmaxx));
+                    const float gy = fmaxf(0.f, fmaxf(bminy - qj.y, qj.y - bmaxy));
+                    const float gz = fmaxf(0.f, fmaxf(bminz - qj.z, qj.z - bmaxz));
+                    ok = gx * gx + gy * gy + gz * gz <= pp.filter2;
+                }
+                const uint64_t bal = __ballot(ok);
+                if (ok) sj[jc + (uint32_t)__popcll(bal & lt)] = qj;
+                jc = uni(jc + (uint32_t)__popcll(bal));
+                wave_sync();
+                if (jc >= 64) {
+                    float sx = 0.f, sy = 0.f, sz = 0.f;
+                    interact<MASK>(sj, 64, pp, s_tab, xi, yi, zi, sx, sy, sz);
+                    kahan_add(ax, kx, sx);
+                    kahan_add(ay, ky, sy);
+                    kahan_add(az, kz, sz);
+                    if (COUNT) ncand += 64ull * cnt;
+                    const uint32_t rem = jc - 64;
+                    float4 tmp = make_float4(0.f, 0.f, 0.f, 0.f);
+                    if ((uint32_t)lane < rem) tmp = sj[64 + lane];
+                    wave_sync();
+                    if ((uint32_t)lane < rem) sj[lane] = tmp;
+                    jc = rem;
+                    wave_sync();
+                }
+            }
+        }
+        if (jc > 0) {
+            float sx = 0.f, sy = 0.f, sz = 0.f;
+            interact<MASK>(sj, (int)jc, pp, s_tab, xi, yi, zi, sx, sy, sz);
+            kahan_add(ax, kx, sx);
+            kahan_add(ay, ky, sy);
+            kahan_add(az, kz, sz);
+            if (COUNT) ncand += (unsigned long long)jc * cnt;
+            jc = 0;
+        }
+        wave_sync();
+        if (valid) acc[perm[i]] = make_float4(ax * pp.G, ay * pp.G, az * pp.G, 0.f);
+        if (COUNT) atomicAdd(cand, lane == 0 ? ncand : 0ull);
+    }
+}
+
+__global__ void k_block_tiles(const uint32_t *__restrict__ off, uint32_t nblocks,
+                              int shift3, uint32_t *__restrict__ block_tiles)
+{
+    uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nblocks) return;
+    uint32_t s = off[(size_t)b << shift3];
+    uint32_t e = off[(size_t)(b + 1) << shift3];
+    block_tiles[b] = (e - s + 63u) / 64u;
+}
+
+__global__ void k_fill_tiles(const uint32_t *__restrict__ off,
+                             const uint32_t *__restrict__ tile_off,
+                             uint32_t nblocks, int shift3,
+                             uint32_t *__restrict__ tstart,
+                             uint32_t *__restrict__ tcount)
+{
+    uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nblocks) return;
+    uint32_t s = off[(size_t)b << shift3];
+    uint32_t e = off[(size_t)(b + 1) << shift3];
+    uint32_t t = tile_off[b];
+    for (uint32_t p = s; p < e; p += 64u, ++t) {
+        tstart[t] = p;
+        tcount[t] = min(64u, e - p);
+    }
+}
+
+}  // namespace
+
+int launch_build_tiles(const uint32_t *off, uint32_t n, const GridInfo &g,
+                       TileInfo &t, hipStream_t s)
+{
+    const int shift3 = 3 * t.block_shift;
+    const uint32_t nblocks = g.nkeys >> shift3;
+    if (nblocks == 0 || nblocks > t.nblocks) {
+        set_error("tile build: %u Morton blocks exceed capacity %u", nblocks,
+                  t.nblocks);
+        return CUCO_ERR_INVALID_ARG;
+    }
+    k_block_tiles<<<grid_for(nblocks, 256), 256, 0, s>>>(off, nblocks, shift3,
+                                                         t.block_tiles);
+    CUCO_LAUNCH_CHECK();
+    int rc = exclusive_scan_u32(t.block_tiles, t.block_tiles, nblocks,
+                                t.scan_scratch, t.counters, s);
+    if (rc) return rc;
+    k_fill_tiles<<<grid_for(nblocks, 256), 256, 0, s>>>(off, t.block_tiles,
+                                                        nblocks, shift3,
+                                                        t.tile_start,
+                                                        t.tile_count);
+    CUCO_LAUNCH_CHECK();
+    (void)n;
+    return CUCO_OK;
+}
+
+int launch_pp32(const float4 *spos, const uint32_t *skeys,
+                const uint32_t *perm, const uint32_t *off, uint32_t n,
+                const GridInfo &g, const PairParams &pp, const float *table32,
+                const TileInfo &t, float4 *acc,
+                unsigned long long *cand_counter, int num_workgroups,
+                hipStream_t s)
+{
+    if (n == 0) return CUCO_OK;
+    P32 p;
+    p.rcut = (float)pp.rcut;
+    p.rcut2 = (float)pp.rcut2;
+    p.filter2 = (float)(pp.rcut2 * (1.0 + 1e-4));
+    p.asmthfac = (float)pp.asmthfac;
+    p.h = (float)pp.h;
+    p.h_inv = (float)pp.h_inv;
+    p.h3_inv = (float)pp.h3_inv;
+    p.G = (float)pp.G;
+    p.cellf = (float)g.cell;
+    p.inv_cellf = (float)(1.0 / g.cell);
+    p.M = g.M;
+    // The cut can ride on the table when rcut sits on a table bin edge (the
+    // reference's RCUT/ASMTH = 4.5 puts it on bin 750).
+    const double edge = pp.asmthfac * pp.rcut;
+    const bool aligned = fabs(edge - nearbyint(edge)) < 1e-9 * edge;
+    int rc = launch_fill_u32(t.counters + 1, 0u, 1, s);
+    if (rc) return rc;
+    const int nwg = num_workgroups > 0 ? num_workgroups : 2048;
+#define CUCO_PP32(MASK, COUNT)                                                 \
+    k_pp32<MASK, COUNT><<<nwg, kThreads, 0, s>>>(spos, skeys, perm, off, g, p, \
+                                                 table32, t.tile_start,        \
+                                                 t.tile_count, t.counters, acc, \
+                                                 cand_counter)
+    if (aligned) {
+        if (cand_counter) CUCO_PP32(false, true); else CUCO_PP32(false, false);
+    } else {
+        if (cand_counter) CUCO_PP32(true, true); else CUCO_PP32(true, false);
+    }
+#undef CUCO_PP32
+    CUCO_LAUNCH_CHECK();
+    return CUCO_OK;
+}
+
+}  // namespace cuco

@@ -1,0 +1,423 @@
+// Cell keys, stable LSD radix sort, CSR cell table and particle reorder.
+//
+// Reference behaviour (liulei/cuco):
+//   key        gravlist_kernel.cu:92-118  (here clamped to M-1, any N)
+//   sort       radixsort.h:170-187, radixsort.cu:1048-1145 -- a stable LSD
+//              key/value sort; the permutation is the unique stable order.
+//   cell table gravlist_kernel.cu:121-160 (cellStart/cellEnd); this engine
+//              keeps CSR offsets (one array, no memset) and derives the
+//              reference's start/end arrays on request.
+//
+// Sort design for gfx950: 8-bit digits, reduce-then-scan (no inter-block
+// spin waits).  Per pass: an upsweep histogram per 4096-key tile, a per-digit
+// scan over tiles, and a downsweep that ranks every key inside its tile with
+// a wave-level multisplit (eight 64-lane ballots), reorders the tile through
+// LDS by digit, and writes each digit run contiguously.
+#include "internal.h"
+
+namespace cuco {
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kItems = kSortTile / kThreads;  // 16
+constexpr int kBins = 256;
+
+__device__ inline uint32_t wave_incl_scan(uint32_t v)
+{
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t t = __shfl_up(v, o, 64);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+
+// Exclusive scan over a 256-thread block; sh must hold 4 words.
+__device__ inline uint32_t block_excl_scan256(uint32_t v, uint32_t *sh,
+                                              uint32_t &total)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t inc = wave_incl_scan(v);
+    if (lane == 63) sh[wave] = inc;
+    __syncthreads();
+    uint32_t off = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+        uint32_t t = sh[w];
+        off += (w < wave) ? t : 0u;
+        tot += t;
+    }
+    __syncthreads();
+    total = tot;
+    return off + inc - v;
+}
+
+__global__ void k_calc_keys(const double4 *__restrict__ pos, uint32_t n,
+                            GridInfo g, uint32_t *__restrict__ keys)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double4 p = pos[i];
+    int ix = cell_coord(g.fac, p.x, g.M);
+    int iy = cell_coord(g.fac, p.y, g.M);
+    int iz = cell_coord(g.fac, p.z, g.M);
+    keys[i] = cell_key(g, ix, iy, iz);
+}
+
+__global__ __launch_bounds__(kThreads) void k_rs_upsweep(
+    const uint32_t *__restrict__ keys, uint32_t n, int shift, uint32_t nblocks,
+    uint32_t *__restrict__ counts)
+{
+    __shared__ uint32_t hist[4][kBins];
+    const int tid = threadIdx.x, wave = tid >> 6;
+#pragma unroll
+    for (int w = 0; w < 4; ++w) hist[w][tid] = 0;
+    __syncthreads();
+    const uint32_t base = blockIdx.x * (uint32_t)kSortTile;
+#pragma unroll 4
+    for (int r = 0; r < kItems; ++r) {
+        uint32_t e = base + r * kThreads + tid;
+        if (e < n) atomicAdd(&hist[wave][(keys[e] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    counts[(size_t)tid * nblocks + blockIdx.x] =
+        hist[0][tid] + hist[1][tid] + hist[2][tid] + hist[3][tid];
+}
+
+// One block per digit: exclusive scan of that digit's counters over tiles.
+__global__ __launch_bounds__(kThreads) void k_rs_scan_digits(
+    uint32_t *__restrict__ counts, uint32_t nblocks, uint32_t *__restrict__ totals)
+{
+    __shared__ uint32_t sh[4];
+    uint32_t *c = counts + (size_t)blockIdx.x * nblocks;
+    uint32_t carry = 0;
+    for (uint32_t b0 = 0; b0 < nblocks; b0 += kThreads) {
+        uint32_t b = b0 + threadIdx.x;
+        uint32_t v = b < nblocks ? c[b] : 0u;
+        uint32_t tot;
+        uint32_t ex = block_excl_scan256(v, sh, tot);
+        if (b < nblocks) c[b] = ex + carry;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) totals[blockIdx.x] = carry;
+}
+
+__global__ __launch_bounds__(kThreads) void k_rs_downsweep(
+    const uint32_t *__restrict__ keys_in, const uint32_t *__restrict__ vals_in,
+    uint32_t *__restrict__ keys_out, uint32_t *__restrict__ vals_out,
+    uint32_t n, int shift, uint32_t nblocks, const uint32_t *__restrict__ counts,
+    const uint32_t *__restrict__ totals)
+{
+    __shared__ uint32_t s_keys[kSortTile];
+    __shared__ uint32_t s_vals[kSortTile];
+    __shared__ uint32_t s_wcnt[4][kBins];
+    __shared__ uint32_t s_run[kBins];
+    __shared__ uint32_t s_tileoff[kBins];
+    __shared__ uint32_t s_gbase[kBins];
+    __shared__ uint32_t s_scan[4];
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint32_t base = blockIdx.x * (uint32_t)kSortTile;
+
+    uint32_t tot;
+    uint32_t dbase = block_excl_scan256(totals[tid], s_scan, tot);
+    s_gbase[tid] = dbase + counts[(size_t)tid * nblocks + blockIdx.x];
+    s_run[tid] = 0;
+#pragma unroll
+    for (int w = 0; w < 4; ++w) s_wcnt[w][tid] = 0;
+    __syncthreads();
+
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    uint32_t my_key[kItems], my_val[kItems], my_rank[kItems];
+
+#pragma unroll
+    for (int r = 0; r < kItems; ++r) {
+        const uint32_t e = base + r * kThreads + tid;
+        const bool valid = e < n;
+        uint32_t k = valid ? keys_in[e] : 0u;
+        uint32_t v = valid ? (vals_in ? vals_in[e] : e) : 0u;
+        uint32_t d = (k >> shift) & 255u;
+        uint64_t m = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const bool bit = (d >> b) & 1u;
+            uint64_t bb = __ballot(bit);
+            m &= bit ? bb : ~bb;
+        }
+        const uint32_t rank_w = (uint32_t)__popcll(m & lt);
+        if (valid && rank_w == 0) s_wcnt[wave][d] = (uint32_t)__popcll(m);
+        __syncthreads();
+        uint32_t pre = s_run[d];
+        for (int w = 0; w < wave; ++w) pre += s_wcnt[w][d];
+        my_rank[r] = pre + rank_w;
+        my_key[r] = k;
+        my_val[r] = v;
+        __syncthreads();
+        s_run[tid] += s_wcnt[0][tid] + s_wcnt[1][tid] + s_wcnt[2][tid] +
+                      s_wcnt[3][tid];
+#pragma unroll
+        for (int w = 0; w < 4; ++w) s_wcnt[w][tid] = 0;
+        __syncthreads();
+    }
+
+    uint32_t tt;
+    s_tileoff[tid] = block_excl_scan256(s_run[tid], s_scan, tt);
+    __syncthreads();
+
+#pragma unroll
+    for (int r = 0; r < kItems; ++r) {
+        const uint32_t e = base + r * kThreads + tid;
+        if (e < n) {
+            uint32_t d = (my_key[r] >> shift) & 255u;
+            uint32_t lp = s_tileoff[d] + my_rank[r];
+            s_keys[lp] = my_key[r];
+            s_vals[lp] = my_val[r];
+        }
+    }
+    __syncthreads();
+
+    const uint32_t tile_n = min((uint32_t)kSortTile, n - base);
+    for (uint32_t lp = tid; lp < tile_n; lp += kThreads) {
+        uint32_t k = s_keys[lp];
+        uint32_t d = (k >> shift) & 255u;
+        uint32_t gp = s_gbase[d] + (lp - s_tileoff[d]);
+        keys_out[gp] = k;
+        if (vals_out) vals_out[gp] = s_vals[lp];
+    }
+}
+
+__global__ void k_cell_offsets(const uint32_t *__restrict__ skeys, uint32_t n,
+                               uint32_t nkeys, uint32_t *__restrict__ off)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > n) return;
+    int64_t prev = (i == 0) ? -1 : (int64_t)skeys[i - 1];
+    int64_t cur = (i == n) ? (int64_t)nkeys : (int64_t)skeys[i];
+    for (int64_t c = prev + 1; c <= cur; ++c) off[c] = i;
+}
+
+__global__ void k_cell_start_end(const uint32_t *__restrict__ off,
+                                 uint32_t ncells, uint32_t *__restrict__ start,
+                                 uint32_t *__restrict__ end)
+{
+    uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncells) return;
+    uint32_t a = off[c], b = off[c + 1];
+    start[c] = b > a ? a : 0xffffffffu;
+    end[c] = b > a ? b : 0xffffffffu;
+}
+
+__global__ void k_reorder64(const double4 *__restrict__ pos,
+                            const uint32_t *__restrict__ perm, uint32_t n,
+                            double4 *__restrict__ spos)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    spos[i] = pos[perm[i]];
+}
+
+// Cell-relative FP32 coordinates: x - ix*cell computed in FP64, then rounded.
+__global__ void k_reorder32(const double4 *__restrict__ pos,
+                            const uint32_t *__restrict__ perm,
+                            const uint32_t *__restrict__ skeys, uint32_t n,
+                            GridInfo g, float4 *__restrict__ spos)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double4 p = pos[perm[i]];
+    int ix, iy, iz;
+    key_cell(g, skeys[i], ix, iy, iz);
+    spos[i] = make_float4((float)(p.x - ix * g.cell), (float)(p.y - iy * g.cell),
+                          (float)(p.z - iz * g.cell), (float)p.w);
+}
+
+// ---- generic exclusive scan (reduce / scan block sums / add)
+constexpr int kScanTile = 1024;  // 256 threads x 4
+
+__global__ __launch_bounds__(kThreads) void k_scan_reduce(
+    const uint32_t *__restrict__ in, uint32_t n, uint32_t *__restrict__ sums)
+{
+    __shared__ uint32_t sh[4];
+    uint32_t base = blockIdx.x * kScanTile + threadIdx.x * 4;
+    uint32_t v = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (base + k < n) v += in[base + k];
+    uint32_t tot;
+    block_excl_scan256(v, sh, tot);
+    if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(kThreads) void k_scan_sums(uint32_t *__restrict__ sums,
+                                                        uint32_t nb,
+                                                        uint32_t *__restrict__ total)
+{
+    __shared__ uint32_t sh[4];
+    uint32_t carry = 0;
+    for (uint32_t b0 = 0; b0 < nb; b0 += kThreads) {
+        uint32_t b = b0 + threadIdx.x;
+        uint32_t v = b < nb ? sums[b] : 0u;
+        uint32_t tot;
+        uint32_t ex = block_excl_scan256(v, sh, tot);
+        if (b < nb) sums[b] = ex + carry;
+        carry += tot;
+    }
+    if (threadIdx.x == 0 && total) *total = carry;
+}
+
+__global__ __launch_bounds__(kThreads) void k_scan_apply(
+    const uint32_t *__restrict__ in, uint32_t *__restrict__ out, uint32_t n,
+    const uint32_t *__restrict__ sums)
+{
+    __shared__ uint32_t sh[4];
+    uint32_t base = blockIdx.x * kScanTile + threadIdx.x * 4;
+    uint32_t v[4];
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        v[k] = (base + k < n) ? in[base + k] : 0u;
+        s += v[k];
+    }
+    uint32_t tot;
+    uint32_t ex = block_excl_scan256(s, sh, tot) + sums[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (base + k < n) out[base + k] = ex;
+        ex += v[k];
+    }
+}
+
+__global__ void k_iota(uint32_t *p, uint32_t n)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = i;
+}
+
+__global__ void k_fill_u32(uint32_t *p, uint32_t value, size_t n)
+{
+    size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i < n) p[i] = value;
+}
+
+}  // namespace
+
+int launch_calc_keys(const double4 *pos, uint32_t n, const GridInfo &g,
+                     uint32_t *keys, hipStream_t s)
+{
+    if (n == 0) return CUCO_OK;
+    k_calc_keys<<<grid_for(n, 256), 256, 0, s>>>(pos, n, g, keys);
+    CUCO_LAUNCH_CHECK();
+    return CUCO_OK;
+}
+
+int radix_sort_pairs(uint32_t *keys, uint32_t *vals, bool vals_identity,
+                     uint32_t n, int key_bits, const SortScratch &sc,
+                     hipStream_t s, uint32_t **keys_out, uint32_t **vals_out)
+{
+    *keys_out = keys;
+    *vals_out = vals;
+    if (n == 0) return CUCO_OK;
+    if (n > sc.capacity) {
+        set_error("radix sort: %u elements exceed scratch capacity %u", n,
+                  sc.capacity);
+        return CUCO_ERR_INVALID_ARG;
+    }
+    const int passes = key_bits <= 0 ? 0 : (key_bits + 7) / 8;
+    const uint32_t nblocks = (n + kSortTile - 1) / kSortTile;
+    uint32_t *bk[2] = {keys, sc.keys_tmp};
+    uint32_t *bv[2] = {vals, sc.vals_tmp};
+    int cur = 0;
+    if (passes == 0 && vals_identity && vals) {
+        k_iota<<<grid_for(n, 256), 256, 0, s>>>(vals, n);
+        CUCO_LAUNCH_CHECK();
+    }
+    for (int p = 0; p < passes; ++p) {
+        const int shift = 8 * p;
+        const uint32_t *vin =
+            (vals && !(p == 0 && vals_identity)) ? bv[cur] : nullptr;
+        uint32_t *vout = vals ? bv[cur ^ 1] : nullptr;
+        k_rs_upsweep<<<nblocks, kThreads, 0, s>>>(bk[cur], n, shift, nblocks,
+                                                  sc.counts);
+        CUCO_LAUNCH_CHECK();
+        k_rs_scan_digits<<<kBins, kThreads, 0, s>>>(sc.counts, nblocks,
+                                                    sc.totals);
+        CUCO_LAUNCH_CHECK();
+        k_rs_downsweep<<<nblocks, kThreads, 0, s>>>(bk[cur], vin, bk[cur ^ 1],
+                                                    vout, n, shift, nblocks,
+                                                    sc.counts, sc.totals);
+        CUCO_LAUNCH_CHECK();
+        cur ^= 1;
+    }
+    *keys_out = bk[cur];
+    *vals_out = vals ? bv[cur] : nullptr;
+    return CUCO_OK;
+}
+
+int launch_cell_offsets(const uint32_t *skeys, uint32_t n, uint32_t nkeys,
+                        uint32_t *off, hipStream_t s)
+{
+    k_cell_offsets<<<grid_for((int64_t)n + 1, 256), 256, 0, s>>>(skeys, n,
+                                                                  nkeys, off);
+    CUCO_LAUNCH_CHECK();
+    return CUCO_OK;
+}
+
+int launch_cell_start_end(const uint32_t *off, uint32_t ncells,
+                          uint32_t *start, uint32_t *end, hipStream_t s)
+{
+    if (ncells == 0) return CUCO_OK;
+    k_cell_start_end<<<grid_for(ncells, 256), 256, 0, s>>>(off, ncells, start,
+                                                           end);
+    CUCO_LAUNCH_CHECK();
+    return CUCO_OK;
+}
+
+int launch_reorder64(const double4 *pos, const uint32_t *perm, uint32_t n,
+                     double4 *spos, hipStream_t s)
+{
+    if (n == 0) return CUCO_OK;
+    k_reorder64<<<grid_for(n, 256), 256, 0, s>>>(pos, perm, n, spos);
+    CUCO_LAUNCH_CHECK();
+    return CUCO_OK;
+}
+
+int launch_reorder32(const double4 *pos, const uint32_t *perm,
+                     const uint32_t *skeys, uint32_t n, const GridInfo &g,
+                     float4 *spos, hipStream_t s)
+{
+    if (n == 0) return CUCO_OK;
+    k_reorder32<<<grid_for(n, 256), 256, 0, s>>>(pos, perm, skeys, n, g, spos);
+    CUCO_LAUNCH_CHECK();
+    return CUCO_OK;
+}
+
+size_t scan_scratch_words(uint32_t n)
+{
+    return (size_t)(n + kScanTile - 1) / kScanTile + 2;
+}
+
+int exclusive_scan_u32(const uint32_t *in, uint32_t *out, uint32_t n,
+                       uint32_t *scratch, uint32_t *total, hipStream_t s)
+{
+    if (n == 0) return CUCO_OK;
+    const uint32_t nb = (n + kScanTile - 1) / kScanTile;
+    k_scan_reduce<<<nb, kThreads, 0, s>>>(in, n, scratch);
+    CUCO_LAUNCH_CHECK();
+    k_scan_sums<<<1, kThreads, 0, s>>>(scratch, nb, total);
+    CUCO_LAUNCH_CHECK();
+    k_scan_apply<<<nb, kThreads, 0, s>>>(in, out, n, scratch);
+    CUCO_LAUNCH_CHECK();
+    return CUCO_OK;
+}
+
+int launch_fill_u32(uint32_t *p, uint32_t value, size_t n, hipStream_t s)
+{
+    if (n == 0) return CUCO_OK;
+    k_fill_u32<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(p, value, n);
+    CUCO_LAUNCH_CHECK();
+    return CUCO_OK;
+}
+
+}  // namespace cuco

@@ -1,0 +1,248 @@
+/*
+ * cuco_engine.h -- C ABI of the MI355X force engine for cuco's PM+P3M gravity.
+ *
+ * Plain C types only: opaque handle, pointers, sizes, int status returns.
+ * The library never exits the process; every entry point returns a
+ * cuco_status and cuco_last_error() gives the message of the last failure on
+ * the calling thread.  One HIP stream per handle; a handle is not
+ * thread-safe.  All host arrays are caller-owned; device arrays behind a
+ * handle are engine-owned and stay resident across drift/kick/force calls.
+ *
+ * Each entry point names the reference interface it replaces (file:line in
+ * liulei/cuco).  Units are the caller's (the reference uses kpc/h,
+ * 1e10 Msun/h, km/s: begrun.c:33-35,65-70).
+ */
+#ifndef CUCO_ENGINE_H
+#define CUCO_ENGINE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum cuco_status {
+    CUCO_OK = 0,
+    CUCO_ERR_INVALID_ARG = 1,  /* bad size, null pointer, unsupported value */
+    CUCO_ERR_HIP = 2,          /* a HIP runtime call failed */
+    CUCO_ERR_OUT_OF_MEMORY = 3,
+    CUCO_ERR_STATE = 4,        /* e.g. force requested before particles exist */
+    CUCO_ERR_UNSUPPORTED = 5   /* e.g. mesh size with a prime factor > 5 */
+} cuco_status;
+
+typedef enum cuco_precision {
+    CUCO_FP64 = 0, /* plain path: FP64 everywhere, reference cell order */
+    CUCO_FP32 = 1  /* chosen path: FP32 cell-relative pairs, FP32 mesh */
+} cuco_precision;
+
+/* Split and mesh parameters (allvars.h:9,24-25; pm_periodic.c:27-28;
+ * gravlist.cpp:36-40,61-70).  asmth and rcut are lengths; soft_h is the
+ * spline softening length h = 2.8*epsilon (All.ForceSoftening). */
+typedef struct cuco_params {
+    int32_t pm_grid;   /* PMGRID: mesh cells per dimension (even, 2^a 3^b 5^c) */
+    int32_t reserved0; /* must be 0 */
+    double box_size;   /* All.BoxSize */
+    double G;          /* All.G */
+    double asmth;      /* All.Asmth = ASMTH*BoxSize/PMGRID */
+    double rcut;       /* All.Rcut = RCUT*Asmth */
+    double soft_h;     /* All.ForceSoftening */
+} cuco_params;
+
+/* PARTICLE record, byte-compatible with allvars.h:35-45 (60 bytes). */
+typedef struct cuco_particle {
+    float Pos[3];
+    float Mass;
+    float Vel[3];
+    float GravAccel[3];
+    float GravPM[3];
+    int32_t Ti_endstep;
+    int32_t Ti_begstep;
+} cuco_particle;
+
+typedef struct cuco_engine cuco_engine;
+
+/* ---------------------------------------------------------------- */
+/* Errors                                                            */
+/* ---------------------------------------------------------------- */
+const char *cuco_status_string(int status);
+const char *cuco_last_error(void);
+
+/* ---------------------------------------------------------------- */
+/* Lifecycle                                                         */
+/* ---------------------------------------------------------------- */
+
+/* Replaces cudaInit (gravlist.cu:25-31), linklist_init's device set-up
+ * (gravlist.cpp:48-115), pm_init_periodic (pm_periodic.c:25-43) and the
+ * RadixSort constructor (radixsort.h:140-150).  Allocates every device
+ * buffer for up to max_particles particles on HIP device `device`. */
+int cuco_create(const cuco_params *params, int64_t max_particles,
+                int32_t device, cuco_engine **out);
+/* Frees everything the handle owns (the reference never frees). */
+int cuco_destroy(cuco_engine *e);
+/* set_softenings + setSoftParam (gravlist.cpp:28-45, gravlist.cu:179-181):
+ * re-sets the spline softening h (comoving), e.g. each step. */
+int cuco_set_softening(cuco_engine *e, double soft_h);
+
+/* ---------------------------------------------------------------- */
+/* Particle state (device resident, FP64)                            */
+/* ---------------------------------------------------------------- */
+
+/* Replaces copyPosToDevice (gravlist.cu:56-68): pos and vel are n*3
+ * doubles (x,y,z per particle); vel may be NULL (zero); mass may be NULL,
+ * in which case every particle has uniform_mass (header.Mass,
+ * gravlist.cpp:111).  Positions are wrapped into [0, BoxSize). */
+int cuco_upload_particles(cuco_engine *e, int64_t n, const double *pos,
+                          const double *vel, const double *mass,
+                          double uniform_mass);
+/* Reads back positions and velocities (n*3 doubles each; either may be NULL). */
+int cuco_download_particles(cuco_engine *e, double *pos, double *vel);
+/* Replaces copyAccelFromDevice (gravlist.cu:70-83) and the GravPM read-out
+ * (pm_periodic.c:308): n*3 doubles in original particle order, from the
+ * last force evaluation of either precision.  Either pointer may be NULL. */
+int cuco_download_accel(cuco_engine *e, double *acc_pp, double *acc_pm);
+
+/* ---------------------------------------------------------------- */
+/* Forces                                                            */
+/* ---------------------------------------------------------------- */
+
+/* Replaces gravity_linklist_cuda (gravlist.cpp:148-193): cell keys,
+ * stable radix sort, cell table, reorder, erfc-truncated PP pair force. */
+int cuco_pp_force(cuco_engine *e, int32_t precision);
+/* Replaces pmforce_periodic (pm_periodic.c:67-314): CIC deposit, r2c FFT,
+ * Green's function, c2r FFT, 4-point gradient, CIC interpolation. */
+int cuco_pm_force(cuco_engine *e, int32_t precision);
+/* Replaces compute_accelerations (accel.c:8-23): box wrap, PM when do_pm,
+ * PP always.  One sort serves both. */
+int cuco_compute_accelerations(cuco_engine *e, int32_t precision,
+                               int32_t do_pm);
+
+/* ---------------------------------------------------------------- */
+/* Integrator (device resident)                                      */
+/* ---------------------------------------------------------------- */
+
+/* move_particles + do_box_wrapping (predict.c:10-41): x += v*drift; wrap. */
+int cuco_drift(cuco_engine *e, double drift_factor);
+/* Kicks of advance_and_find_timesteps (timestep.c:223-226, 253-255):
+ * v += a_pp*kick_pp; v += a_pm*kick_pm, for all particles. */
+int cuco_kick(cuco_engine *e, double kick_pp, double kick_pm);
+/* nsteps global KDK steps; factors is nsteps*3 (k1, drift, k2) per step:
+ *   v += a*k1; x += v*drift; wrap; forces; v += a*k2
+ * (a = a_pp then a_pm).  Forces must be current on entry.  With
+ * use_graph != 0 one step is captured once as a hipGraph and replayed. */
+int cuco_kdk_steps(cuco_engine *e, int32_t precision, int32_t nsteps,
+                   const double *factors, int32_t use_graph);
+
+/* ---------------------------------------------------------------- */
+/* Drift / kick factors (host)                                       */
+/* ---------------------------------------------------------------- */
+
+#define CUCO_DRIFT_TABLE_LENGTH 1000 /* allvars.h:29 */
+
+/* init_drift_table (driftfac.c:14-42): drift[i] = int da/(H a^3) and
+ * kick[i] = int da/(H a^2) from a_begin to
+ * exp(ln a_begin + (ln a_max - ln a_begin)/1000*(i+1)),
+ * H = hubble*sqrt(om/a^3 + (1-om-ol)/a^2 + ol).  Double tables. */
+int cuco_init_drift_table(double a_begin, double a_max, double omega0,
+                          double omega_lambda, double hubble, double *drift,
+                          double *kick);
+/* get_drift_factor / get_gravkick_factor (driftfac.c:66-126) between two
+ * values of ln a (the reference takes integer times on the ln a timeline). */
+double cuco_table_factor(const double *table, double a_begin, double a_max,
+                         double loga0, double loga1);
+/* Factors of nsteps global KDK steps uniform in ln a from a_start to a_end:
+ * per step (kick to the midpoint, drift over the step, kick from the
+ * midpoint), the kicks of timestep.c:215-218 for a global step. */
+int cuco_kdk_factors(const double *drift, const double *kick, double a_begin,
+                     double a_max, double a_start, double a_end, int32_t nsteps,
+                     double *factors);
+
+/* ---------------------------------------------------------------- */
+/* Compatibility shims on PARTICLE-layout arrays                     */
+/* ---------------------------------------------------------------- */
+
+/* pmforce_periodic() on a caller's P[] (pm_periodic.c:67-314): uploads
+ * Pos/Mass, runs the PM force, writes P[i].GravPM. */
+int cuco_pmforce_periodic(cuco_engine *e, cuco_particle *P, int64_t n,
+                          int32_t precision);
+/* gravity_linklist_cuda() on a caller's P[] (gravlist.cpp:148-193):
+ * uploads Pos/Mass, runs the PP force, writes P[i].GravAccel. */
+int cuco_gravity_linklist(cuco_engine *e, cuco_particle *P, int64_t n,
+                          int32_t precision);
+
+/* ---------------------------------------------------------------- */
+/* Device-pointer API (mirrors proto.cuh:26-58)                      */
+/* ---------------------------------------------------------------- */
+
+/* allocateArray / freeArray / copyArray{To,From}Device (gravlist.cu:33-54). */
+int cuco_device_alloc(void **dptr, size_t bytes);
+int cuco_device_free(void *dptr);
+int cuco_copy_to_device(void *dst, const void *src, size_t bytes);
+int cuco_copy_from_device(void *dst, const void *src, size_t bytes);
+/* RadixSort::sort(keys, values, n, keyBits) (radixsort.h:170-187): stable
+ * key/value sort of n uint32 pairs on the device, ordering by bits
+ * [0, key_bits) of the key.  values may be NULL (keys only).  Scratch is
+ * allocated per call. */
+int cuco_radix_sort(uint32_t *d_keys, uint32_t *d_values, uint32_t n,
+                    uint32_t key_bits);
+
+/* Unnormalized 3D FFTs on device arrays, replacing the cuFFT plans of
+ * pm_periodic.c:34-35 and their executions cufftExecR2C (:140) and
+ * cufftExecC2R (:201).  Real arrays are M^3 (x-major), complex arrays
+ * M*M*(M/2+1) interleaved (re, im); precision selects double or float
+ * elements.  c2r overwrites its input; c2r(r2c(x)) = M^3 x. */
+int cuco_fft3d_r2c(int32_t M, int32_t precision, const void *d_in, void *d_out);
+int cuco_fft3d_c2r(int32_t M, int32_t precision, void *d_in, void *d_out);
+
+/* ---------------------------------------------------------------- */
+/* Inspection (parity tests, diagnostics)                            */
+/* ---------------------------------------------------------------- */
+
+/* Sorted keys and the sorting permutation (sorted position -> original
+ * index) of the last evaluation; n entries each. */
+int cuco_get_sorted(cuco_engine *e, uint32_t *keys, uint32_t *index);
+/* Cell table of the last evaluation in the reference's format
+ * (gravlist_kernel.cu:136-155): ncells = PMGRID^3 entries each, 0xffffffff
+ * for empty cells in both arrays.  Only for the FP64 (reference-order) key. */
+int cuco_get_cell_table(cuco_engine *e, uint32_t *cell_start,
+                        uint32_t *cell_end);
+/* The 1000-entry short-range table uploaded to the device (gravlist.cpp:56-59). */
+int cuco_get_shortrange_table(cuco_engine *e, float *table);
+/* Mesh of the last PM evaluation: which = 0 density (after deposit),
+ * 1 potential (after c2r).  PMGRID^3 doubles, x-major. */
+int cuco_get_pm_grid(cuco_engine *e, int32_t which, double *out);
+/* Exact count of ordered pairs i != j with r^2 <= rcut^2 (FP64 test) and
+ * of candidate pairs the FP32 kernel evaluates, for the current positions. */
+int cuco_count_pairs(cuco_engine *e, uint64_t *pairs_in,
+                     uint64_t *fp32_candidates);
+
+/* ---------------------------------------------------------------- */
+/* Timing (HIP events on the handle's stream)                        */
+/* ---------------------------------------------------------------- */
+
+enum {
+    CUCO_STAGE_KEYS = 0,
+    CUCO_STAGE_SORT,
+    CUCO_STAGE_CELLS,   /* cell table + reorder */
+    CUCO_STAGE_PP,
+    CUCO_STAGE_DEPOSIT,
+    CUCO_STAGE_FFT_FWD, /* z r2c + y pass */
+    CUCO_STAGE_GREEN,   /* fused x forward, Green's function, x inverse */
+    CUCO_STAGE_FFT_INV, /* y pass + z c2r */
+    CUCO_STAGE_INTERP,  /* gradient + interpolation */
+    CUCO_STAGE_COUNT
+};
+/* Enable (1) or disable (0) per-stage event timing. */
+int cuco_set_timing(cuco_engine *e, int32_t enable);
+/* Milliseconds per stage summed over evaluations since the last reset,
+ * and the number of evaluations; reset != 0 clears the sums afterwards. */
+int cuco_get_stage_times(cuco_engine *e, double *ms, int32_t nstages,
+                         int32_t *evaluations, int32_t reset);
+const char *cuco_stage_name(int32_t stage);
+/* Blocks until all work queued on the handle's stream is done. */
+int cuco_synchronize(cuco_engine *e);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
