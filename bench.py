@@ -1,0 +1,212 @@
+#!/usr/bin/env python3
+"""Force-evaluation throughput of the MI355X gravity engine.
+
+One step = one complete force evaluation of the workload: cell keys, stable
+radix sort, cell table, PP pair force and the whole PM chain (deposit, r2c,
+Green's function, c2r, gradient + interpolation).  Particles are resident in
+HBM before the timed region.  The headline value is the FP32 path's particle
+force evaluations per second at 256^3 particles / 512^3 mesh (the clustered
+500 Mpc/h box with 2000 NFW halos); the FP64 path, the FP32-vs-FP64 error
+percentiles, the PP kernel's roofline position and a CPU baseline (the FP64
+oracle on this host) are reported beside it.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "particle force evals/s at 256^3 (FP32 vs FP64 GPU) + 99th-pct force err"
+PEAK_FP32_TFLOPS = 157.3     # MI355X FP32 vector, AMD published
+PEAK_HBM_GBS = 8000.0
+FLOPS_PER_INTERACTION = 30
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def percentiles(err):
+    return {"median": float(np.median(err)), "p99": float(np.percentile(err, 99)),
+            "max": float(err.max())}
+
+
+def cpu_baseline(box, pos, mass, n_targets, seed=5006):
+    """FP64 oracle on the host: PP on a random target sample (the cell list
+    over all particles is built as in a full evaluation) plus the full PM;
+    returns the extrapolated whole-box force evals/s."""
+    import oracle
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
+    threads = min(threads, os.cpu_count() or threads)
+    from cuco_amd import ics
+    p = oracle.params(box.M, box.L, ics.G_INTERNAL, box.asmth, box.rcut,
+                      box.soft_h, nthreads=threads)
+    rng = np.random.default_rng(seed)
+    targets = np.sort(rng.choice(box.N, n_targets, replace=False))
+    t0 = time.perf_counter()
+    oracle.pp_force(p, pos, mass, targets[:1])
+    t_setup = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    oracle.pp_force(p, pos, mass, targets)
+    t_sample = time.perf_counter() - t0
+    t_pp = t_setup + max(t_sample - t_setup, 0.0) * box.N / n_targets
+    t0 = time.perf_counter()
+    oracle.pm_force(p, pos, mass)
+    t_pm = time.perf_counter() - t0
+    value = box.N / (t_pp + t_pm)
+    return {
+        "value": value,
+        "unit": "particle force evals/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"FP64 oracle (OpenMP, {threads} threads): PP for {n_targets} "
+                   f"random targets of the {box.N}-particle box, cell list over "
+                   f"all particles (setup {t_setup:.2f}s, sample {t_sample:.2f}s, "
+                   f"extrapolated PP {t_pp:.1f}s); full PM on the {box.M}^3 "
+                   f"mesh {t_pm:.1f}s"),
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__,
+                                 formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="halo256")
+    ap.add_argument("--fp64-steps", type=int, default=2)
+    ap.add_argument("--cpu-targets", type=int, default=20000)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not "
+              "supported (the engine is single-GPU; sharding is described in "
+              "DESIGN.md)", file=sys.stderr)
+        return 2
+
+    from cuco_amd import engine as E
+    from cuco_amd import ics
+
+    box = ics.WORKLOADS[args.workload]
+    t0 = time.perf_counter()
+    pos, vel, mass = ics.generate(box)
+    log(f"[bench] {box.name}: N={box.N} M={box.M} generated in {time.perf_counter() - t0:.1f}s")
+
+    eng = E.Engine(max_particles=box.N, **box.engine_kwargs())
+    eng.upload(pos, vel, mass)
+    log("[bench] uploaded")
+
+    # FP64 plain path (reference for the error percentiles).
+    eng.compute_accelerations(E.FP64, True)
+    eng.synchronize()
+    ref_pp, ref_pm = eng.accelerations()
+    eng.set_timing(True)
+    eng.stage_times(reset=True)
+    t0 = time.perf_counter()
+    for _ in range(args.fp64_steps):
+        eng.compute_accelerations(E.FP64, True)
+    eng.synchronize()
+    t64 = (time.perf_counter() - t0) / args.fp64_steps
+    st64, _ = eng.stage_times(reset=True)
+    log(f"[bench] fp64 {t64 * 1e3:.1f} ms/eval")
+
+    # FP32 chosen path: warmup, then the timed steps.
+    for _ in range(args.warmup):
+        eng.compute_accelerations(E.FP32, True)
+    eng.synchronize()
+    eng.stage_times(reset=True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        eng.compute_accelerations(E.FP32, True)
+    eng.synchronize()
+    t32 = time.perf_counter() - t0
+    st32, nev = eng.stage_times(reset=True)
+    eng.set_timing(False)
+    ms32 = t32 / args.steps * 1e3
+    log(f"[bench] fp32 {ms32:.2f} ms/eval over {args.steps} steps")
+
+    a_pp, a_pm = eng.accelerations()
+    tot = ref_pp + ref_pm
+    den = np.linalg.norm(tot, axis=1)
+    den = np.where(den > 0, den, 1.0)
+    errors = {
+        "pp": percentiles(np.linalg.norm(a_pp - ref_pp, axis=1) / den),
+        "pm": percentiles(np.linalg.norm(a_pm - ref_pm, axis=1) / den),
+        "total": percentiles(np.linalg.norm(a_pp + a_pm - tot, axis=1) / den),
+    }
+    log(f"[bench] errors {errors['total']}")
+
+    pairs_in, candidates = eng.count_pairs()
+    pp_ms = st32["pp"] / max(nev, 1)
+    achieved = FLOPS_PER_INTERACTION * pairs_in / (pp_ms * 1e-3) / 1e12
+    roofline = {
+        "bound": "mfma",
+        "note": ("compute roofline of the FP32 pair kernel; it runs on the FP32 "
+                 "vector ALUs (a pair force is no dense contraction, so no MFMA) "
+                 "against MI355X's published FP32 peak; packed FP32 is disabled "
+                 "in every kernel, which caps this kernel at half of that peak"),
+        "kernel": "k_pp32 (FP32 pair force)",
+        "achieved": achieved,
+        "peak": PEAK_FP32_TFLOPS,
+        "unit": "TFLOP/s",
+        "frac": achieved / PEAK_FP32_TFLOPS,
+        "traffic": None,
+        "pp_ms": pp_ms,
+        "pairs_in": pairs_in,
+        "candidates": candidates,
+        "flops_per_interaction": FLOPS_PER_INTERACTION,
+    }
+
+    value = box.N * args.steps / t32
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "particle force evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms32,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"{box.name}: {box.n_side}^3 particles, {box.M}^3 mesh, "
+                         f"{box.L / 1000:g} Mpc/h, 50% in {box.n_halos} NFW halos "
+                         "+ 50% uniform; one complete PP+PM force evaluation per step"),
+            "n_particles": box.N,
+            "pm_grid": box.M,
+            "path": "fp32 cell-relative pairs + fp32 mesh, fp64 positions",
+        },
+        "fp64_value": box.N / t64,
+        "fp64_ms_per_step": t64 * 1e3,
+        "fp32_over_fp64": t64 * 1e3 / ms32,
+        "error_vs_fp64": errors,
+        "stage_ms_fp32": {k: v / max(nev, 1) for k, v in st32.items()},
+        "stage_ms_fp64": {k: v / max(args.fp64_steps, 1) for k, v in st64.items()},
+        "roofline": roofline,
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        t0 = time.perf_counter()
+        out["cpu_baseline"] = cpu_baseline(box, pos, mass, args.cpu_targets)
+        log(f"[bench] cpu baseline in {time.perf_counter() - t0:.1f}s")
+    print(json.dumps(out), flush=True)
+    eng.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

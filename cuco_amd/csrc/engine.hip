@@ -745,6 +745,9 @@ int cuco_compute_accelerations(cuco_engine *e, int32_t precision, int32_t do_pm_
     CHECK_ENGINE(e);
     TRY(check_prec(precision));
     TRY(check_ready(e));
+    // A complete evaluation, as gravity_linklist_cuda is: keys and sort are
+    // recomputed even when the positions have not moved.
+    e->sorted_mode = -1;
     TRY(begin_eval(e));
     int rc = do_forces(e, precision, do_pm_flag);
     end_eval(e);

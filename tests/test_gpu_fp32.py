@@ -1,0 +1,78 @@
+"""Chosen FP32 path: cell-relative FP32 pairs with Kahan accumulation, FP32
+mesh.  Per-particle relative acceleration error against FP64 (median and
+99th percentile), separately for PM, PP and total, with |a_total| of the
+FP64 reference as the denominator.  Bar: 99th percentile <= 1e-3."""
+import numpy as np
+import pytest
+
+from cuco_amd import engine as E
+from cuco_amd import ics
+
+from helpers import halo_box, make_engine, oracle_params, percentiles, rel_err
+
+pytestmark = pytest.mark.gpu
+
+P99_BAR = 1e-3
+
+
+def fp32_errors(box, pos, mass, orc=None):
+    eng = make_engine(box)
+    eng.upload(pos, None, mass)
+    if orc is not None:
+        p = oracle_params(box)
+        ref_pp, _ = orc.pp_force(p, pos, mass)
+        ref_pm = orc.pm_force(p, pos, mass)
+    else:
+        eng.compute_accelerations(E.FP64, True)
+        ref_pp, ref_pm = eng.accelerations()
+    eng.compute_accelerations(E.FP32, True)
+    a_pp, a_pm = eng.accelerations()
+    tot = ref_pp + ref_pm
+    errs = {
+        "pp": percentiles(rel_err(a_pp, ref_pp, tot)),
+        "pm": percentiles(rel_err(a_pm, ref_pm, tot)),
+        "total": percentiles(rel_err(a_pp + a_pm, tot, tot)),
+    }
+    return eng, errs
+
+
+def test_lattice_error_percentiles(orc):
+    box = ics.WORKLOADS["lattice32"]
+    pos, vel, mass = ics.generate(box)
+    eng, errs = fp32_errors(box, pos, mass, orc)
+    print(errs)
+    for k in ("pp", "pm", "total"):
+        assert errs[k]["p99"] <= P99_BAR, (k, errs)
+        assert errs[k]["median"] <= 1e-4, (k, errs)
+
+
+def test_clustered_error_percentiles(orc):
+    box = halo_box()
+    pos, vel, mass = ics.generate(box)
+    eng, errs = fp32_errors(box, pos, mass, orc)
+    print(errs)
+    assert errs["total"]["p99"] <= P99_BAR, errs
+
+
+def test_halo128_error_percentiles():
+    box = ics.WORKLOADS["halo128"]
+    pos, vel, mass = ics.generate(box)
+    eng, errs = fp32_errors(box, pos, mass)
+    print(errs)
+    assert errs["total"]["p99"] <= P99_BAR, errs
+    assert errs["pp"]["p99"] <= P99_BAR, errs
+    assert errs["pm"]["p99"] <= P99_BAR, errs
+
+
+def test_fp32_is_deterministic_and_counts_cover_pairs():
+    box = halo_box(seed=4)
+    pos, vel, mass = ics.generate(box)
+    eng = make_engine(box)
+    eng.upload(pos, None, mass)
+    eng.compute_accelerations(E.FP32, True)
+    a1 = eng.accelerations()
+    eng.compute_accelerations(E.FP32, True)
+    a2 = eng.accelerations()
+    assert np.array_equal(a1[0], a2[0]) and np.array_equal(a1[1], a2[1])
+    pairs, cand = eng.count_pairs()
+    assert cand >= pairs > 0

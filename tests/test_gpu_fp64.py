@@ -1,0 +1,147 @@
+"""Plain FP64 path against the FP64 oracle.
+
+PP is compared bit for bit (same operation order, no fused multiply-adds,
+correctly rounded sqrt and division on both sides); PM to 1e-10 relative to
+|a_total| per particle (the FFTs differ in summation order); the sorted
+permutation and the reference-format cell table must be identical."""
+import numpy as np
+import pytest
+
+from cuco_amd import engine as E
+from cuco_amd import ics
+
+from helpers import halo_box, make_engine, oracle_params, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def lattice():
+    box = ics.WORKLOADS["lattice32"]
+    pos, vel, mass = ics.generate(box)
+    return box, pos, vel, mass
+
+
+def check_fp64(box, pos, mass, orc, targets=None):
+    p = oracle_params(box)
+    eng = make_engine(box)
+    eng.upload(pos, None, mass)
+    eng.compute_accelerations(E.FP64, True)
+    app, apm = eng.accelerations()
+    if targets is None:
+        ref_pp, npairs = orc.pp_force(p, pos, mass)
+        ref_pm = orc.pm_force(p, pos, mass)
+        tot = ref_pp + ref_pm
+        assert np.array_equal(app, ref_pp), "FP64 PP must be bit-identical"
+        assert rel_err(apm, ref_pm, tot).max() <= 1e-10
+        assert eng.count_pairs()[0] == npairs
+    else:
+        ref_pp, _ = orc.pp_force(p, pos, mass, targets)
+        ref_pm = orc.pm_force(p, pos, mass)[targets]
+        tot = ref_pp + ref_pm
+        assert np.array_equal(app[targets], ref_pp)
+        assert rel_err(apm[targets], ref_pm, tot).max() <= 1e-10
+    return eng, p
+
+
+def test_lattice_forces_order_and_cells(lattice, orc):
+    box, pos, vel, mass = lattice
+    eng, p = check_fp64(box, pos, mass, orc)
+    eng.compute_accelerations(E.FP64, True)
+    keys, idx = eng.sorted_order()
+    ok, oi = orc.calc_keys(p, pos)
+    ok, oi = orc.sort_pairs(ok, oi)
+    assert np.array_equal(keys, ok) and np.array_equal(idx, oi)
+    s, e = eng.cell_table()
+    os_, oe = orc.cell_table(ok, box.M ** 3)
+    assert np.array_equal(s, os_) and np.array_equal(e, oe)
+    assert np.array_equal(eng.shortrange_table(), orc.shortrange_table())
+
+
+def test_lattice_density_and_potential_grids(lattice, orc):
+    box, pos, vel, mass = lattice
+    p = oracle_params(box)
+    eng = make_engine(box)
+    eng.upload(pos, None, mass)
+    eng.pmforce_periodic(E.FP64)
+    rho = eng.pm_grid(0)
+    ref_rho = orc.pm_deposit(p, pos, mass)
+    assert np.abs(rho - ref_rho).max() <= 1e-12 * np.abs(ref_rho).max()
+    phi = eng.pm_grid(1)
+    ref_phi = orc.pm_potential(p, ref_rho)
+    assert np.abs(phi - ref_phi).max() <= 1e-11 * np.abs(ref_phi).max()
+
+
+def test_clustered_small_box(orc):
+    box = halo_box()
+    pos, vel, mass = ics.generate(box)
+    check_fp64(box, pos, mass, orc)
+
+
+def test_unequal_masses_and_coincident_particles(orc):
+    box = ics.small_box("mix", 24, 32, 40.0, 5)
+    pos, vel, mass = ics.generate(box)
+    rng = np.random.default_rng(5)
+    mass = mass * rng.uniform(0.5, 1.5, box.N)
+    pos[1] = pos[0]          # coincident pair: zero mutual force
+    pos[2] = [0.0, 0.0, 0.0]  # on the box corner
+    pos[3] = [np.nextafter(box.L, 0.0), 5.0, box.L * 0.5]
+    check_fp64(box, pos, mass, orc)
+
+
+def test_ragged_particle_counts(orc):
+    # N not a multiple of any block size; the reference launch grid drops
+    # the tail (gravlist.cu:92-98) -- every particle must get its force.
+    box = ics.small_box("ragged", 21, 32, 40.0, 11)
+    pos, vel, mass = ics.generate(box)
+    n = 9261 - 5
+    pos, mass = pos[:n], mass[:n]
+    eng = make_engine(box)
+    eng.upload(pos, None, mass)
+    eng.gravity_linklist(E.FP64)
+    app = eng.accelerations(pm=False)
+    ref, _ = orc.pp_force(oracle_params(box), pos, mass)
+    assert np.array_equal(app, ref)
+
+
+def test_halo128_subset(orc):
+    box = ics.WORKLOADS["halo128"]
+    pos, vel, mass = ics.generate(box)
+    rng = np.random.default_rng(2)
+    targets = np.sort(rng.choice(box.N, 3000, replace=False))
+    check_fp64(box, pos, mass, orc, targets)
+
+
+def test_particle_record_shims(lattice, orc):
+    box, pos, vel, mass = lattice
+    P = np.zeros(box.N, dtype=E.PARTICLE_DTYPE)
+    P["Pos"] = pos.astype(np.float32)
+    P["Mass"] = mass.astype(np.float32)
+    eng = make_engine(box)
+    eng.gravity_particles(P, E.FP64)
+    eng.pmforce_particles(P, E.FP64)
+    pf = P["Pos"].astype(np.float64)
+    p = oracle_params(box)
+    ref_pp, _ = orc.pp_force(p, pf, P["Mass"].astype(np.float64))
+    ref_pm = orc.pm_force(p, pf, P["Mass"].astype(np.float64))
+    np.testing.assert_array_equal(P["GravAccel"], ref_pp.astype(np.float32))
+    tot = np.linalg.norm(ref_pp + ref_pm, axis=1)
+    assert (np.linalg.norm(P["GravPM"] - ref_pm, axis=1) / tot).max() < 1e-6
+
+
+def test_errors_are_reported_not_fatal(lattice):
+    box, pos, vel, mass = lattice
+    eng = make_engine(box, capacity=100)
+    with pytest.raises(E.CucoError) as ei:
+        eng.upload(pos, None, mass)          # beyond capacity
+    assert ei.value.status == 1
+    with pytest.raises(E.CucoError) as ei:
+        eng.compute_accelerations(E.FP64)     # nothing uploaded
+    assert ei.value.status == 4
+    bad = pos[:100].copy()
+    bad[7, 1] = np.nan
+    with pytest.raises(E.CucoError):
+        eng.upload(bad, None, mass[:100])
+    eng.upload(pos[:100], None, mass[:100])
+    with pytest.raises(E.CucoError):
+        eng.compute_accelerations(7)
