@@ -68,6 +68,7 @@ def lib():
             "orc_pp_force": (C.c_int, [P, dp, dp, C.c_int64, i64p, C.c_int64,
                                        dp, C.POINTER(C.c_uint64)]),
             "orc_pp_force_bruteforce": (C.c_int, [P, dp, dp, C.c_int64, dp]),
+            "orc_pp_last_timing": (None, [dp, dp]),
             "orc_pm_deposit": (None, [P, dp, dp, C.c_int64, dp]),
             "orc_pm_potential": (C.c_int, [P, dp, dp]),
             "orc_pm_interpolate": (C.c_int, [P, dp, dp, C.c_int64, dp]),
@@ -158,6 +159,13 @@ def pp_force(p, pos, mass, targets=None):
     if rc != 0:
         raise RuntimeError(f"orc_pp_force failed: {rc}")
     return acc, int(cnt.value)
+
+
+def pp_last_timing():
+    """(setup_s, eval_s) of the last pp_force call."""
+    a, b = C.c_double(), C.c_double()
+    lib().orc_pp_last_timing(C.byref(a), C.byref(b))
+    return a.value, b.value
 
 
 def pp_force_bruteforce(p, pos, mass):

@@ -25,6 +25,23 @@ typedef double complex cplx;
 #define M_PI 3.14159265358979323846
 #endif
 
+static double g_pp_setup_s = 0, g_pp_eval_s = 0;
+
+static double wall_seconds(void)
+{
+#ifdef _OPENMP
+    return omp_get_wtime();
+#else
+    return 0.0;
+#endif
+}
+
+void orc_pp_last_timing(double *setup_s, double *eval_s)
+{
+    *setup_s = g_pp_setup_s;
+    *eval_s = g_pp_eval_s;
+}
+
 static int orc_threads(const orc_params *p)
 {
 #ifdef _OPENMP
@@ -197,6 +214,7 @@ int orc_pp_force(const orc_params *p, const double *pos, const double *mass,
     /* A cell range wider than the mesh would visit a cell twice. */
     if (2.0 * p->rcut * p->to_grid_fac + 2.0 > (double)M) return -2;
 
+    const double t_start = wall_seconds();
     uint32_t *keys = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(n ? n : 1));
     uint32_t *index = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(n ? n : 1));
     uint32_t *cstart = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)ncells);
@@ -217,6 +235,7 @@ int orc_pp_force(const orc_params *p, const double *pos, const double *mass,
         spos[4 * i + 3] = mass[o];
     }
 
+    const double t_setup_done = wall_seconds();
     const int64_t nt = targets ? ntargets : n;
     uint64_t total_in = 0;
     const double fac = p->to_grid_fac, rcut = p->rcut;
@@ -268,6 +287,8 @@ int orc_pp_force(const orc_params *p, const double *pos, const double *mass,
         total_in += in;
     }
     if (pairs_in) *pairs_in = total_in;
+    g_pp_setup_s = t_setup_done - t_start;
+    g_pp_eval_s = wall_seconds() - t_setup_done;
     free(keys); free(index); free(cstart); free(cend); free(spos);
     return 0;
 }

@@ -99,6 +99,10 @@ int orc_pp_force(const orc_params *p, const double *pos, const double *mass,
                  int64_t n, const int64_t *targets, int64_t ntargets,
                  double *acc, uint64_t *pairs_in);
 
+/* Wall-clock seconds of the last orc_pp_force: cell-list set-up (keys, sort,
+ * table, reorder) and the pair loop.  Used to time the CPU baseline. */
+void orc_pp_last_timing(double *setup_s, double *eval_s);
+
 /* Brute-force O(N^2) twin of orc_pp_force (every pair, minimum image, same
  * pair law) used to pin the cell-list traversal. */
 int orc_pp_force_bruteforce(const orc_params *p, const double *pos,
