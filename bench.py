@@ -83,6 +83,9 @@ def main() -> int:
     ap.add_argument("--fp64-steps", type=int, default=2)
     ap.add_argument("--cpu-targets", type=int, default=100000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--pmc-probe", action="store_true",
+                    help="counter pass: one FP32 evaluation of the workload and "
+                         "a 1 GiB device copy of known size, nothing else")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -102,6 +105,15 @@ def main() -> int:
     eng = E.Engine(max_particles=box.N, **box.engine_kwargs())
     eng.upload(pos, vel, mass)
     log("[bench] uploaded")
+    if args.pmc_probe:
+        eng.compute_accelerations(E.FP32, True)
+        eng.synchronize()
+        nbytes = 1 << 30
+        src, dst = E.DeviceBuffer(nbytes), E.DeviceBuffer(nbytes)
+        dst.copy_from(src, nbytes)
+        log("[bench] pmc probe done: one FP32 evaluation, one 1 GiB copy")
+        eng.close()
+        return 0
 
     # FP64 plain path (reference for the error percentiles).
     eng.compute_accelerations(E.FP64, True)

@@ -78,6 +78,7 @@ struct cuco_engine {
     uint32_t capacity = 0;
     uint32_t n = 0;
     bool have_particles = false;
+    double uniform_mass = 0;  // > 0 when every particle has this mass
 
     double4 *pos = nullptr, *vel = nullptr;
     double4 *acc_pp64 = nullptr, *acc_pm64 = nullptr;
@@ -103,6 +104,7 @@ struct cuco_engine {
     void *rho[2] = {nullptr, nullptr};
     void *phi[2] = {nullptr, nullptr};
     void *cgrid[2] = {nullptr, nullptr};
+    void *dep_scratch[2] = {nullptr, nullptr};
     PmTables tb[2];
     double pm_fac = 0;
 
@@ -291,6 +293,7 @@ int ensure_pm(cuco_engine *e)
     TRY(dev_alloc(e->rho[k], sizeof(T) * M * M * M));
     TRY(dev_alloc(e->phi[k], sizeof(T) * M * M * M));
     TRY(dev_alloc(e->cgrid[k], sizeof(Cx<T>) * M * M * (M / 2 + 1)));
+    TRY(dev_alloc(e->dep_scratch[k], pm_deposit_scratch_bytes(e->capacity, k == 0)));
     return make_pm_tables<T>(e, e->tb[k]);
 }
 
@@ -349,7 +352,7 @@ int do_pp(cuco_engine *e, int precision, unsigned long long *counter)
     } else {
         TRY(launch_pp32(e->spos32, e->skeys, e->perm, e->off, e->n, e->gm, e->pp,
                         e->table32, e->tiles, e->acc_pp32, counter,
-                        e->pp32_workgroups, e->stream));
+                        e->pp32_workgroups, e->uniform_mass, e->stream));
     }
     TRY(stage_stop(e, CUCO_STAGE_PP));
     e->pp_prec = precision;
@@ -369,7 +372,7 @@ int do_pm_t(cuco_engine *e, int precision)
     T *phi = static_cast<T *>(e->phi[k]);
     Cx<T> *c = static_cast<Cx<T> *>(e->cgrid[k]);
     TRY(stage_start(e, CUCO_STAGE_DEPOSIT));
-    TRY(pm_deposit<T>(spos, e->off, e->n, g, rho, e->stream));
+    TRY(pm_deposit<T>(spos, e->skeys, e->off, e->n, g, rho, e->dep_scratch[k], e->stream));
     TRY(stage_stop(e, CUCO_STAGE_DEPOSIT));
     TRY(stage_start(e, CUCO_STAGE_FFT_FWD));
     TRY(pm_fft_forward<T>(rho, c, e->M, e->tb[k], e->stream));
@@ -605,6 +608,7 @@ int cuco_destroy(cuco_engine *e)
     free_ptr(e->tiles.scan_scratch);
     for (int k = 0; k < 2; ++k) {
         free_ptr(e->rho[k]); free_ptr(e->phi[k]); free_ptr(e->cgrid[k]);
+        free_ptr(e->dep_scratch[k]);
         free_ptr(e->tb[k].tw); free_ptr(e->tb[k].gexp); free_ptr(e->tb[k].gsinc);
     }
     free_ptr(e->dfactors); free_ptr(e->dstep);
@@ -637,6 +641,7 @@ int cuco_upload_particles(cuco_engine *e, int64_t n, const double *pos,
     }
     std::vector<double4> hp((size_t)n), hv((size_t)n);
     const double L = e->prm.box_size;
+    double common = mass ? mass[0] : uniform_mass;
     for (int64_t i = 0; i < n; ++i) {
         double x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
         if (!std::isfinite(x) || !std::isfinite(y) || !std::isfinite(z)) {
@@ -647,6 +652,7 @@ int cuco_upload_particles(cuco_engine *e, int64_t n, const double *pos,
         TRY(wrap_host(&y, L));
         TRY(wrap_host(&z, L));
         hp[i] = make_double4(x, y, z, mass ? mass[i] : uniform_mass);
+        if (hp[i].w != common) common = 0;
         hv[i] = vel ? make_double4(vel[3 * i], vel[3 * i + 1], vel[3 * i + 2], 0.0)
                     : make_double4(0, 0, 0, 0);
     }
@@ -655,6 +661,7 @@ int cuco_upload_particles(cuco_engine *e, int64_t n, const double *pos,
     CUCO_HIP(hipStreamSynchronize(e->stream));
     if ((uint32_t)n != e->n) destroy_graph(e);
     e->n = (uint32_t)n;
+    e->uniform_mass = common > 0 ? common : 0;
     e->have_particles = true;
     e->sorted_mode = -1;
     e->pp_prec = e->pm_prec = -1;
@@ -917,6 +924,17 @@ int cuco_copy_to_device(void *dst, const void *src, size_t bytes)
 int cuco_copy_from_device(void *dst, const void *src, size_t bytes)
 {
     CUCO_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return CUCO_OK;
+}
+
+int cuco_copy_device_to_device(void *dst, const void *src, size_t bytes)
+{
+    if (!dst || !src || bytes % 16) {
+        set_error("null pointer or size %zu not a multiple of 16", bytes);
+        return CUCO_ERR_INVALID_ARG;
+    }
+    TRY(launch_copy16(dst, src, bytes, nullptr));
+    CUCO_HIP(hipDeviceSynchronize());
     return CUCO_OK;
 }
 

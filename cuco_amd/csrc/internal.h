@@ -46,6 +46,7 @@ size_t scan_scratch_words(uint32_t n);
 int exclusive_scan_u32(const uint32_t *in, uint32_t *out, uint32_t n,
                        uint32_t *scratch, uint32_t *total, hipStream_t s);
 int launch_fill_u32(uint32_t *p, uint32_t value, size_t n, hipStream_t s);
+int launch_copy16(void *dst, const void *src, size_t bytes, hipStream_t s);
 
 // ------------------------------------------------------------ PP
 int launch_pp64(const double4 *spos, const uint32_t *perm, const uint32_t *off,
@@ -68,7 +69,7 @@ int launch_pp32(const float4 *spos, const uint32_t *skeys,
                 const uint32_t *perm, const uint32_t *off, uint32_t n,
                 const GridInfo &g, const PairParams &pp, const float *table32,
                 const TileInfo &t, float4 *acc, unsigned long long *cand_counter,
-                int num_workgroups, hipStream_t s);
+                int num_workgroups, double uniform_mass, hipStream_t s);
 
 // ------------------------------------------------------------ PM
 struct PmTables {
@@ -76,9 +77,13 @@ struct PmTables {
     void *gexp = nullptr;    // M: exp(-k^2 asmth2) with signed k
     void *gsinc = nullptr;   // M: 1/sinc(pi k/M)^4
 };
+// scratch: pm_deposit_scratch_bytes(n, fp64) bytes (per-cell corner sums
+// and per-wave partials).
 template <typename T>
-int pm_deposit(const void *spos, const uint32_t *off, uint32_t n,
-               const GridInfo &g, T *rho, hipStream_t s);
+int pm_deposit(const void *spos, const uint32_t *skeys, const uint32_t *off,
+               uint32_t n, const GridInfo &g, T *rho, void *scratch,
+               hipStream_t s);
+size_t pm_deposit_scratch_bytes(uint32_t n, int fp64);
 template <typename T>
 int pm_fft_forward(const T *rho, Cx<T> *c, int M, const PmTables &tb,
                    hipStream_t s);

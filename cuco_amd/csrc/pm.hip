@@ -329,13 +329,27 @@ __global__ __launch_bounds__(kFftThreads) void k_lines_green(
 }
 
 // ------------------------------------------------------------- deposit
-// Particle view: absolute FP64 position, or FP32 cell-relative position.
-struct View64 {
+// Two passes, both deterministic and free of atomics:
+//
+//  k_cic_segments: one lane per sorted particle.  Each lane forms its eight
+//    CIC corner contributions m*wx*wy*wz (pm_periodic.c:123-131), and the
+//    wave runs a segmented inclusive scan keyed by the cell key (particles of
+//    a cell are consecutive after the sort).  A cell that lies inside one
+//    wave gets its eight corner sums stored at slot = its first sorted
+//    index; for cells that straddle waves every wave stores the sums of its
+//    first and of its last segment.
+//  k_cic_gather: one thread per mesh node, nodes walked in 4x4x16 bricks so
+//    the cell-table reads of neighbouring nodes share cache lines.  A node
+//    adds, in a fixed order, the corner sum of each of its eight neighbour
+//    cells (a straddling cell adds its wave partials in wave order).
+
+struct Src64 {
     const double4 *p;
-    __device__ inline void weights(uint32_t j, const GridInfo &g, int, int, int,
+    __device__ inline void weights(uint32_t j, uint32_t key, const GridInfo &g,
                                    double &dx, double &dy, double &dz,
                                    double &m) const
     {
+        (void)key;
         const double4 q = p[j];
         // pm_periodic.c:99-118: d = fac*x - slab, with the clamped slab.
         dx = g.fac * q.x - cell_coord(g.fac, q.x, g.M);
@@ -345,13 +359,14 @@ struct View64 {
     }
 };
 
-struct View32 {
+struct Src32 {
     const float4 *p;
     float facf;
-    __device__ inline void weights(uint32_t j, const GridInfo &, int, int, int,
+    __device__ inline void weights(uint32_t j, uint32_t key, const GridInfo &,
                                    float &dx, float &dy, float &dz,
                                    float &m) const
     {
+        (void)key;
         const float4 q = p[j];
         dx = q.x * facf;
         dy = q.y * facf;
@@ -360,41 +375,116 @@ struct View32 {
     }
 };
 
-template <typename T, typename V, typename ACC>
-__global__ __launch_bounds__(256) void k_deposit(V view, const uint32_t *__restrict__ off,
-                                                 GridInfo g, T *__restrict__ rho)
+template <typename T>
+__device__ inline T shfl_up_t(T v, int o) { return __shfl_up(v, o, 64); }
+
+template <typename T, typename V>
+__global__ __launch_bounds__(256) void k_cic_segments(
+    V src, const uint32_t *__restrict__ skeys, uint32_t n, GridInfo g,
+    T *__restrict__ cell_sums, T *__restrict__ first_part,
+    T *__restrict__ last_part)
+{
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const uint32_t wave = i >> 6;
+    const uint32_t base = wave << 6;
+    const bool valid = i < n;
+    const uint32_t key = valid ? skeys[i] : 0xffffffffu;
+    T v[8];
+    if (valid) {
+        T dx, dy, dz, m;
+        src.weights(i, key, g, dx, dy, dz, m);
+        const T wx[2] = {(T)1.0 - dx, dx}, wy[2] = {(T)1.0 - dy, dy},
+                wz[2] = {(T)1.0 - dz, dz};
+#pragma unroll
+        for (int c = 0; c < 8; ++c) v[c] = m * wx[c >> 2] * wy[(c >> 1) & 1] * wz[c & 1];
+    } else {
+#pragma unroll
+        for (int c = 0; c < 8; ++c) v[c] = 0;
+    }
+    // Segmented inclusive scan; keys are non-decreasing along the wave, so
+    // lane-o holding the same key means every lane between does too.
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t ko = __shfl_up(key, o, 64);
+        const bool take = lane >= o && ko == key;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            const T t = shfl_up_t(v[c], o);
+            v[c] += take ? t : (T)0;
+        }
+    }
+    const uint32_t knext = __shfl_down(key, 1, 64);
+    const bool seg_end = valid && (lane == 63 || knext != key);
+    // First lane of this lane's segment inside the wave.
+    const uint64_t heads = __ballot(lane == 0 || __shfl_up(key, 1, 64) != key);
+    const uint64_t below = heads & ((lane == 63) ? ~0ull : ((2ull << lane) - 1));
+    const int start = 63 - __clzll(below);
+    const uint32_t prev_key = base > 0 ? skeys[base - 1] : 0xffffffffu;
+    const uint32_t next_key = base + 64 < n ? skeys[base + 64] : 0xffffffffu;
+    if (seg_end) {
+        const bool open_left = start == 0 && prev_key == key;
+        const bool open_right = lane == 63 && next_key == key;
+        if (!open_left && !open_right) {
+#pragma unroll
+            for (int c = 0; c < 8; ++c) cell_sums[(size_t)(base + start) * 8 + c] = v[c];
+        }
+        if (start == 0) {
+#pragma unroll
+            for (int c = 0; c < 8; ++c) first_part[(size_t)wave * 8 + c] = v[c];
+        }
+        if (lane == 63) {
+#pragma unroll
+            for (int c = 0; c < 8; ++c) last_part[(size_t)wave * 8 + c] = v[c];
+        }
+    }
+}
+
+constexpr int kBx = 4, kBy = 4, kBz = 16;  // node brick, 256 threads
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_cic_gather(
+    const uint32_t *__restrict__ off, GridInfo g, const T *__restrict__ cell_sums,
+    const T *__restrict__ first_part, const T *__restrict__ last_part,
+    T *__restrict__ rho)
 {
     const int M = g.M;
-    const size_t node = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-    const size_t M3 = (size_t)M * M * M;
-    if (node >= M3) return;
-    const int z = (int)(node % M);
-    const int y = (int)((node / M) % M);
-    const int x = (int)(node / ((size_t)M * M));
-    ACC sum = 0;
+    const int nbz = (M + kBz - 1) / kBz, nby = (M + kBy - 1) / kBy;
+    const int b = blockIdx.x;
+    const int bz = b % nbz, by = (b / nbz) % nby, bx = b / (nbz * nby);
+    const int t = threadIdx.x;
+    const int z = bz * kBz + (t % kBz);
+    const int y = by * kBy + ((t / kBz) % kBy);
+    const int x = bx * kBx + t / (kBz * kBy);
+    if (x >= M || y >= M || z >= M) return;
+    double sum = 0;
 #pragma unroll
     for (int a = 0; a < 2; ++a) {
         const int cx = wrap_index(x - a, M);
 #pragma unroll
-        for (int b = 0; b < 2; ++b) {
-            const int cy = wrap_index(y - b, M);
+        for (int bb = 0; bb < 2; ++bb) {
+            const int cy = wrap_index(y - bb, M);
 #pragma unroll
             for (int c = 0; c < 2; ++c) {
                 const int cz = wrap_index(z - c, M);
                 const uint32_t key = cell_key(g, cx, cy, cz);
                 const uint32_t s = off[key], e = off[key + 1];
-                for (uint32_t j = s; j < e; ++j) {
-                    T dx, dy, dz, m;
-                    view.weights(j, g, cx, cy, cz, dx, dy, dz, m);
-                    const T wx = a == 0 ? (T)1.0 - dx : dx;
-                    const T wy = b == 0 ? (T)1.0 - dy : dy;
-                    const T wz = c == 0 ? (T)1.0 - dz : dz;
-                    sum += (ACC)(m * wx * wy * wz);
+                if (e == s) continue;
+                // Corner of this node seen from the cell: a, bb, c = 1 means
+                // the node is the cell's upper neighbour along that axis.
+                const int corner = (a << 2) | (bb << 1) | c;
+                const uint32_t w0 = s >> 6, w1 = (e - 1) >> 6;
+                if (w0 == w1) {
+                    sum += (double)cell_sums[(size_t)s * 8 + corner];
+                } else {
+                    sum += (double)last_part[(size_t)w0 * 8 + corner];
+                    for (uint32_t w = w0 + 1; w <= w1; ++w)
+                        sum += (double)first_part[(size_t)w * 8 + corner];
                 }
             }
         }
     }
-    rho[node] = (T)sum;
+    rho[((size_t)x * M + y) * M + z] = (T)sum;
 }
 
 // ------------------------------------------------------------- interpolation
@@ -496,23 +586,42 @@ bool fft_size_supported(int M)
 }
 
 template <typename T>
-int pm_deposit(const void *spos, const uint32_t *off, uint32_t n,
-               const GridInfo &g, T *rho, hipStream_t s)
+int pm_deposit(const void *spos, const uint32_t *skeys, const uint32_t *off,
+               uint32_t n, const GridInfo &g, T *rho, void *scratch,
+               hipStream_t s)
 {
-    const size_t M3 = (size_t)g.M * g.M * g.M;
-    const unsigned grid = (unsigned)((M3 + 255) / 256);
-    (void)n;
-    if (sizeof(T) == 8) {
-        View64 v{static_cast<const double4 *>(spos)};
-        k_deposit<double, View64, double><<<grid, 256, 0, s>>>(
-            v, off, g, reinterpret_cast<double *>(rho));
-    } else {
-        View32 v{static_cast<const float4 *>(spos), (float)g.fac};
-        k_deposit<float, View32, double><<<grid, 256, 0, s>>>(
-            v, off, g, reinterpret_cast<float *>(rho));
+    const int M = g.M;
+    T *cell_sums = static_cast<T *>(scratch);
+    const size_t nwaves = ((size_t)n + 63) / 64;
+    T *first_part = cell_sums + (size_t)n * 8;
+    T *last_part = first_part + nwaves * 8;
+    if (n > 0) {
+        if (sizeof(T) == 8) {
+            Src64 v{static_cast<const double4 *>(spos)};
+            k_cic_segments<double, Src64><<<grid_for(n, 256), 256, 0, s>>>(
+                v, skeys, n, g, reinterpret_cast<double *>(cell_sums),
+                reinterpret_cast<double *>(first_part),
+                reinterpret_cast<double *>(last_part));
+        } else {
+            Src32 v{static_cast<const float4 *>(spos), (float)g.fac};
+            k_cic_segments<float, Src32><<<grid_for(n, 256), 256, 0, s>>>(
+                v, skeys, n, g, reinterpret_cast<float *>(cell_sums),
+                reinterpret_cast<float *>(first_part),
+                reinterpret_cast<float *>(last_part));
+        }
+        CUCO_LAUNCH_CHECK();
     }
+    const unsigned nb = (unsigned)(((M + kBx - 1) / kBx) * ((M + kBy - 1) / kBy) *
+                                   ((M + kBz - 1) / kBz));
+    k_cic_gather<T><<<nb, 256, 0, s>>>(off, g, cell_sums, first_part, last_part, rho);
     CUCO_LAUNCH_CHECK();
     return CUCO_OK;
+}
+
+size_t pm_deposit_scratch_bytes(uint32_t n, int fp64)
+{
+    const size_t elem = fp64 ? 8 : 4;
+    return elem * 8 * ((size_t)n + ((size_t)n + 63) / 64 * 2 + 64);
 }
 
 template <typename T>
@@ -592,10 +701,12 @@ int pm_fft_x(Cx<T> *c, int M, bool inverse, hipStream_t s, const PmTables &tb)
 
 template int pm_fft_x<double>(Cx<double> *, int, bool, hipStream_t, const PmTables &);
 template int pm_fft_x<float>(Cx<float> *, int, bool, hipStream_t, const PmTables &);
-template int pm_deposit<double>(const void *, const uint32_t *, uint32_t,
-                                const GridInfo &, double *, hipStream_t);
-template int pm_deposit<float>(const void *, const uint32_t *, uint32_t,
-                               const GridInfo &, float *, hipStream_t);
+template int pm_deposit<double>(const void *, const uint32_t *, const uint32_t *,
+                                uint32_t, const GridInfo &, double *, void *,
+                                hipStream_t);
+template int pm_deposit<float>(const void *, const uint32_t *, const uint32_t *,
+                               uint32_t, const GridInfo &, float *, void *,
+                               hipStream_t);
 template int pm_fft_forward<double>(const double *, Cx<double> *, int,
                                     const PmTables &, hipStream_t);
 template int pm_fft_forward<float>(const float *, Cx<float> *, int,

@@ -32,11 +32,20 @@ constexpr int kWaves = 4;
 constexpr int kThreads = 64 * kWaves;
 constexpr int kRanges = 256;   // cell ranges per wave before the j stream
 constexpr int kJBuf = 128;     // staged j particles per wave
+constexpr int kTab = 4096;     // LDS erfc table; zero from the cut bin on
 
+// All lengths of the kernel are scaled by s = asmthfac (table bins per
+// length unit), so a pair's table bin is simply (int)r.  A pair term in
+// scaled units is dx_s * m * rinv_s^3 * T; the physical acceleration is
+// G * s^2 times the scaled sum.
 struct P32 {
-    float rcut, rcut2, filter2;
-    float asmthfac, h, h_inv, h3_inv, G;
-    float cellf, inv_cellf;
+    float s;                      // asmthfac
+    float rcut, rcut2, filter2;   // scaled
+    float h, h_inv, h3_inv;       // scaled spline softening
+    float cellf, inv_cellf;       // scaled cell size
+    float eps;                    // added to r^2: keeps rsq finite at r = 0
+    float out_scale;              // G * s^2
+    float mass;                   // uniform particle mass (UNIFORM variant)
     int M;
 };
 
@@ -105,7 +114,13 @@ __device__ inline int unwrap_delta(int d, int M)
 }
 
 // Pair sum of one lane against cnt staged j particles.
-template <bool MASK>
+//
+// GENERIC=false is the reference configuration: rcut sits on a table bin
+// edge (RCUT/ASMTH = 4.5 puts it on bin 750) so the table's zeros do the cut,
+// and the host has checked that no candidate can reach bin kTab.  GENERIC
+// keeps the explicit r^2 <= rcut^2 mask and clamps the bin.  UNIFORM folds
+// the common particle mass into the LDS table.
+template <bool GENERIC, bool UNIFORM>
 __device__ inline void interact(const float4 *__restrict__ sj, int cnt,
                                 const P32 &pp, const float *__restrict__ tab,
                                 float xi, float yi, float zi, float &sx,
@@ -117,28 +132,29 @@ __device__ inline void interact(const float4 *__restrict__ sj, int cnt,
         const float dx = q.x - xi;
         const float dy = q.y - yi;
         const float dz = q.z - zi;
-        float r2 = dx * dx + dy * dy + dz * dz;
-        const bool inside = r2 <= pp.rcut2;
-        // Self and coincident pairs: dx = dy = dz = 0, so the clamp only keeps
-        // rsq finite and the term is exactly zero (gravlist_kernel.cu:64-69).
-        r2 = fmaxf(r2, 1e-20f);
+        // Self and coincident pairs: dx = dy = dz = 0, so eps only keeps rsq
+        // finite and the term is exactly zero (gravlist_kernel.cu:64-69).
+        const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, pp.eps)));
         const float rinv = __builtin_amdgcn_rsqf(r2);
         const float r = r2 * rinv;
         const float rinv3 = rinv * rinv * rinv;
-        float fac = q.w * rinv3;
+        float fac = UNIFORM ? rinv3 : q.w * rinv3;
         if (r < pp.h) {
+            // gravlist_kernel.cu:47-56 with 1/u^3 = h^3 rinv^3
             const float u = r * pp.h_inv;
+            const float m = UNIFORM ? 1.0f : q.w;
             if (u < 0.5f)
-                fac = q.w * pp.h3_inv * (10.66667f + u * u * (32.0f * u - 38.4f));
+                fac = m * pp.h3_inv * (10.66667f + u * u * (32.0f * u - 38.4f));
             else
-                fac = q.w * pp.h3_inv *
+                fac = m * pp.h3_inv *
                           (21.33333f - 48.0f * u + 38.4f * u * u -
                            10.66667f * u * u * u) -
                       0.06667f * fac;
         }
-        const int idx = (int)fminf(pp.asmthfac * r, (float)(kTabPadded - 1));
+        int idx = (int)r;
+        if (GENERIC) idx = min(idx, kTab - 1);
         float t = tab[idx];
-        if (MASK) t = inside ? t : 0.0f;
+        if (GENERIC) t = r2 <= pp.rcut2 ? t : 0.0f;
         fac *= t;
         sx += dx * fac;
         sy += dy * fac;
@@ -154,7 +170,7 @@ __device__ inline void kahan_add(float &sum, float &comp, float v)
     sum = t;
 }
 
-template <bool MASK, bool COUNT>
+template <bool GENERIC, bool UNIFORM>
 __global__ __launch_bounds__(kThreads) void k_pp32(
     const float4 *__restrict__ spos, const uint32_t *__restrict__ skeys,
     const uint32_t *__restrict__ perm, const uint32_t *__restrict__ off,
@@ -163,13 +179,14 @@ __global__ __launch_bounds__(kThreads) void k_pp32(
     uint32_t *__restrict__ counters, float4 *__restrict__ acc,
     unsigned long long *__restrict__ cand)
 {
-    __shared__ float s_tab[kTabPadded];
+    __shared__ float s_tab[kTab];
     __shared__ uint32_t s_rs[kWaves][kRanges];
     __shared__ uint32_t s_rp[kWaves][kRanges + 1];
     __shared__ uint32_t s_rq[kWaves][kRanges];
     __shared__ float4 s_j[kWaves][kJBuf];
 
-    for (int k = threadIdx.x; k < kTabPadded; k += kThreads) s_tab[k] = table32[k];
+    for (int k = threadIdx.x; k < kTab; k += kThreads)
+        s_tab[k] = k < kTabPadded ? (UNIFORM ? pp.mass * table32[k] : table32[k]) : 0.0f;
     __syncthreads();
 
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -199,9 +216,9 @@ __global__ __launch_bounds__(kThreads) void k_pp32(
         key_cell(g, skeys[i0], c0x, c0y, c0z);
         key_cell(g, skeys[i], cx, cy, cz);
         const float4 pi = spos[i];
-        const float xi = pi.x + (float)unwrap_delta(cx - c0x, M) * pp.cellf;
-        const float yi = pi.y + (float)unwrap_delta(cy - c0y, M) * pp.cellf;
-        const float zi = pi.z + (float)unwrap_delta(cz - c0z, M) * pp.cellf;
+        const float xi = pi.x * pp.s + (float)unwrap_delta(cx - c0x, M) * pp.cellf;
+        const float yi = pi.y * pp.s + (float)unwrap_delta(cy - c0y, M) * pp.cellf;
+        const float zi = pi.z * pp.s + (float)unwrap_delta(cz - c0z, M) * pp.cellf;
 
         const float bminx = uni(wave_min(valid ? xi : inf));
         const float bminy = uni(wave_min(valid ? yi : inf));
@@ -294,8 +311,9 @@ __global__ __launch_bounds__(kThreads) void k_pp32(
                     int qx, qy, qz;
                     unpack_q(rq[lo], qx, qy, qz);
                     const float4 pj = spos[j];
-                    qj = make_float4(pj.x + qx * pp.cellf, pj.y + qy * pp.cellf,
-                                     pj.z + qz * pp.cellf, pj.w);
+                    qj = make_float4(pj.x * pp.s + qx * pp.cellf,
+                                     pj.y * pp.s + qy * pp.cellf,
+                                     pj.z * pp.s + qz * pp.cellf, pj.w);
                     const float gx = fmaxf(0.f, fmaxf(bminx - qj.x, qj.x - bmaxx));
                     const float gy = fmaxf(0.f, fmaxf(bminy - qj.y, qj.y - bmaxy));
                     const float gz = fmaxf(0.f, fmaxf(bminz - qj.z, qj.z - bmaxz));
@@ -307,11 +325,11 @@ __global__ __launch_bounds__(kThreads) void k_pp32(
                 wave_sync();
                 if (jc >= 64) {
                     float sx = 0.f, sy = 0.f, sz = 0.f;
-                    interact<MASK>(sj, 64, pp, s_tab, xi, yi, zi, sx, sy, sz);
+                    interact<GENERIC, UNIFORM>(sj, 64, pp, s_tab, xi, yi, zi, sx, sy, sz);
                     kahan_add(ax, kx, sx);
                     kahan_add(ay, ky, sy);
                     kahan_add(az, kz, sz);
-                    if (COUNT) ncand += 64ull * cnt;
+                    if (cand) ncand += 64ull * cnt;
                     const uint32_t rem = jc - 64;
                     float4 tmp = make_float4(0.f, 0.f, 0.f, 0.f);
                     if ((uint32_t)lane < rem) tmp = sj[64 + lane];
@@ -324,16 +342,18 @@ __global__ __launch_bounds__(kThreads) void k_pp32(
         }
         if (jc > 0) {
             float sx = 0.f, sy = 0.f, sz = 0.f;
-            interact<MASK>(sj, (int)jc, pp, s_tab, xi, yi, zi, sx, sy, sz);
+            interact<GENERIC, UNIFORM>(sj, (int)jc, pp, s_tab, xi, yi, zi, sx, sy, sz);
             kahan_add(ax, kx, sx);
             kahan_add(ay, ky, sy);
             kahan_add(az, kz, sz);
-            if (COUNT) ncand += (unsigned long long)jc * cnt;
+            if (cand) ncand += (unsigned long long)jc * cnt;
             jc = 0;
         }
         wave_sync();
-        if (valid) acc[perm[i]] = make_float4(ax * pp.G, ay * pp.G, az * pp.G, 0.f);
-        if (COUNT) atomicAdd(cand, lane == 0 ? ncand : 0ull);
+        if (valid)
+            acc[perm[i]] = make_float4(ax * pp.out_scale, ay * pp.out_scale,
+                                       az * pp.out_scale, 0.f);
+        if (cand) atomicAdd(cand, lane == 0 ? ncand : 0ull);
     }
 }
 
@@ -396,37 +416,46 @@ int launch_pp32(const float4 *spos, const uint32_t *skeys,
                 const GridInfo &g, const PairParams &pp, const float *table32,
                 const TileInfo &t, float4 *acc,
                 unsigned long long *cand_counter, int num_workgroups,
-                hipStream_t s)
+                double uniform_mass, hipStream_t s)
 {
     if (n == 0) return CUCO_OK;
+    const double sc = pp.asmthfac;
     P32 p;
-    p.rcut = (float)pp.rcut;
-    p.rcut2 = (float)pp.rcut2;
-    p.filter2 = (float)(pp.rcut2 * (1.0 + 1e-4));
-    p.asmthfac = (float)pp.asmthfac;
-    p.h = (float)pp.h;
-    p.h_inv = (float)pp.h_inv;
-    p.h3_inv = (float)pp.h3_inv;
-    p.G = (float)pp.G;
-    p.cellf = (float)g.cell;
-    p.inv_cellf = (float)(1.0 / g.cell);
+    p.s = (float)sc;
+    p.rcut = (float)(pp.rcut * sc);
+    p.rcut2 = (float)(pp.rcut2 * sc * sc);
+    p.filter2 = (float)(pp.rcut2 * sc * sc * (1.0 + 1e-4));
+    p.h = (float)(pp.h * sc);
+    p.h_inv = (float)(1.0 / (pp.h * sc));
+    p.h3_inv = p.h_inv * p.h_inv * p.h_inv;
+    p.cellf = (float)(g.cell * sc);
+    p.inv_cellf = (float)(1.0 / (g.cell * sc));
+    p.eps = 1e-12f;
+    p.out_scale = (float)(pp.G * sc * sc);
+    p.mass = (float)uniform_mass;
     p.M = g.M;
-    // The cut can ride on the table when rcut sits on a table bin edge (the
-    // reference's RCUT/ASMTH = 4.5 puts it on bin 750).
+    // The table alone can make the cut when rcut sits on a bin edge and no
+    // candidate can reach past the LDS table: a candidate lies within rcut
+    // (x 1.0001) of a tile's box, and the box spans at most one Morton block
+    // (plus the clamp of the last cell), so r <= rcut + sqrt(3)(2^shift+1) cell.
     const double edge = pp.asmthfac * pp.rcut;
     const bool aligned = fabs(edge - nearbyint(edge)) < 1e-9 * edge;
+    const double rmax = sc * (pp.rcut * 1.001 +
+                              sqrt(3.0) * ((1 << t.block_shift) + 1) * g.cell);
+    const bool generic = !(aligned && rmax < kTab - 2);
+    const bool uniform = uniform_mass > 0;
     int rc = launch_fill_u32(t.counters + 1, 0u, 1, s);
     if (rc) return rc;
     const int nwg = num_workgroups > 0 ? num_workgroups : 2048;
-#define CUCO_PP32(MASK, COUNT)                                                 \
-    k_pp32<MASK, COUNT><<<nwg, kThreads, 0, s>>>(spos, skeys, perm, off, g, p, \
-                                                 table32, t.tile_start,        \
-                                                 t.tile_count, t.counters, acc, \
-                                                 cand_counter)
-    if (aligned) {
-        if (cand_counter) CUCO_PP32(false, true); else CUCO_PP32(false, false);
+#define CUCO_PP32(GENERIC, UNIFORM)                                             \
+    k_pp32<GENERIC, UNIFORM><<<nwg, kThreads, 0, s>>>(spos, skeys, perm, off, g, \
+                                                      p, table32, t.tile_start,  \
+                                                      t.tile_count, t.counters,  \
+                                                      acc, cand_counter)
+    if (generic) {
+        if (uniform) CUCO_PP32(true, true); else CUCO_PP32(true, false);
     } else {
-        if (cand_counter) CUCO_PP32(true, true); else CUCO_PP32(true, false);
+        if (uniform) CUCO_PP32(false, true); else CUCO_PP32(false, false);
     }
 #undef CUCO_PP32
     CUCO_LAUNCH_CHECK();

@@ -13,6 +13,8 @@
 // scan over tiles, and a downsweep that ranks every key inside its tile with
 // a wave-level multisplit (eight 64-lane ballots), reorders the tile through
 // LDS by digit, and writes each digit run contiguously.
+#include <algorithm>
+
 #include "internal.h"
 
 namespace cuco {
@@ -188,14 +190,40 @@ __global__ __launch_bounds__(kThreads) void k_rs_downsweep(
     }
 }
 
-__global__ void k_cell_offsets(const uint32_t *__restrict__ skeys, uint32_t n,
-                               uint32_t nkeys, uint32_t *__restrict__ off)
+// Each i in [0, n] owns the keys (key[i-1], key[i]] and writes off[c] = i
+// for them (key[-1] = -1, key[n] = nkeys): every entry is written exactly
+// once.  The key gaps are uneven (voids), so a wave concatenates the ranges
+// of its 64 lanes and fills them cooperatively.
+__global__ __launch_bounds__(256) void k_cell_offsets(
+    const uint32_t *__restrict__ skeys, uint32_t n, uint32_t nkeys,
+    uint32_t *__restrict__ off)
 {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i > n) return;
-    int64_t prev = (i == 0) ? -1 : (int64_t)skeys[i - 1];
-    int64_t cur = (i == n) ? (int64_t)nkeys : (int64_t)skeys[i];
-    for (int64_t c = prev + 1; c <= cur; ++c) off[c] = i;
+    __shared__ uint32_t s_pre[4][65];
+    __shared__ uint32_t s_lo[4][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    uint32_t lo = 0, len = 0;
+    if (i <= n) {
+        const int64_t prev = (i == 0) ? -1 : (int64_t)skeys[i - 1];
+        const int64_t cur = (i == n) ? (int64_t)nkeys : (int64_t)skeys[i];
+        lo = (uint32_t)(prev + 1);
+        len = cur > prev ? (uint32_t)(cur - prev) : 0u;
+    }
+    uint32_t inc = wave_incl_scan(len);
+    s_pre[w][lane] = inc - len;
+    s_lo[w][lane] = lo;
+    if (lane == 63) s_pre[w][64] = inc;
+    __syncthreads();
+    const uint32_t total = s_pre[w][64];
+    const uint32_t first = (blockIdx.x * 256u) + (uint32_t)(w * 64);
+    for (uint32_t f = lane; f < total; f += 64) {
+        uint32_t a = 0, b = 64;   // s_pre[w][a] <= f < s_pre[w][b]
+        while (b - a > 1) {
+            const uint32_t mid = (a + b) >> 1;
+            if (s_pre[w][mid] <= f) a = mid; else b = mid;
+        }
+        off[s_lo[w][a] + (f - s_pre[w][a])] = first + a;
+    }
 }
 
 __global__ void k_cell_start_end(const uint32_t *__restrict__ off,
@@ -295,6 +323,14 @@ __global__ void k_iota(uint32_t *p, uint32_t n)
     if (i < n) p[i] = i;
 }
 
+__global__ void k_copy16(uint4 *__restrict__ dst, const uint4 *__restrict__ src,
+                         size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 __global__ void k_fill_u32(uint32_t *p, uint32_t value, size_t n)
 {
     size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
@@ -360,6 +396,7 @@ int launch_cell_offsets(const uint32_t *skeys, uint32_t n, uint32_t nkeys,
 {
     k_cell_offsets<<<grid_for((int64_t)n + 1, 256), 256, 0, s>>>(skeys, n,
                                                                   nkeys, off);
+    // (n + 1 owners; the last block's lanes past n own nothing.)
     CUCO_LAUNCH_CHECK();
     return CUCO_OK;
 }
@@ -408,6 +445,17 @@ int exclusive_scan_u32(const uint32_t *in, uint32_t *out, uint32_t n,
     k_scan_sums<<<1, kThreads, 0, s>>>(scratch, nb, total);
     CUCO_LAUNCH_CHECK();
     k_scan_apply<<<nb, kThreads, 0, s>>>(in, out, n, scratch);
+    CUCO_LAUNCH_CHECK();
+    return CUCO_OK;
+}
+
+int launch_copy16(void *dst, const void *src, size_t bytes, hipStream_t s)
+{
+    const size_t n = bytes / 16;
+    if (n == 0) return CUCO_OK;
+    const unsigned grid = (unsigned)std::min<size_t>((n + 255) / 256, 8192);
+    k_copy16<<<grid, 256, 0, s>>>(static_cast<uint4 *>(dst),
+                                  static_cast<const uint4 *>(src), n);
     CUCO_LAUNCH_CHECK();
     return CUCO_OK;
 }

@@ -44,7 +44,7 @@ EXPORTED = [
     "cuco_get_cell_table", "cuco_get_shortrange_table", "cuco_get_pm_grid",
     "cuco_count_pairs", "cuco_set_timing", "cuco_get_stage_times",
     "cuco_stage_name", "cuco_synchronize", "cuco_init_drift_table",
-    "cuco_table_factor", "cuco_kdk_factors",
+    "cuco_table_factor", "cuco_kdk_factors", "cuco_copy_device_to_device",
 ]
 
 
@@ -120,6 +120,7 @@ def lib():
         "cuco_device_free": (C.c_int, [vp]),
         "cuco_copy_to_device": (C.c_int, [vp, vp, C.c_size_t]),
         "cuco_copy_from_device": (C.c_int, [vp, vp, C.c_size_t]),
+        "cuco_copy_device_to_device": (C.c_int, [vp, vp, C.c_size_t]),
         "cuco_radix_sort": (C.c_int, [vp, vp, C.c_uint32, C.c_uint32]),
         "cuco_fft3d_r2c": (C.c_int, [i32, i32, vp, vp]),
         "cuco_fft3d_c2r": (C.c_int, [i32, i32, vp, vp]),
@@ -212,6 +213,10 @@ class DeviceBuffer:
                                            self.ptr, arr.nbytes),
                "cuco_copy_from_device")
         return arr
+
+    def copy_from(self, other: "DeviceBuffer", nbytes: int) -> None:
+        _check(lib().cuco_copy_device_to_device(self.ptr, other.ptr, nbytes),
+               "cuco_copy_device_to_device")
 
     def free(self) -> None:
         if self.ptr:

@@ -179,6 +179,10 @@ int cuco_device_alloc(void **dptr, size_t bytes);
 int cuco_device_free(void *dptr);
 int cuco_copy_to_device(void *dst, const void *src, size_t bytes);
 int cuco_copy_from_device(void *dst, const void *src, size_t bytes);
+/* Device-to-device copy with the engine's own streaming kernel (16-byte
+ * vector loads/stores; bytes must be a multiple of 16).  Its counters give
+ * the HBM bytes of a copy of known size. */
+int cuco_copy_device_to_device(void *dst, const void *src, size_t bytes);
 /* RadixSort::sort(keys, values, n, keyBits) (radixsort.h:170-187): stable
  * key/value sort of n uint32 pairs on the device, ordering by bits
  * [0, key_bits) of the key.  values may be NULL (keys only).  Scratch is
