@@ -414,10 +414,13 @@ __global__ __launch_bounds__(256) void k_cic_segments(
             v[c] += take ? t : (T)0;
         }
     }
+    // Both shuffles run on every lane: a shuffle inside a short-circuit
+    // condition would read an inactive source lane.
     const uint32_t knext = __shfl_down(key, 1, 64);
+    const uint32_t kprev = __shfl_up(key, 1, 64);
     const bool seg_end = valid && (lane == 63 || knext != key);
     // First lane of this lane's segment inside the wave.
-    const uint64_t heads = __ballot(lane == 0 || __shfl_up(key, 1, 64) != key);
+    const uint64_t heads = __ballot(lane == 0 || kprev != key);
     const uint64_t below = heads & ((lane == 63) ? ~0ull : ((2ull << lane) - 1));
     const int start = 63 - __clzll(below);
     const uint32_t prev_key = base > 0 ? skeys[base - 1] : 0xffffffffu;

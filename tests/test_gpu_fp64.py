@@ -145,3 +145,28 @@ def test_errors_are_reported_not_fatal(lattice):
     eng.upload(pos[:100], None, mass[:100])
     with pytest.raises(E.CucoError):
         eng.compute_accelerations(7)
+
+
+@pytest.mark.parametrize("precision", [E.FP64, E.FP32])
+def test_deposit_cells_spanning_many_waves(orc, precision):
+    """Dense cells hold hundreds of particles (their sorted run crosses many
+    64-particle waves) and pairs sit on wave boundaries; the mesh density must
+    still match the oracle's direct CIC deposit."""
+    box = ics.small_box("dense", 16, 32, 40.0, 8)
+    rng = np.random.default_rng(8)
+    cell = box.L / box.M
+    blobs = [np.array([3.5, 7.5, 11.5]) * cell, np.array([20.2, 0.1, 31.9]) * cell]
+    pos = np.concatenate([
+        rng.uniform(0, box.L, (3000, 3)),
+        blobs[0] + rng.uniform(-0.45, 0.45, (700, 3)) * cell,
+        blobs[1] + rng.uniform(-0.05, 0.05, (333, 3)) * cell,
+    ]) % box.L
+    mass = rng.uniform(0.5, 1.5, len(pos))
+    eng = make_engine(box, capacity=len(pos))
+    eng.upload(pos, None, mass)
+    eng.pmforce_periodic(precision)
+    rho = eng.pm_grid(0)
+    ref = orc.pm_deposit(oracle_params(box), pos, mass)
+    tol = 1e-12 if precision == E.FP64 else 2e-6
+    assert np.abs(rho - ref).max() <= tol * np.abs(ref).max()
+    assert rho.sum() == pytest.approx(mass.sum(), rel=1e-12 if precision == E.FP64 else 1e-5)
