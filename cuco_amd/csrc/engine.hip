@@ -7,6 +7,7 @@
 // (gravlist.cpp:148-193) is key -> sort -> cell table -> force;
 // pmforce_periodic (pm_periodic.c:67-314) is deposit -> r2c -> Green -> c2r
 // -> gradient/interpolation.
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstring>
@@ -100,6 +101,9 @@ struct cuco_engine {
     PairParams pp{};
     TileInfo tiles;
     int pp32_workgroups = 0;
+    int pp32_wg_per_cu = 8;
+    int ncu = 256;
+    int min_block_shift = 2;
 
     void *rho[2] = {nullptr, nullptr};
     void *phi[2] = {nullptr, nullptr};
@@ -547,8 +551,10 @@ int cuco_create(const cuco_params *params, int64_t max_particles, int32_t device
     ALLOC(e->table, sizeof(float) * kNTab);
     ALLOC(e->table32, sizeof(float) * kTabPadded);
     ALLOC(e->dcount, sizeof(unsigned long long) * 2);
+    // Tile arrays sized for the smallest tunable Morton block (most tiles).
     e->tiles.block_shift = gm.bits >= 4 ? 3 : gm.bits - 1;
-    e->tiles.nblocks = gm.nkeys >> (3 * e->tiles.block_shift);
+    e->min_block_shift = std::min(2, e->tiles.block_shift);
+    e->tiles.nblocks = gm.nkeys >> (3 * e->min_block_shift);
     e->tiles.max_tiles = (e->capacity + 63) / 64 + e->tiles.nblocks;
     ALLOC(e->tiles.tile_start, sizeof(uint32_t) * e->tiles.max_tiles);
     ALLOC(e->tiles.tile_count, sizeof(uint32_t) * e->tiles.max_tiles);
@@ -568,7 +574,8 @@ int cuco_create(const cuco_params *params, int64_t max_particles, int32_t device
         int ncu = 0;
         if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || ncu <= 0)
             ncu = 256;
-        e->pp32_workgroups = ncu * 8;
+        e->ncu = ncu;
+        e->pp32_workgroups = ncu * e->pp32_wg_per_cu;
     }
     if (rc != CUCO_OK) {
         std::string msg = g_last_error;
@@ -1046,6 +1053,90 @@ int cuco_count_pairs(cuco_engine *e, uint64_t *pairs_in, uint64_t *fp32_candidat
     CUCO_HIP(hipStreamSynchronize(e->stream));
     if (pairs_in) *pairs_in = h[0];
     if (fp32_candidates) *fp32_candidates = h[1];
+    return CUCO_OK;
+}
+
+int cuco_get_tuning(cuco_engine *e, cuco_tuning *t)
+{
+    CHECK_ENGINE(e);
+    if (!t) {
+        set_error("null tuning");
+        return CUCO_ERR_INVALID_ARG;
+    }
+    t->pp_block_shift = e->tiles.block_shift;
+    t->pp_workgroups_per_cu = e->pp32_wg_per_cu;
+    t->reserved0 = t->reserved1 = 0;
+    return CUCO_OK;
+}
+
+int cuco_set_tuning(cuco_engine *e, const cuco_tuning *t)
+{
+    CHECK_ENGINE(e);
+    if (!t || t->pp_block_shift < e->min_block_shift || t->pp_block_shift > 4 ||
+        t->pp_block_shift >= e->gm.bits || t->pp_workgroups_per_cu < 1 ||
+        t->pp_workgroups_per_cu > 32) {
+        set_error("tuning out of range");
+        return CUCO_ERR_INVALID_ARG;
+    }
+    if (t->pp_block_shift != e->tiles.block_shift) {
+        e->tiles.block_shift = t->pp_block_shift;
+        if (e->sorted_mode == kKeyMorton) e->sorted_mode = -1;  // rebuild tiles
+        destroy_graph(e);
+    }
+    if (t->pp_workgroups_per_cu != e->pp32_wg_per_cu) {
+        e->pp32_wg_per_cu = t->pp_workgroups_per_cu;
+        e->pp32_workgroups = e->ncu * e->pp32_wg_per_cu;
+        destroy_graph(e);
+    }
+    return CUCO_OK;
+}
+
+int cuco_autotune(cuco_engine *e, cuco_tuning *chosen, double *best_ms)
+{
+    CHECK_ENGINE(e);
+    TRY(check_ready(e));
+    hipEvent_t a, b;
+    CUCO_HIP(hipEventCreate(&a));
+    CUCO_HIP(hipEventCreate(&b));
+    const bool timing = e->timing;
+    e->timing = false;
+    cuco_tuning best{e->tiles.block_shift, e->pp32_wg_per_cu, 0, 0};
+    float best_t = 1e30f;
+    int rc = CUCO_OK;
+    for (int shift = e->min_block_shift; shift <= std::min(4, e->gm.bits - 1) && !rc; ++shift) {
+        for (int wg : {4, 8, 16}) {
+            cuco_tuning t{shift, wg, 0, 0};
+            rc = cuco_set_tuning(e, &t);
+            if (!rc) rc = sort_particles(e, kKeyMorton);
+            float ms = 1e30f;
+            for (int rep = 0; rep < 2 && !rc; ++rep) {
+                if (hipEventRecord(a, e->stream) != hipSuccess) rc = CUCO_ERR_HIP;
+                if (!rc) rc = launch_pp32(e->spos32, e->skeys, e->perm, e->off, e->n, e->gm, e->pp,
+                                          e->table32, e->tiles, e->acc_pp32, nullptr,
+                                          e->pp32_workgroups, e->uniform_mass, e->stream);
+                if (!rc && (hipEventRecord(b, e->stream) != hipSuccess ||
+                            hipEventSynchronize(b) != hipSuccess)) rc = CUCO_ERR_HIP;
+                float t_ms = 0;
+                if (!rc && hipEventElapsedTime(&t_ms, a, b) == hipSuccess) ms = std::min(ms, t_ms);
+            }
+            if (!rc && ms < best_t) {
+                best_t = ms;
+                best = t;
+            }
+            if (rc) break;
+        }
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    e->timing = timing;
+    if (rc) {
+        if (rc == CUCO_ERR_HIP && g_last_error.empty()) set_error("autotune timing failed");
+        return rc;
+    }
+    TRY(cuco_set_tuning(e, &best));
+    e->pp_prec = CUCO_FP32;
+    if (chosen) *chosen = best;
+    if (best_ms) *best_ms = best_t;
     return CUCO_OK;
 }
 

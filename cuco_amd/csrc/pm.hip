@@ -460,31 +460,41 @@ __global__ __launch_bounds__(256) void k_cic_gather(
     const int y = by * kBy + ((t / kBz) % kBy);
     const int x = bx * kBx + t / (kBz * kBy);
     if (x >= M || y >= M || z >= M) return;
+    // Corner c = (a, bb, cc): the node is the cell's upper neighbour along
+    // the axes whose bit is set; cell = node - (a, bb, cc).
+    uint32_t cs[8], ce[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        const int cx = wrap_index(x - (c >> 2), M);
+        const int cy = wrap_index(y - ((c >> 1) & 1), M);
+        const int cz = wrap_index(z - (c & 1), M);
+        const uint32_t key = cell_key(g, cx, cy, cz);
+        cs[c] = off[key];
+        ce[c] = off[key + 1];
+    }
+    // Cells inside one wave: one load each, issued together (the index of an
+    // empty or straddling cell is redirected to a valid slot and masked).
+    T v[8];
+    bool straddle = false;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        const bool nonempty = ce[c] > cs[c];
+        const bool single = nonempty && (cs[c] >> 6) == ((ce[c] - 1) >> 6);
+        straddle |= nonempty && !single;
+        const T t = cell_sums[(single ? (size_t)cs[c] * 8 : 0) + c];
+        v[c] = single ? t : (T)0;
+    }
     double sum = 0;
 #pragma unroll
-    for (int a = 0; a < 2; ++a) {
-        const int cx = wrap_index(x - a, M);
-#pragma unroll
-        for (int bb = 0; bb < 2; ++bb) {
-            const int cy = wrap_index(y - bb, M);
-#pragma unroll
-            for (int c = 0; c < 2; ++c) {
-                const int cz = wrap_index(z - c, M);
-                const uint32_t key = cell_key(g, cx, cy, cz);
-                const uint32_t s = off[key], e = off[key + 1];
-                if (e == s) continue;
-                // Corner of this node seen from the cell: a, bb, c = 1 means
-                // the node is the cell's upper neighbour along that axis.
-                const int corner = (a << 2) | (bb << 1) | c;
-                const uint32_t w0 = s >> 6, w1 = (e - 1) >> 6;
-                if (w0 == w1) {
-                    sum += (double)cell_sums[(size_t)s * 8 + corner];
-                } else {
-                    sum += (double)last_part[(size_t)w0 * 8 + corner];
-                    for (uint32_t w = w0 + 1; w <= w1; ++w)
-                        sum += (double)first_part[(size_t)w * 8 + corner];
-                }
-            }
+    for (int c = 0; c < 8; ++c) sum += (double)v[c];
+    if (straddle) {
+        for (int c = 0; c < 8; ++c) {
+            if (ce[c] <= cs[c]) continue;
+            const uint32_t w0 = cs[c] >> 6, w1 = (ce[c] - 1) >> 6;
+            if (w0 == w1) continue;
+            sum += (double)last_part[(size_t)w0 * 8 + c];
+            for (uint32_t w = w0 + 1; w <= w1; ++w)
+                sum += (double)first_part[(size_t)w * 8 + c];
         }
     }
     rho[((size_t)x * M + y) * M + z] = (T)sum;

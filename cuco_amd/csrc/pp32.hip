@@ -30,7 +30,7 @@ namespace {
 
 constexpr int kWaves = 4;
 constexpr int kThreads = 64 * kWaves;
-constexpr int kRanges = 256;   // cell ranges per wave before the j stream
+constexpr int kRanges = 128;   // cell ranges per wave before the j stream
 constexpr int kJBuf = 128;     // staged j particles per wave
 constexpr int kTab = 4096;     // LDS erfc table; zero from the cut bin on
 
@@ -113,6 +113,20 @@ __device__ inline int unwrap_delta(int d, int M)
     return d;
 }
 
+// Newtonian-or-spline factor of one pair (gravlist_kernel.cu:47-56 with
+// 1/u^3 = h^3 rinv^3); fac enters as m*rinv^3.
+template <bool UNIFORM>
+__device__ inline float spline_fac(float r, float fac, float mj, const P32 &pp)
+{
+    const float u = r * pp.h_inv;
+    const float m = UNIFORM ? 1.0f : mj;
+    const float lo = m * pp.h3_inv * (10.66667f + u * u * (32.0f * u - 38.4f));
+    const float hi = m * pp.h3_inv *
+                         (21.33333f - 48.0f * u + 38.4f * u * u - 10.66667f * u * u * u) -
+                     0.06667f * fac;
+    return r < pp.h ? (u < 0.5f ? lo : hi) : fac;
+}
+
 // Pair sum of one lane against cnt staged j particles.
 //
 // GENERIC=false is the reference configuration: rcut sits on a table bin
@@ -120,45 +134,70 @@ __device__ inline int unwrap_delta(int d, int M)
 // and the host has checked that no candidate can reach bin kTab.  GENERIC
 // keeps the explicit r^2 <= rcut^2 mask and clamps the bin.  UNIFORM folds
 // the common particle mass into the LDS table.
+//
+// Candidates go four at a time: the four chains (LDS reads, rsq, table
+// lookup) are branch-free and interleave; the softened law is applied in
+// one wave-level branch per group, taken only when some lane of the wave
+// has a pair inside h.
 template <bool GENERIC, bool UNIFORM>
 __device__ inline void interact(const float4 *__restrict__ sj, int cnt,
                                 const P32 &pp, const float *__restrict__ tab,
                                 float xi, float yi, float zi, float &sx,
                                 float &sy, float &sz)
 {
-#pragma unroll 4
-    for (int k = 0; k < cnt; ++k) {
-        const float4 q = sj[k];
-        const float dx = q.x - xi;
-        const float dy = q.y - yi;
-        const float dz = q.z - zi;
-        // Self and coincident pairs: dx = dy = dz = 0, so eps only keeps rsq
-        // finite and the term is exactly zero (gravlist_kernel.cu:64-69).
+    int k = 0;
+    for (; k + 4 <= cnt; k += 4) {
+        float dx[4], dy[4], dz[4], r[4], fac[4], t[4], r2[4], mj[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const float4 p = sj[k + q];
+            dx[q] = p.x - xi;
+            dy[q] = p.y - yi;
+            dz[q] = p.z - zi;
+            mj[q] = p.w;
+            // Self and coincident pairs: dx = dy = dz = 0, eps keeps rsq
+            // finite and the term is exactly zero (gravlist_kernel.cu:64-69).
+            r2[q] = fmaf(dx[q], dx[q], fmaf(dy[q], dy[q], fmaf(dz[q], dz[q], pp.eps)));
+            const float rinv = __builtin_amdgcn_rsqf(r2[q]);
+            r[q] = r2[q] * rinv;
+            const float rinv3 = rinv * rinv * rinv;
+            fac[q] = UNIFORM ? rinv3 : mj[q] * rinv3;
+            int idx = (int)r[q];
+            if (GENERIC) idx = min(idx, kTab - 1);
+            t[q] = tab[idx];
+        }
+        const bool close = (r[0] < pp.h) | (r[1] < pp.h) | (r[2] < pp.h) | (r[3] < pp.h);
+        if (close) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) fac[q] = spline_fac<UNIFORM>(r[q], fac[q], mj[q], pp);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            float tv = t[q];
+            if (GENERIC) tv = r2[q] <= pp.rcut2 ? tv : 0.0f;
+            const float f = fac[q] * tv;
+            sx = fmaf(dx[q], f, sx);
+            sy = fmaf(dy[q], f, sy);
+            sz = fmaf(dz[q], f, sz);
+        }
+    }
+    for (; k < cnt; ++k) {
+        const float4 p = sj[k];
+        const float dx = p.x - xi, dy = p.y - yi, dz = p.z - zi;
         const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, pp.eps)));
         const float rinv = __builtin_amdgcn_rsqf(r2);
         const float r = r2 * rinv;
         const float rinv3 = rinv * rinv * rinv;
-        float fac = UNIFORM ? rinv3 : q.w * rinv3;
-        if (r < pp.h) {
-            // gravlist_kernel.cu:47-56 with 1/u^3 = h^3 rinv^3
-            const float u = r * pp.h_inv;
-            const float m = UNIFORM ? 1.0f : q.w;
-            if (u < 0.5f)
-                fac = m * pp.h3_inv * (10.66667f + u * u * (32.0f * u - 38.4f));
-            else
-                fac = m * pp.h3_inv *
-                          (21.33333f - 48.0f * u + 38.4f * u * u -
-                           10.66667f * u * u * u) -
-                      0.06667f * fac;
-        }
+        float fac = UNIFORM ? rinv3 : p.w * rinv3;
+        fac = spline_fac<UNIFORM>(r, fac, p.w, pp);
         int idx = (int)r;
         if (GENERIC) idx = min(idx, kTab - 1);
-        float t = tab[idx];
-        if (GENERIC) t = r2 <= pp.rcut2 ? t : 0.0f;
-        fac *= t;
-        sx += dx * fac;
-        sy += dy * fac;
-        sz += dz * fac;
+        float tv = tab[idx];
+        if (GENERIC) tv = r2 <= pp.rcut2 ? tv : 0.0f;
+        const float f = fac * tv;
+        sx = fmaf(dx, f, sx);
+        sy = fmaf(dy, f, sy);
+        sz = fmaf(dz, f, sz);
     }
 }
 
@@ -391,6 +430,7 @@ int launch_build_tiles(const uint32_t *off, uint32_t n, const GridInfo &g,
 {
     const int shift3 = 3 * t.block_shift;
     const uint32_t nblocks = g.nkeys >> shift3;
+    // t.nblocks is the capacity (blocks of the smallest tunable shift).
     if (nblocks == 0 || nblocks > t.nblocks) {
         set_error("tile build: %u Morton blocks exceed capacity %u", nblocks,
                   t.nblocks);

@@ -45,7 +45,13 @@ EXPORTED = [
     "cuco_count_pairs", "cuco_set_timing", "cuco_get_stage_times",
     "cuco_stage_name", "cuco_synchronize", "cuco_init_drift_table",
     "cuco_table_factor", "cuco_kdk_factors", "cuco_copy_device_to_device",
+    "cuco_get_tuning", "cuco_set_tuning", "cuco_autotune",
 ]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("pp_block_shift", C.c_int32), ("pp_workgroups_per_cu", C.c_int32),
+                ("reserved0", C.c_int32), ("reserved1", C.c_int32)]
 
 
 class CucoError(RuntimeError):
@@ -121,6 +127,9 @@ def lib():
         "cuco_copy_to_device": (C.c_int, [vp, vp, C.c_size_t]),
         "cuco_copy_from_device": (C.c_int, [vp, vp, C.c_size_t]),
         "cuco_copy_device_to_device": (C.c_int, [vp, vp, C.c_size_t]),
+        "cuco_get_tuning": (C.c_int, [vp, C.POINTER(Tuning)]),
+        "cuco_set_tuning": (C.c_int, [vp, C.POINTER(Tuning)]),
+        "cuco_autotune": (C.c_int, [vp, C.POINTER(Tuning), dp]),
         "cuco_radix_sort": (C.c_int, [vp, vp, C.c_uint32, C.c_uint32]),
         "cuco_fft3d_r2c": (C.c_int, [i32, i32, vp, vp]),
         "cuco_fft3d_c2r": (C.c_int, [i32, i32, vp, vp]),
@@ -411,6 +420,24 @@ class Engine:
         _check(lib().cuco_count_pairs(self.h, C.byref(a), C.byref(b)),
                "cuco_count_pairs")
         return int(a.value), int(b.value)
+
+    def tuning(self) -> dict:
+        t = Tuning()
+        _check(lib().cuco_get_tuning(self.h, C.byref(t)), "cuco_get_tuning")
+        return {"pp_block_shift": t.pp_block_shift,
+                "pp_workgroups_per_cu": t.pp_workgroups_per_cu}
+
+    def set_tuning(self, pp_block_shift: int, pp_workgroups_per_cu: int) -> None:
+        t = Tuning(pp_block_shift, pp_workgroups_per_cu, 0, 0)
+        _check(lib().cuco_set_tuning(self.h, C.byref(t)), "cuco_set_tuning")
+
+    def autotune(self):
+        """Picks the FP32 pair-kernel tuning by timing the candidates."""
+        t = Tuning()
+        ms = C.c_double(0)
+        _check(lib().cuco_autotune(self.h, C.byref(t), C.byref(ms)), "cuco_autotune")
+        return ({"pp_block_shift": t.pp_block_shift,
+                 "pp_workgroups_per_cu": t.pp_workgroups_per_cu}, ms.value)
 
     def set_timing(self, on: bool) -> None:
         _check(lib().cuco_set_timing(self.h, int(on)), "cuco_set_timing")

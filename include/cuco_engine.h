@@ -118,6 +118,27 @@ int cuco_compute_accelerations(cuco_engine *e, int32_t precision,
                                int32_t do_pm);
 
 /* ---------------------------------------------------------------- */
+/* Tuning (no reference counterpart: the reference hard-codes cell = */
+/* mesh cell and 256-thread launches, gravlist.cu:22-23)             */
+/* ---------------------------------------------------------------- */
+
+typedef struct cuco_tuning {
+    int32_t pp_block_shift;       /* FP32 tiles stay inside Morton blocks of
+                                     2^shift cells per dimension (2..4) */
+    int32_t pp_workgroups_per_cu; /* persistent FP32 pair-kernel workgroups
+                                     per compute unit (1..32) */
+    int32_t reserved0;
+    int32_t reserved1;
+} cuco_tuning;
+
+int cuco_get_tuning(cuco_engine *e, cuco_tuning *t);
+int cuco_set_tuning(cuco_engine *e, const cuco_tuning *t);
+/* Times the FP32 pair force on the current particles for every candidate
+ * tuning, keeps the fastest and returns it; *best_ms (optional) receives its
+ * kernel time.  Leaves the accelerations of the last trial. */
+int cuco_autotune(cuco_engine *e, cuco_tuning *chosen, double *best_ms);
+
+/* ---------------------------------------------------------------- */
 /* Integrator (device resident)                                      */
 /* ---------------------------------------------------------------- */
 

@@ -76,3 +76,26 @@ def test_fp32_is_deterministic_and_counts_cover_pairs():
     assert np.array_equal(a1[0], a2[0]) and np.array_equal(a1[1], a2[1])
     pairs, cand = eng.count_pairs()
     assert cand >= pairs > 0
+
+
+def test_every_tuning_gives_the_same_forces():
+    box = halo_box(seed=12)
+    pos, vel, mass = ics.generate(box)
+    eng = make_engine(box)
+    eng.upload(pos, None, mass)
+    eng.compute_accelerations(E.FP32, True)
+    ref = eng.accelerations()[0]
+    scale = np.linalg.norm(ref, axis=1).max()
+    for shift in (2, 3, 4):
+        for wg in (1, 4, 16):
+            eng.set_tuning(shift, wg)
+            eng.compute_accelerations(E.FP32, True)
+            a = eng.accelerations()[0]
+            assert np.abs(a - ref).max() <= 1e-5 * scale, (shift, wg)
+    best, ms = eng.autotune()
+    assert best["pp_block_shift"] in (2, 3, 4) and ms > 0
+    assert eng.tuning() == best
+    eng.compute_accelerations(E.FP32, True)
+    assert np.abs(eng.accelerations()[0] - ref).max() <= 1e-5 * scale
+    with pytest.raises(E.CucoError):
+        eng.set_tuning(9, 4)
