@@ -73,6 +73,75 @@ def cpu_baseline(box, pos, mass, n_targets, seed=5006):
     }
 
 
+def ensemble_bench(E, ics, nboxes, steps):
+    """The 64-box ensemble: FP32 KDK steps (each one hipGraph launch) of all
+    boxes batched in one handle against the same boxes run one after
+    another through single-box handles.  Both sides replay an already
+    captured graph from the same initial state; the final states must be
+    bit-identical."""
+    boxes = ics.ensemble_boxes(nboxes)
+    t0 = time.perf_counter()
+    data = [ics.generate(b) for b in boxes]
+    log(f"[bench] ensemble: {len(boxes)} boxes, {sum(b.N for b in boxes)} particles "
+        f"generated in {time.perf_counter() - t0:.1f}s")
+    b0 = boxes[0]
+    d, k = E.init_drift_table(b0.a_start, b0.a_end, ics.OMEGA_M, ics.OMEGA_L,
+                              ics.HUBBLE_INTERNAL)
+    f = E.kdk_factors(d, k, b0.a_start, b0.a_end, b0.a_start, b0.a_end, steps)
+    pos = np.concatenate([x[0] for x in data])
+    vel = np.concatenate([x[1] for x in data])
+    mass = np.concatenate([x[2] for x in data])
+
+    ens = E.Ensemble([b.engine_kwargs() for b in boxes], [b.N for b in boxes])
+
+    def run_batched():
+        ens.upload(pos, vel, mass)
+        ens.compute_accelerations(E.FP32, True)
+        ens.synchronize()
+        t = time.perf_counter()
+        ens.kdk_steps(f, E.FP32, use_graph=True)
+        return time.perf_counter() - t
+
+    run_batched()                       # captures the step graph
+    t_batched = run_batched()
+    bpos, _ = ens.download_particles()
+    ens.close()
+    log(f"[bench] ensemble batched {t_batched * 1e3 / steps:.2f} ms/step")
+
+    t_looped, identical = 0.0, True
+    for b, (box, (p, v, m)) in enumerate(zip(boxes, data)):
+        eng = E.Engine(max_particles=box.N, **box.engine_kwargs())
+        for rep in range(2):            # the first run captures the graph
+            eng.upload(p, v, m)
+            eng.compute_accelerations(E.FP32, True)
+            eng.synchronize()
+            t = time.perf_counter()
+            eng.kdk_steps(f, E.FP32, use_graph=True)
+            dt = time.perf_counter() - t
+        t_looped += dt
+        sp, _ = eng.download_particles()
+        o = sum(x.N for x in boxes[:b])
+        identical = identical and bool(np.array_equal(bpos[o:o + box.N], sp))
+        eng.close()
+    log(f"[bench] ensemble looped {t_looped * 1e3 / steps:.2f} ms/step, "
+        f"identical={identical}")
+    n_total = sum(b.N for b in boxes)
+    sides = sorted({b.n_side for b in boxes})
+    return {
+        "workload": (f"{len(boxes)} boxes, N_b in {{{', '.join(f'{s}^3' for s in sides)}}} "
+                     f"(seed 4004), mesh 2 N_b^(1/3), L_b = N_b^(1/3) x 3.125 Mpc/h, "
+                     f"Zel'dovich ICs; FP32 KDK steps, one hipGraph per step"),
+        "n_boxes": len(boxes),
+        "n_particles": n_total,
+        "steps": steps,
+        "batched_ms_per_step": t_batched * 1e3 / steps,
+        "looped_ms_per_step": t_looped * 1e3 / steps,
+        "batched_over_looped": t_looped / t_batched,
+        "batched_evals_per_s": n_total * steps / t_batched,
+        "bit_identical_to_looped": identical,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__,
                                  formatter_class=argparse.RawDescriptionHelpFormatter)
@@ -83,6 +152,9 @@ def main() -> int:
     ap.add_argument("--fp64-steps", type=int, default=2)
     ap.add_argument("--cpu-targets", type=int, default=100000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--ensemble-boxes", type=int, default=64,
+                    help="boxes of the batched-vs-looped ensemble run (0: skip)")
+    ap.add_argument("--ensemble-steps", type=int, default=20)
     ap.add_argument("--pmc-probe", action="store_true",
                     help="counter pass: one FP32 evaluation of the workload and "
                          "a 1 GiB device copy of known size, nothing else")
@@ -212,12 +284,15 @@ def main() -> int:
         "roofline": roofline,
         "cpu_baseline": None,
     }
+    eng.close()
+    if args.ensemble_boxes > 0:
+        out["ensemble"] = ensemble_bench(E, ics, args.ensemble_boxes,
+                                         args.ensemble_steps)
     if not args.no_cpu_baseline:
         t0 = time.perf_counter()
         out["cpu_baseline"] = cpu_baseline(box, pos, mass, args.cpu_targets)
         log(f"[bench] cpu baseline in {time.perf_counter() - t0:.1f}s")
     print(json.dumps(out), flush=True)
-    eng.close()
     return 0
 
 

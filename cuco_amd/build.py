@@ -39,7 +39,7 @@ COMMON = [
 # C reference writes them; only the FP32 pair kernel may fuse.
 CONTRACT = {"pp32.hip": "-ffp-contract=fast"}
 SOURCES = ["engine.hip", "sort.hip", "pp64.hip", "pp32.hip", "pm.hip",
-           "integrate.hip", "driftfac.cpp"]
+           "integrate.hip", "driftfac.cpp", "snapshot.cpp"]
 CXX = os.environ.get("CXX", "g++")
 
 

@@ -103,6 +103,65 @@ __host__ __device__ inline int wrap_index(int a, int M)
     return a < 0 ? a + M : a;
 }
 
+// ---------------------------------------------------------------- FFT plans
+// Stockham stages of a length-n transform: radix 4 first, then 2, 3, 5.
+struct FftPlan {
+    int nstages;
+    int radix[16];
+};
+
+inline FftPlan make_plan(int n)
+{
+    FftPlan p{};
+    int m = n;
+    while (m % 4 == 0 && p.nstages < 16) { p.radix[p.nstages++] = 4; m /= 4; }
+    while (m % 2 == 0 && p.nstages < 16) { p.radix[p.nstages++] = 2; m /= 2; }
+    while (m % 3 == 0 && p.nstages < 16) { p.radix[p.nstages++] = 3; m /= 3; }
+    while (m % 5 == 0 && p.nstages < 16) { p.radix[p.nstages++] = 5; m /= 5; }
+    if (m != 1) p.nstages = -1;
+    return p;
+}
+
+// ---------------------------------------------------------------- boxes
+// Every launch works on a set of independent periodic boxes (an ensemble; a
+// single simulation is a set of one).  Box b owns the particles [p0, p0+n)
+// in both the original and the sorted order (its keys form one contiguous
+// range starting at key_base), one real and one complex mesh, and a slice of
+// the twiddle and Green's function tables.
+struct BoxDev {
+    GridInfo g;         // mesh and key layout of the box
+    uint32_t key_base;  // first global key of the box
+    uint32_t p0, n;     // particle range
+    uint32_t wave0;     // first 64-particle deposit slot of the box
+    uint64_t grid0;     // first element of the real mesh (M^3)
+    uint64_t cgrid0;    // first element of the complex mesh (M*M*(M/2+1))
+    uint32_t tab0;      // first entry of the twiddle / Green tables (M each)
+    int32_t pad0;
+    double pm_fac;      // G/(pi L) * M/(2L) (pm_periodic.c:85-86)
+    FftPlan half, full; // plans for length M/2 (rows) and M (lines)
+};
+
+// What one launch sees: the boxes and the exclusive prefix over boxes of
+// the launch's work units (blocks, or tiles for the pair kernel).
+struct BoxView {
+    const BoxDev *box;
+    const uint32_t *map;  // nboxes+1 entries, map[0] = 0
+    int nboxes;
+};
+
+// Box holding work unit u (u < map[nboxes]): the last b with map[b] <= u.
+// Boxes without units are skipped.  Called with wave-uniform u.
+__device__ inline int find_box(const uint32_t *__restrict__ map, int nboxes,
+                               uint32_t u)
+{
+    int lo = 0, hi = nboxes;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (map[mid] <= u) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
 // ---------------------------------------------------------------- complex
 template <typename T>
 struct Cx {
@@ -148,6 +207,33 @@ void set_error(const char *fmt, ...);
 inline unsigned grid_for(int64_t n, int block)
 {
     return (unsigned)((n + block - 1) / block);
+}
+
+// Workgroups go to the 8 XCDs (each with its own L2) round-robin in
+// blockIdx order.  Launch xcd_grid(nb) blocks and map each to
+// xcd_block(nb): XCD k then works on the k-th contiguous eighth of the
+// logical blocks, so neighbouring blocks (which share cache lines) share an
+// L2.  Returns -1 for the padding blocks.
+constexpr unsigned kXcds = 8;
+inline unsigned xcd_grid(unsigned nb) { return (nb + kXcds - 1) / kXcds * kXcds; }
+__device__ inline long xcd_block(unsigned nb)
+{
+    const unsigned per = (nb + kXcds - 1) / kXcds;
+    const unsigned b = blockIdx.x;
+    const unsigned logical = (b % kXcds) * per + b / kXcds;
+    return logical < nb ? (long)logical : -1;
+}
+
+// Logical block (XCD-remapped) -> (box, block inside the box).  Returns
+// false for padding blocks.
+__device__ inline bool box_block(const BoxView &v, int &bi, uint32_t &lb)
+{
+    const uint32_t total = v.map[v.nboxes];
+    const long l = xcd_block(total);
+    if (l < 0) return false;
+    bi = find_box(v.map, v.nboxes, (uint32_t)l);
+    lb = (uint32_t)l - v.map[bi];
+    return true;
 }
 
 }  // namespace cuco

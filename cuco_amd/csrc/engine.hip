@@ -71,15 +71,36 @@ int dev_alloc(T *&p, size_t bytes)
 
 }  // namespace
 
+// Host view of one box: its parameters and particle slots.
+struct HostBox {
+    cuco_params prm{};
+    uint32_t cap = 0;  // particle slots
+    uint32_t n = 0;    // particles now
+    PairParams pp{};
+    GridInfo g[2]{};   // per KeyMode
+    double pm_fac = 0;
+};
+
 struct cuco_engine {
     int device = 0;
     hipStream_t stream = nullptr;
-    cuco_params prm{};
-    int M = 0;
+    bool ensemble = false;  // box sizes fixed at creation
+    std::vector<HostBox> box;
     uint32_t capacity = 0;
     uint32_t n = 0;
     bool have_particles = false;
     double uniform_mass = 0;  // > 0 when every particle has this mass
+
+    // Box descriptors and launch maps on the device, with host mirrors.
+    BoxSet bs;
+    std::vector<BoxDev> hdev[2];
+    std::vector<uint32_t> hmaps;
+    uint32_t nkeys[2] = {0, 0};
+    int key_bits[2] = {0, 0};
+    int min_bits = 0;  // smallest Morton bits per coordinate over boxes
+    size_t grid_elems = 0, cgrid_elems = 0, tab_elems = 0;
+    P32 *p32 = nullptr;  // per box
+    bool table_cut = false;
 
     double4 *pos = nullptr, *vel = nullptr;
     double4 *acc_pp64 = nullptr, *acc_pm64 = nullptr;
@@ -93,12 +114,10 @@ struct cuco_engine {
     int sorted_mode = -1;
     double4 *spos64 = nullptr;
     float4 *spos32 = nullptr;
-    GridInfo gx{}, gm{};
 
     float *table = nullptr;    // 1000 entries, gravlist.cpp:56-59
     float *table32 = nullptr;  // 1024 entries for the FP32 kernel
     float htable[kNTab];
-    PairParams pp{};
     TileInfo tiles;
     int pp32_workgroups = 0;
     int pp32_wg_per_cu = 8;
@@ -110,7 +129,16 @@ struct cuco_engine {
     void *cgrid[2] = {nullptr, nullptr};
     void *dep_scratch[2] = {nullptr, nullptr};
     PmTables tb[2];
-    double pm_fac = 0;
+
+    // Integer timeline (individual power-of-two timesteps).
+    int32_t *ti_beg = nullptr, *ti_end = nullptr;
+    int32_t *ti_scalars = nullptr;  // [0] min Ti_endstep, [1] error flags, [2] active count
+    double *kick_table = nullptr;
+    bool tl_init = false;
+    cuco_timeline tl{};
+    std::vector<double> drift_tab, kick_tab;
+    double tl_log_begin = 0, tl_log_max = 0, tl_interval = 0;
+    cuco_timeline_state tls{};
 
     unsigned long long *dcount = nullptr;
     double *dfactors = nullptr;
@@ -222,25 +250,11 @@ void make_table(float *t)
     }
 }
 
-int upload_tables(cuco_engine *e)
-{
-    make_table(e->htable);
-    float t32[kTabPadded];
-    const double edge = e->pp.asmthfac * e->pp.rcut;
-    const bool aligned = fabs(edge - nearbyint(edge)) < 1e-9 * edge;
-    const int kcut = aligned ? (int)nearbyint(edge) : kNTab;
-    for (int k = 0; k < kTabPadded; ++k)
-        t32[k] = (k < kNTab && k < kcut) ? e->htable[k] : 0.0f;
-    CUCO_HIP(hipMemcpy(e->table, e->htable, sizeof(float) * kNTab, hipMemcpyHostToDevice));
-    CUCO_HIP(hipMemcpy(e->table32, t32, sizeof(t32), hipMemcpyHostToDevice));
-    return CUCO_OK;
-}
-
-void set_pair_params(cuco_engine *e)
+void set_pair_params(HostBox &b)
 {
     // gravlist.cpp:36-40, :61-70
-    const cuco_params &p = e->prm;
-    PairParams &q = e->pp;
+    const cuco_params &p = b.prm;
+    PairParams &q = b.pp;
     q.L = p.box_size;
     q.boxhalf = p.box_size / 2.0;
     q.rcut = p.rcut;
@@ -252,15 +266,95 @@ void set_pair_params(cuco_engine *e)
     q.G = p.G;
     q.fac = p.pm_grid / p.box_size;
     q.M = p.pm_grid;
+    b.pm_fac = p.G / (M_PI * p.box_size);  // pm_periodic.c:85-86
+    b.pm_fac *= 1 / (2 * p.box_size / p.pm_grid);
 }
 
-template <typename T>
-int make_pm_tables(cuco_engine *e, PmTables &tb)
+// The FP32 table, the per-box FP32 pair constants and the choice between
+// the table cut and the explicit r^2 test.  The table is shared by all
+// boxes, so its zero tail starts at the cut bin only when every box puts
+// rcut on the same bin edge.
+int update_pair_constants(cuco_engine *e)
 {
-    const int M = e->M;
-    std::vector<Cx<T>> tw(M);
-    std::vector<T> ge(M), gs(M);
-    const double asmth2 = std::pow(2 * M_PI * e->prm.asmth / e->prm.box_size, 2);
+    make_table(e->htable);
+    bool same_edge = true;
+    double edge0 = e->box[0].pp.asmthfac * e->box[0].pp.rcut;
+    bool cut = true;
+    std::vector<P32> h;
+    for (const HostBox &b : e->box) {
+        const double edge = b.pp.asmthfac * b.pp.rcut;
+        same_edge = same_edge && fabs(edge - edge0) <= 1e-9 * edge0;
+        cut = cut && pp32_table_cut(b.pp, b.g[kKeyMorton], e->tiles.block_shift);
+        h.push_back(make_p32(b.pp, b.g[kKeyMorton], e->uniform_mass));
+    }
+    const bool aligned = fabs(edge0 - nearbyint(edge0)) < 1e-9 * edge0;
+    e->table_cut = cut && same_edge;
+    const int kcut = (same_edge && aligned) ? (int)nearbyint(edge0) : kNTab;
+    float t32[kTabPadded];
+    for (int k = 0; k < kTabPadded; ++k)
+        t32[k] = (k < kNTab && k < kcut) ? e->htable[k] : 0.0f;
+    CUCO_HIP(hipMemcpy(e->table, e->htable, sizeof(float) * kNTab, hipMemcpyHostToDevice));
+    CUCO_HIP(hipMemcpy(e->table32, t32, sizeof(t32), hipMemcpyHostToDevice));
+    CUCO_HIP(hipMemcpy(e->p32, h.data(), sizeof(P32) * h.size(), hipMemcpyHostToDevice));
+    return CUCO_OK;
+}
+
+// Box descriptors and block maps for the current particle counts.
+int update_boxes(cuco_engine *e)
+{
+    const int nb = (int)e->box.size();
+    const size_t stride = (size_t)nb + 1;
+    e->hmaps.assign(kMapCount * stride, 0u);
+    for (int m = 0; m < 2; ++m) e->hdev[m].assign(nb, BoxDev{});
+    uint32_t p0 = 0, wave0 = 0, kb[2] = {0, 0};
+    uint64_t grid0 = 0, cgrid0 = 0, tab0 = 0;
+    for (int b = 0; b < nb; ++b) {
+        const HostBox &h = e->box[b];
+        const int M = h.prm.pm_grid;
+        for (int m = 0; m < 2; ++m) {
+            BoxDev &d = e->hdev[m][b];
+            d.g = h.g[m];
+            d.key_base = kb[m];
+            d.p0 = p0;
+            d.n = h.n;
+            d.wave0 = wave0;
+            d.grid0 = grid0;
+            d.cgrid0 = cgrid0;
+            d.tab0 = (uint32_t)tab0;
+            d.pm_fac = h.pm_fac;
+            d.half = make_plan(M / 2);
+            d.full = make_plan(M);
+            kb[m] += h.g[m].nkeys;
+        }
+        uint32_t *mp = e->hmaps.data();
+        mp[kMapPart * stride + b + 1] = (h.n + 255) / 256;
+        for (int map = kMapGatherX; map < kMapCount; ++map)
+            mp[map * stride + b + 1] = pm_blocks(M, map);
+        p0 += h.n;
+        wave0 += (h.cap + 63) / 64;
+        grid0 += (uint64_t)M * M * M;
+        cgrid0 += (uint64_t)M * M * (M / 2 + 1);
+        tab0 += M;
+    }
+    for (int map = 0; map < kMapCount; ++map) {
+        uint32_t *mp = e->hmaps.data() + map * stride;
+        for (int b = 0; b < nb; ++b) mp[b + 1] += mp[b];
+        e->bs.total[map] = mp[nb];
+    }
+    for (int m = 0; m < 2; ++m)
+        CUCO_HIP(hipMemcpy(e->bs.dev[m], e->hdev[m].data(), sizeof(BoxDev) * nb,
+                           hipMemcpyHostToDevice));
+    CUCO_HIP(hipMemcpy(e->bs.maps, e->hmaps.data(), sizeof(uint32_t) * e->hmaps.size(),
+                       hipMemcpyHostToDevice));
+    return CUCO_OK;
+}
+
+// Twiddles and Green's function factors of every box, concatenated in box
+// order (box b at BoxDev::tab0).
+template <typename T>
+void fill_pm_tables(int M, double asmth, double L, Cx<T> *tw, T *ge, T *gs)
+{
+    const double asmth2 = std::pow(2 * M_PI * asmth / L, 2);
     for (int k = 0; k < M; ++k) {
         const double a = 2.0 * M_PI * k / M;
         tw[k] = {(T)cos(a), (T)-sin(a)};
@@ -274,18 +368,53 @@ int make_pm_tables(cuco_engine *e, PmTables &tb)
         const double ff = 1.0 / f;
         gs[k] = (T)(ff * ff * ff * ff);
     }
+}
+
+template <typename T>
+int upload_pm_tables(const std::vector<int> &Ms, const std::vector<double> &asmth,
+                     const std::vector<double> &L, PmTables &tb)
+{
+    size_t total = 0;
+    for (int M : Ms) total += M;
+    std::vector<Cx<T>> tw(total);
+    std::vector<T> ge(total), gs(total);
+    size_t o = 0;
+    for (size_t b = 0; b < Ms.size(); ++b) {
+        fill_pm_tables<T>(Ms[b], asmth[b], L[b], tw.data() + o, ge.data() + o, gs.data() + o);
+        o += Ms[b];
+    }
     Cx<T> *dtw = nullptr;
     T *dge = nullptr, *dgs = nullptr;
-    TRY(dev_alloc(dtw, sizeof(Cx<T>) * M));
-    TRY(dev_alloc(dge, sizeof(T) * M));
-    TRY(dev_alloc(dgs, sizeof(T) * M));
-    CUCO_HIP(hipMemcpy(dtw, tw.data(), sizeof(Cx<T>) * M, hipMemcpyHostToDevice));
-    CUCO_HIP(hipMemcpy(dge, ge.data(), sizeof(T) * M, hipMemcpyHostToDevice));
-    CUCO_HIP(hipMemcpy(dgs, gs.data(), sizeof(T) * M, hipMemcpyHostToDevice));
+    TRY(dev_alloc(dtw, sizeof(Cx<T>) * total));
     tb.tw = dtw;
+    TRY(dev_alloc(dge, sizeof(T) * total));
     tb.gexp = dge;
+    TRY(dev_alloc(dgs, sizeof(T) * total));
     tb.gsinc = dgs;
+    CUCO_HIP(hipMemcpy(dtw, tw.data(), sizeof(Cx<T>) * total, hipMemcpyHostToDevice));
+    CUCO_HIP(hipMemcpy(dge, ge.data(), sizeof(T) * total, hipMemcpyHostToDevice));
+    CUCO_HIP(hipMemcpy(dgs, gs.data(), sizeof(T) * total, hipMemcpyHostToDevice));
     return CUCO_OK;
+}
+
+void free_pm_tables(PmTables &tb)
+{
+    free_ptr(tb.tw);
+    free_ptr(tb.gexp);
+    free_ptr(tb.gsinc);
+}
+
+template <typename T>
+int make_pm_tables(cuco_engine *e, PmTables &tb)
+{
+    std::vector<int> Ms;
+    std::vector<double> as, Ls;
+    for (const HostBox &b : e->box) {
+        Ms.push_back(b.prm.pm_grid);
+        as.push_back(b.prm.asmth);
+        Ls.push_back(b.prm.box_size);
+    }
+    return upload_pm_tables<T>(Ms, as, Ls, tb);
 }
 
 template <typename T>
@@ -293,11 +422,11 @@ int ensure_pm(cuco_engine *e)
 {
     const int k = sizeof(T) == 8 ? 0 : 1;
     if (e->cgrid[k]) return CUCO_OK;
-    const size_t M = e->M;
-    TRY(dev_alloc(e->rho[k], sizeof(T) * M * M * M));
-    TRY(dev_alloc(e->phi[k], sizeof(T) * M * M * M));
-    TRY(dev_alloc(e->cgrid[k], sizeof(Cx<T>) * M * M * (M / 2 + 1)));
-    TRY(dev_alloc(e->dep_scratch[k], pm_deposit_scratch_bytes(e->capacity, k == 0)));
+    TRY(dev_alloc(e->rho[k], sizeof(T) * e->grid_elems));
+    TRY(dev_alloc(e->phi[k], sizeof(T) * e->grid_elems));
+    TRY(dev_alloc(e->cgrid[k], sizeof(Cx<T>) * e->cgrid_elems));
+    TRY(dev_alloc(e->dep_scratch[k],
+                  pm_deposit_scratch_bytes(e->bs.capacity, e->bs.wave_slots, k == 0)));
     return make_pm_tables<T>(e, e->tb[k]);
 }
 
@@ -305,21 +434,20 @@ int ensure_pm(cuco_engine *e)
 int sort_particles(cuco_engine *e, int mode)
 {
     if (e->sorted_mode == mode) return CUCO_OK;
-    const GridInfo &g = mode == kKeyXMajor ? e->gx : e->gm;
     TRY(stage_start(e, CUCO_STAGE_KEYS));
-    TRY(launch_calc_keys(e->pos, e->n, g, e->keys, e->stream));
+    TRY(launch_calc_keys(e->pos, e->bs, mode, e->keys, e->stream));
     TRY(stage_stop(e, CUCO_STAGE_KEYS));
     TRY(stage_start(e, CUCO_STAGE_SORT));
-    TRY(radix_sort_pairs(e->keys, e->vals, true, e->n, g.key_bits, e->sc,
+    TRY(radix_sort_pairs(e->keys, e->vals, true, e->n, e->key_bits[mode], e->sc,
                          e->stream, &e->skeys, &e->perm));
     TRY(stage_stop(e, CUCO_STAGE_SORT));
     TRY(stage_start(e, CUCO_STAGE_CELLS));
-    TRY(launch_cell_offsets(e->skeys, e->n, g.nkeys, e->off, e->stream));
+    TRY(launch_cell_offsets(e->skeys, e->n, e->nkeys[mode], e->off, e->stream));
     if (mode == kKeyXMajor) {
         TRY(launch_reorder64(e->pos, e->perm, e->n, e->spos64, e->stream));
     } else {
-        TRY(launch_reorder32(e->pos, e->perm, e->skeys, e->n, g, e->spos32, e->stream));
-        TRY(launch_build_tiles(e->off, e->n, g, e->tiles, e->stream));
+        TRY(launch_reorder32(e->pos, e->perm, e->skeys, e->bs, mode, e->spos32, e->stream));
+        TRY(launch_build_tiles(e->off, e->bs, e->nkeys[mode], e->tiles, e->stream));
     }
     TRY(stage_stop(e, CUCO_STAGE_CELLS));
     e->sorted_mode = mode;
@@ -346,17 +474,23 @@ int check_prec(int32_t precision)
 
 int mode_of(int precision) { return precision == CUCO_FP64 ? kKeyXMajor : kKeyMorton; }
 
-int do_pp(cuco_engine *e, int precision, unsigned long long *counter)
+int do_pp(cuco_engine *e, int precision, unsigned long long *counter,
+          const int32_t *active_end = nullptr, int32_t ti_current = 0)
 {
     TRY(sort_particles(e, mode_of(precision)));
     TRY(stage_start(e, CUCO_STAGE_PP));
     if (precision == CUCO_FP64) {
-        TRY(launch_pp64(e->spos64, e->perm, e->off, e->n, e->pp, e->table,
-                        e->acc_pp64, counter, e->stream));
+        for (size_t b = 0; b < e->box.size(); ++b) {
+            const BoxDev &d = e->hdev[kKeyXMajor][b];
+            TRY(launch_pp64(e->spos64, e->perm, e->off + d.key_base, d.p0, d.n,
+                            e->box[b].pp, e->table, e->acc_pp64, counter,
+                            active_end, ti_current, e->stream));
+        }
     } else {
-        TRY(launch_pp32(e->spos32, e->skeys, e->perm, e->off, e->n, e->gm, e->pp,
-                        e->table32, e->tiles, e->acc_pp32, counter,
-                        e->pp32_workgroups, e->uniform_mass, e->stream));
+        TRY(launch_pp32(e->spos32, e->skeys, e->perm, e->off, e->bs, e->p32,
+                        e->table_cut, e->uniform_mass, e->table32, e->tiles,
+                        e->acc_pp32, counter, e->pp32_workgroups, active_end,
+                        ti_current, e->stream));
     }
     TRY(stage_stop(e, CUCO_STAGE_PP));
     e->pp_prec = precision;
@@ -367,28 +501,28 @@ template <typename T>
 int do_pm_t(cuco_engine *e, int precision)
 {
     TRY(ensure_pm<T>(e));
-    TRY(sort_particles(e, mode_of(precision)));
+    const int mode = mode_of(precision);
+    TRY(sort_particles(e, mode));
     const int k = sizeof(T) == 8 ? 0 : 1;
-    const GridInfo &g = precision == CUCO_FP64 ? e->gx : e->gm;
     const void *spos = precision == CUCO_FP64 ? (const void *)e->spos64 : (const void *)e->spos32;
     void *acc = precision == CUCO_FP64 ? (void *)e->acc_pm64 : (void *)e->acc_pm32;
     T *rho = static_cast<T *>(e->rho[k]);
     T *phi = static_cast<T *>(e->phi[k]);
     Cx<T> *c = static_cast<Cx<T> *>(e->cgrid[k]);
     TRY(stage_start(e, CUCO_STAGE_DEPOSIT));
-    TRY(pm_deposit<T>(spos, e->skeys, e->off, e->n, g, rho, e->dep_scratch[k], e->stream));
+    TRY(pm_deposit<T>(spos, e->skeys, e->off, e->bs, mode, rho, e->dep_scratch[k], e->stream));
     TRY(stage_stop(e, CUCO_STAGE_DEPOSIT));
     TRY(stage_start(e, CUCO_STAGE_FFT_FWD));
-    TRY(pm_fft_forward<T>(rho, c, e->M, e->tb[k], e->stream));
+    TRY(pm_fft_forward<T>(rho, c, e->bs, e->tb[k], e->stream));
     TRY(stage_stop(e, CUCO_STAGE_FFT_FWD));
     TRY(stage_start(e, CUCO_STAGE_GREEN));
-    TRY(pm_green_x<T>(c, e->M, e->tb[k], e->stream));
+    TRY(pm_green_x<T>(c, e->bs, e->tb[k], e->stream));
     TRY(stage_stop(e, CUCO_STAGE_GREEN));
     TRY(stage_start(e, CUCO_STAGE_FFT_INV));
-    TRY(pm_fft_inverse<T>(c, phi, e->M, e->tb[k], e->stream));
+    TRY(pm_fft_inverse<T>(c, phi, e->bs, e->tb[k], e->stream));
     TRY(stage_stop(e, CUCO_STAGE_FFT_INV));
     TRY(stage_start(e, CUCO_STAGE_INTERP));
-    TRY(pm_interp<T>(phi, spos, e->skeys, e->perm, e->n, g, e->pm_fac, acc, e->stream));
+    TRY(pm_interp<T>(phi, spos, e->skeys, e->perm, e->bs, mode, acc, e->stream));
     TRY(stage_stop(e, CUCO_STAGE_INTERP));
     e->pm_prec = precision;
     return CUCO_OK;
@@ -453,6 +587,70 @@ int wrap_host(double *x, double L)
     return CUCO_OK;
 }
 
+// A one-box set for the standalone 3D FFTs (unit box, no particles).
+struct FftBox {
+    BoxSet bs;
+    PmTables tb;
+    ~FftBox()
+    {
+        free_ptr(bs.dev[0]);
+        free_ptr(bs.maps);
+        free_pm_tables(tb);
+    }
+};
+
+template <typename T>
+int make_fft_box(int M, FftBox &f)
+{
+    BoxDev d{};
+    d.g.M = M;
+    d.g.L = 1.0;
+    d.g.fac = M;
+    d.g.cell = 1.0 / M;
+    d.half = make_plan(M / 2);
+    d.full = make_plan(M);
+    f.bs.nboxes = 1;
+    std::vector<uint32_t> maps(kMapCount * 2, 0u);
+    for (int map = kMapGatherX; map < kMapCount; ++map) maps[map * 2 + 1] = pm_blocks(M, map);
+    for (int map = 0; map < kMapCount; ++map) f.bs.total[map] = maps[map * 2 + 1];
+    TRY(dev_alloc(f.bs.dev[0], sizeof(BoxDev)));
+    f.bs.dev[1] = f.bs.dev[0];
+    TRY(dev_alloc(f.bs.maps, sizeof(uint32_t) * maps.size()));
+    CUCO_HIP(hipMemcpy(f.bs.dev[0], &d, sizeof(BoxDev), hipMemcpyHostToDevice));
+    CUCO_HIP(hipMemcpy(f.bs.maps, maps.data(), sizeof(uint32_t) * maps.size(), hipMemcpyHostToDevice));
+    return upload_pm_tables<T>({M}, {1.0}, {1.0}, f.tb);
+}
+
+template <typename T>
+int fft3d_single_t(int M, bool inverse, void *d_in, void *d_out)
+{
+    FftBox f;
+    TRY(make_fft_box<T>(M, f));
+    if (!inverse) {
+        TRY(pm_fft_forward<T>(static_cast<const T *>(d_in), static_cast<Cx<T> *>(d_out), f.bs, f.tb, nullptr));
+        TRY(pm_fft_x<T>(static_cast<Cx<T> *>(d_out), f.bs, false, f.tb, nullptr));
+    } else {
+        TRY(pm_fft_x<T>(static_cast<Cx<T> *>(d_in), f.bs, true, f.tb, nullptr));
+        TRY(pm_fft_inverse<T>(static_cast<Cx<T> *>(d_in), static_cast<T *>(d_out), f.bs, f.tb, nullptr));
+    }
+    if (hipDeviceSynchronize() != hipSuccess) {
+        set_error("FFT failed on the device");
+        return CUCO_ERR_HIP;
+    }
+    return CUCO_OK;
+}
+
+int fft3d_single(int M, int precision, bool inverse, void *d_in, void *d_out)
+{
+    TRY(check_prec(precision));
+    if (!fft_size_supported(M) || !d_in || !d_out) {
+        set_error("unsupported FFT size %d or null pointer", M);
+        return CUCO_ERR_UNSUPPORTED;
+    }
+    return precision == CUCO_FP64 ? fft3d_single_t<double>(M, inverse, d_in, d_out)
+                                  : fft3d_single_t<float>(M, inverse, d_in, d_out);
+}
+
 }  // namespace
 
 extern "C" {
@@ -472,14 +670,12 @@ const char *cuco_status_string(int status)
 
 const char *cuco_last_error(void) { return g_last_error.c_str(); }
 
-int cuco_create(const cuco_params *params, int64_t max_particles, int32_t device,
-                cuco_engine **out)
+}  // extern "C"
+
+namespace {
+
+int check_params(const cuco_params *params)
 {
-    if (!params || !out) {
-        set_error("null argument");
-        return CUCO_ERR_INVALID_ARG;
-    }
-    *out = nullptr;
     const int M = params->pm_grid;
     if (M < 16 || M > 1024 || !fft_size_supported(M)) {
         set_error("pm_grid %d unsupported (need 16..1024, even, factors 2,3,5)", M);
@@ -496,36 +692,76 @@ int cuco_create(const cuco_params *params, int64_t max_particles, int32_t device
                   params->box_size);
         return CUCO_ERR_INVALID_ARG;
     }
-    if (max_particles <= 0 || max_particles > 0x7fffffffLL) {
-        set_error("max_particles %lld out of range", (long long)max_particles);
-        return CUCO_ERR_INVALID_ARG;
-    }
-    CUCO_HIP(hipSetDevice(device));
-    cuco_engine *e = new cuco_engine();
-    e->device = device;
-    e->prm = *params;
-    e->M = M;
-    e->capacity = (uint32_t)max_particles;
-    set_pair_params(e);
+    return CUCO_OK;
+}
 
-    GridInfo &gx = e->gx;
+void set_grids(HostBox &b)
+{
+    const int M = b.prm.pm_grid;
+    GridInfo &gx = b.g[kKeyXMajor];
     gx.M = M;
     gx.key_mode = kKeyXMajor;
     gx.bits = ceil_log2((uint64_t)M);
     gx.key_bits = ceil_log2((uint64_t)M * M * M);
     gx.nkeys = (uint32_t)((uint64_t)M * M * M);
-    gx.L = params->box_size;
-    gx.fac = M / params->box_size;
-    gx.cell = params->box_size / M;
-    GridInfo &gm = e->gm;
+    gx.L = b.prm.box_size;
+    gx.fac = M / b.prm.box_size;
+    gx.cell = b.prm.box_size / M;
+    GridInfo &gm = b.g[kKeyMorton];
     gm = gx;
     gm.key_mode = kKeyMorton;
     gm.bits = ceil_log2((uint64_t)M);
     gm.key_bits = 3 * gm.bits;
     gm.nkeys = (uint32_t)(1ull << gm.key_bits);
+}
 
-    e->pm_fac = params->G / (M_PI * params->box_size);  // pm_periodic.c:85-86
-    e->pm_fac *= 1 / (2 * params->box_size / M);
+int create_engine(const cuco_params *params, const uint32_t *caps, int nboxes,
+                  bool ensemble, int32_t device, cuco_engine **out)
+{
+    *out = nullptr;
+    uint64_t cap_total = 0, waves = 0, keys[2] = {0, 0};
+    for (int b = 0; b < nboxes; ++b) {
+        TRY(check_params(&params[b]));
+        cap_total += caps[b];
+        waves += (caps[b] + 63) / 64;
+    }
+    if (cap_total == 0 || cap_total > 0x7fffffffULL) {
+        set_error("total particle count %llu out of range", (unsigned long long)cap_total);
+        return CUCO_ERR_INVALID_ARG;
+    }
+    CUCO_HIP(hipSetDevice(device));
+    cuco_engine *e = new cuco_engine();
+    e->device = device;
+    e->ensemble = ensemble;
+    e->box.resize(nboxes);
+    e->min_bits = 64;
+    for (int b = 0; b < nboxes; ++b) {
+        HostBox &h = e->box[b];
+        h.prm = params[b];
+        h.cap = caps[b];
+        h.n = ensemble ? caps[b] : 0;
+        set_pair_params(h);
+        set_grids(h);
+        for (int m = 0; m < 2; ++m) keys[m] += h.g[m].nkeys;
+        e->min_bits = std::min(e->min_bits, h.g[kKeyMorton].bits);
+        const uint64_t M = h.prm.pm_grid;
+        e->grid_elems += M * M * M;
+        e->cgrid_elems += M * M * (M / 2 + 1);
+        e->tab_elems += M;
+    }
+    if (keys[0] >= 0xffffffffULL || keys[1] >= 0xffffffffULL) {
+        delete e;
+        set_error("the boxes' cell keys exceed 32 bits");
+        return CUCO_ERR_UNSUPPORTED;
+    }
+    for (int m = 0; m < 2; ++m) {
+        e->nkeys[m] = (uint32_t)keys[m];
+        e->key_bits[m] = ceil_log2(keys[m]);
+    }
+    e->capacity = (uint32_t)cap_total;
+    e->bs.nboxes = nboxes;
+    e->bs.capacity = e->capacity;
+    e->bs.wave_slots = (uint32_t)waves;
 
     const size_t cap = e->capacity;
     int rc = CUCO_OK;
@@ -544,17 +780,24 @@ int cuco_create(const cuco_params *params, int64_t max_particles, int32_t device
     ALLOC(e->sc.counts, sizeof(uint32_t) * sort_count_words(e->capacity));
     ALLOC(e->sc.totals, sizeof(uint32_t) * 256);
     e->sc.capacity = e->capacity;
-    const size_t offw = (size_t)std::max(gx.nkeys, gm.nkeys) + 1;
+    const size_t offw = (size_t)std::max(e->nkeys[0], e->nkeys[1]) + 1;
     ALLOC(e->off, sizeof(uint32_t) * offw);
     ALLOC(e->spos64, sizeof(double4) * cap);
     ALLOC(e->spos32, sizeof(float4) * cap);
     ALLOC(e->table, sizeof(float) * kNTab);
     ALLOC(e->table32, sizeof(float) * kTabPadded);
     ALLOC(e->dcount, sizeof(unsigned long long) * 2);
+    ALLOC(e->p32, sizeof(P32) * nboxes);
+    ALLOC(e->bs.dev[0], sizeof(BoxDev) * nboxes);
+    ALLOC(e->bs.dev[1], sizeof(BoxDev) * nboxes);
+    ALLOC(e->bs.maps, sizeof(uint32_t) * kMapCount * (nboxes + 1));
+    ALLOC(e->bs.tile_map, sizeof(uint32_t) * (nboxes + 1));
     // Tile arrays sized for the smallest tunable Morton block (most tiles).
-    e->tiles.block_shift = gm.bits >= 4 ? 3 : gm.bits - 1;
+    // A Morton block never straddles two boxes: every box's key range is a
+    // multiple of 8^bits with bits > shift.
+    e->tiles.block_shift = e->min_bits >= 4 ? 3 : e->min_bits - 1;
     e->min_block_shift = std::min(2, e->tiles.block_shift);
-    e->tiles.nblocks = gm.nkeys >> (3 * e->min_block_shift);
+    e->tiles.nblocks = e->nkeys[kKeyMorton] >> (3 * e->min_block_shift);
     e->tiles.max_tiles = (e->capacity + 63) / 64 + e->tiles.nblocks;
     ALLOC(e->tiles.tile_start, sizeof(uint32_t) * e->tiles.max_tiles);
     ALLOC(e->tiles.tile_count, sizeof(uint32_t) * e->tiles.max_tiles);
@@ -569,7 +812,8 @@ int cuco_create(const cuco_params *params, int64_t max_particles, int32_t device
             rc = CUCO_ERR_HIP;
         }
     }
-    if (rc == CUCO_OK) rc = upload_tables(e);
+    if (rc == CUCO_OK) rc = update_pair_constants(e);
+    if (rc == CUCO_OK) rc = update_boxes(e);
     if (rc == CUCO_OK) {
         int ncu = 0;
         if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || ncu <= 0)
@@ -584,6 +828,59 @@ int cuco_create(const cuco_params *params, int64_t max_particles, int32_t device
         return rc;
     }
     *out = e;
+    return CUCO_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cuco_create(const cuco_params *params, int64_t max_particles, int32_t device,
+                cuco_engine **out)
+{
+    if (!params || !out) {
+        set_error("null argument");
+        return CUCO_ERR_INVALID_ARG;
+    }
+    *out = nullptr;
+    if (max_particles <= 0 || max_particles > 0x7fffffffLL) {
+        set_error("max_particles %lld out of range", (long long)max_particles);
+        return CUCO_ERR_INVALID_ARG;
+    }
+    const uint32_t cap = (uint32_t)max_particles;
+    return create_engine(params, &cap, 1, false, device, out);
+}
+
+int cuco_ensemble_create(const cuco_params *params, const int64_t *n_per_box,
+                         int32_t nboxes, int32_t device, cuco_engine **out)
+{
+    if (!params || !n_per_box || !out) {
+        set_error("null argument");
+        return CUCO_ERR_INVALID_ARG;
+    }
+    *out = nullptr;
+    if (nboxes < 1 || nboxes > 65536) {
+        set_error("nboxes %d outside 1..65536", nboxes);
+        return CUCO_ERR_INVALID_ARG;
+    }
+    std::vector<uint32_t> caps(nboxes);
+    for (int b = 0; b < nboxes; ++b) {
+        if (n_per_box[b] <= 0 || n_per_box[b] > 0x7fffffffLL) {
+            set_error("box %d: particle count %lld out of range", b, (long long)n_per_box[b]);
+            return CUCO_ERR_INVALID_ARG;
+        }
+        caps[b] = (uint32_t)n_per_box[b];
+    }
+    return create_engine(params, caps.data(), nboxes, true, device, out);
+}
+
+int cuco_ensemble_size(cuco_engine *e, int32_t *nboxes, int64_t *n_per_box)
+{
+    CHECK_ENGINE(e);
+    if (nboxes) *nboxes = (int32_t)e->box.size();
+    if (n_per_box)
+        for (size_t b = 0; b < e->box.size(); ++b)
+            n_per_box[b] = e->ensemble ? e->box[b].cap : e->box[b].n;
     return CUCO_OK;
 }
 
@@ -610,15 +907,20 @@ int cuco_destroy(cuco_engine *e)
     free_ptr(e->sc.counts); free_ptr(e->sc.totals);
     free_ptr(e->off); free_ptr(e->spos64); free_ptr(e->spos32);
     free_ptr(e->table); free_ptr(e->table32); free_ptr(e->dcount);
+    free_ptr(e->p32);
+    free_ptr(e->bs.dev[0]); free_ptr(e->bs.dev[1]);
+    free_ptr(e->bs.maps); free_ptr(e->bs.tile_map);
     free_ptr(e->tiles.tile_start); free_ptr(e->tiles.tile_count);
     free_ptr(e->tiles.block_tiles); free_ptr(e->tiles.counters);
     free_ptr(e->tiles.scan_scratch);
     for (int k = 0; k < 2; ++k) {
         free_ptr(e->rho[k]); free_ptr(e->phi[k]); free_ptr(e->cgrid[k]);
         free_ptr(e->dep_scratch[k]);
-        free_ptr(e->tb[k].tw); free_ptr(e->tb[k].gexp); free_ptr(e->tb[k].gsinc);
+        free_pm_tables(e->tb[k]);
     }
     free_ptr(e->dfactors); free_ptr(e->dstep);
+    free_ptr(e->ti_beg); free_ptr(e->ti_end);
+    free_ptr(e->ti_scalars); free_ptr(e->kick_table);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
     return CUCO_OK;
@@ -631,10 +933,51 @@ int cuco_set_softening(cuco_engine *e, double soft_h)
         set_error("softening must be positive");
         return CUCO_ERR_INVALID_ARG;
     }
-    e->prm.soft_h = soft_h;
-    set_pair_params(e);
+    for (HostBox &b : e->box) {
+        b.prm.soft_h = soft_h;
+        set_pair_params(b);
+    }
     destroy_graph(e);
-    return CUCO_OK;
+    return update_pair_constants(e);
+}
+
+int cuco_set_params(cuco_engine *e, int32_t box, const cuco_params *params)
+{
+    CHECK_ENGINE(e);
+    if (!params || box < -1 || box >= (int)e->box.size()) {
+        set_error("null params or box %d outside -1..%d", box, (int)e->box.size() - 1);
+        return CUCO_ERR_INVALID_ARG;
+    }
+    TRY(check_params(params));
+    const int b0 = box < 0 ? 0 : box, b1 = box < 0 ? (int)e->box.size() : box + 1;
+    for (int b = b0; b < b1; ++b) {
+        const cuco_params &cur = e->box[b].prm;
+        if (params->pm_grid != cur.pm_grid || params->box_size != cur.box_size) {
+            set_error("box %d: pm_grid and box_size are fixed when the handle is created", b);
+            return CUCO_ERR_INVALID_ARG;
+        }
+    }
+    const bool asmth_changed = [&] {
+        for (int b = b0; b < b1; ++b)
+            if (e->box[b].prm.asmth != params->asmth) return true;
+        return false;
+    }();
+    for (int b = b0; b < b1; ++b) {
+        e->box[b].prm = *params;
+        set_pair_params(e->box[b]);
+    }
+    if (asmth_changed) {
+        // The Green's function tables depend on asmth: rebuild them.
+        for (int k = 0; k < 2; ++k) {
+            if (!e->cgrid[k]) continue;
+            free_pm_tables(e->tb[k]);
+            TRY(k == 0 ? make_pm_tables<double>(e, e->tb[0]) : make_pm_tables<float>(e, e->tb[1]));
+        }
+    }
+    TRY(update_boxes(e));
+    destroy_graph(e);
+    e->pp_prec = e->pm_prec = -1;
+    return update_pair_constants(e);
 }
 
 int cuco_upload_particles(cuco_engine *e, int64_t n, const double *pos,
@@ -642,36 +985,55 @@ int cuco_upload_particles(cuco_engine *e, int64_t n, const double *pos,
                           double uniform_mass)
 {
     CHECK_ENGINE(e);
-    if (n <= 0 || n > (int64_t)e->capacity || !pos) {
+    if (e->ensemble) {
+        if (n != (int64_t)e->capacity || !pos) {
+            set_error("ensemble upload needs exactly %u particles (got %lld) and positions",
+                      e->capacity, (long long)n);
+            return CUCO_ERR_INVALID_ARG;
+        }
+    } else if (n <= 0 || n > (int64_t)e->capacity || !pos) {
         set_error("n=%lld outside [1, %u] or null positions", (long long)n, e->capacity);
         return CUCO_ERR_INVALID_ARG;
     }
     std::vector<double4> hp((size_t)n), hv((size_t)n);
-    const double L = e->prm.box_size;
     double common = mass ? mass[0] : uniform_mass;
-    for (int64_t i = 0; i < n; ++i) {
-        double x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
-        if (!std::isfinite(x) || !std::isfinite(y) || !std::isfinite(z)) {
-            set_error("non-finite position for particle %lld", (long long)i);
-            return CUCO_ERR_INVALID_ARG;
+    int64_t i = 0;
+    for (size_t b = 0; b < e->box.size(); ++b) {
+        const double L = e->box[b].prm.box_size;
+        const int64_t nb = e->ensemble ? (int64_t)e->box[b].cap : n;
+        for (int64_t k = 0; k < nb; ++k, ++i) {
+            double x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
+            if (!std::isfinite(x) || !std::isfinite(y) || !std::isfinite(z)) {
+                set_error("non-finite position for particle %lld", (long long)i);
+                return CUCO_ERR_INVALID_ARG;
+            }
+            TRY(wrap_host(&x, L));
+            TRY(wrap_host(&y, L));
+            TRY(wrap_host(&z, L));
+            hp[i] = make_double4(x, y, z, mass ? mass[i] : uniform_mass);
+            if (hp[i].w != common) common = 0;
+            hv[i] = vel ? make_double4(vel[3 * i], vel[3 * i + 1], vel[3 * i + 2], 0.0)
+                        : make_double4(0, 0, 0, 0);
         }
-        TRY(wrap_host(&x, L));
-        TRY(wrap_host(&y, L));
-        TRY(wrap_host(&z, L));
-        hp[i] = make_double4(x, y, z, mass ? mass[i] : uniform_mass);
-        if (hp[i].w != common) common = 0;
-        hv[i] = vel ? make_double4(vel[3 * i], vel[3 * i + 1], vel[3 * i + 2], 0.0)
-                    : make_double4(0, 0, 0, 0);
     }
     CUCO_HIP(hipMemcpyAsync(e->pos, hp.data(), sizeof(double4) * n, hipMemcpyHostToDevice, e->stream));
     CUCO_HIP(hipMemcpyAsync(e->vel, hv.data(), sizeof(double4) * n, hipMemcpyHostToDevice, e->stream));
     CUCO_HIP(hipStreamSynchronize(e->stream));
     if ((uint32_t)n != e->n) destroy_graph(e);
     e->n = (uint32_t)n;
-    e->uniform_mass = common > 0 ? common : 0;
+    if (!e->ensemble) e->box[0].n = (uint32_t)n;
+    const double um = common > 0 ? common : 0;
+    const bool mass_changed = um != e->uniform_mass;
+    e->uniform_mass = um;
+    TRY(update_boxes(e));
+    if (mass_changed) {
+        destroy_graph(e);
+        TRY(update_pair_constants(e));
+    }
     e->have_particles = true;
     e->sorted_mode = -1;
     e->pp_prec = e->pm_prec = -1;
+    e->tl_init = false;
     return CUCO_OK;
 }
 
@@ -772,7 +1134,7 @@ int cuco_drift(cuco_engine *e, double drift_factor)
 {
     CHECK_ENGINE(e);
     TRY(check_ready(e));
-    TRY(launch_drift(e->pos, e->vel, e->n, e->prm.box_size, drift_factor, e->stream));
+    TRY(launch_drift(e->pos, e->vel, e->bs, drift_factor, e->stream));
     e->sorted_mode = -1;
     return CUCO_OK;
 }
@@ -801,7 +1163,7 @@ int cuco_kdk_steps(cuco_engine *e, int32_t precision, int32_t nsteps,
     if (!use_graph) {
         for (int s = 0; s < nsteps; ++s) {
             TRY(kick_current(e, factors[3 * s], factors[3 * s]));
-            TRY(launch_drift(e->pos, e->vel, e->n, e->prm.box_size, factors[3 * s + 1], e->stream));
+            TRY(launch_drift(e->pos, e->vel, e->bs, factors[3 * s + 1], e->stream));
             e->sorted_mode = -1;
             TRY(begin_eval(e));
             int rc = do_forces(e, precision, 1);
@@ -833,7 +1195,7 @@ int cuco_kdk_steps(cuco_engine *e, int32_t precision, int32_t nsteps,
         const void *app = fp32 ? (const void *)e->acc_pp32 : (const void *)e->acc_pp64;
         const void *apm = fp32 ? (const void *)e->acc_pm32 : (const void *)e->acc_pm64;
         int rc = launch_kick_table(e->vel, app, apm, fp32, e->n, e->dfactors, e->dstep, 0, e->stream);
-        if (!rc) rc = launch_drift_table(e->pos, e->vel, e->n, e->prm.box_size, e->dfactors, e->dstep, e->stream);
+        if (!rc) rc = launch_drift_table(e->pos, e->vel, e->bs, e->dfactors, e->dstep, e->stream);
         e->sorted_mode = -1;
         if (!rc) rc = do_forces(e, precision, 1);
         if (!rc) rc = launch_kick_table(e->vel, app, apm, fp32, e->n, e->dfactors, e->dstep, 2, e->stream);
@@ -863,6 +1225,298 @@ int cuco_kdk_steps(cuco_engine *e, int32_t precision, int32_t nsteps,
     CUCO_HIP(hipStreamSynchronize(e->stream));
     e->sorted_mode = -1;  // the sort buffers hold the last step's order
     return CUCO_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// get_drift_factor / get_gravkick_factor (driftfac.c:66-126) at two integer
+// times; the device kick uses the same operations (integrate.hip).
+double timeline_factor(const cuco_engine *e, const std::vector<double> &table,
+                       int time0, int time1)
+{
+    auto at = [&](double u) {
+        int i = (int)u;
+        if (i >= kDriftTable) i = kDriftTable - 1;
+        if (i < 1) return u * table[0];
+        return table[i - 1] + (table[i] - table[i - 1]) * (u - i);
+    };
+    const double lb = e->tl_log_begin, lm = e->tl_log_max;
+    const double a1 = lb + time0 * e->tl_interval;
+    const double a2 = lb + time1 * e->tl_interval;
+    const double u1 = (a1 - lb) / (lm - lb) * kDriftTable;
+    const double u2 = (a2 - lb) / (lm - lb) * kDriftTable;
+    return at(u2) - at(u1);
+}
+
+double timeline_time(const cuco_engine *e, int ti)
+{
+    return e->tl.time_begin * exp(ti * e->tl_interval);  // run.c:57
+}
+
+// Device min of Ti_endstep, then the number of particles at that time.
+__global__ void k_count_eq(const int32_t *__restrict__ ti_end, uint32_t n,
+                           int32_t *__restrict__ scalars)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool eq = i < n && ti_end[i] == scalars[0];
+    const uint64_t b = __ballot(eq);
+    if ((threadIdx.x & 63) == 0 && b) atomicAdd(&scalars[2], (int32_t)__popcll(b));
+}
+
+// Softening of gravlist.cpp:28-45 at scale factor a: comoving SofteningHalo
+// until it exceeds SofteningHaloMaxPhys physically.
+double softening_table(const cuco_timeline &t, double a)
+{
+    return t.softening_halo * a > t.softening_halo_max_phys
+               ? t.softening_halo_max_phys / a
+               : t.softening_halo;
+}
+
+int timeline_step(cuco_engine *e, int precision, int32_t ti_stop, bool &stopped)
+{
+    cuco_timeline_state &s = e->tls;
+    stopped = false;
+    // find_next_sync_point_and_drift (run.c:29-64)
+    TRY(launch_ti_min(e->ti_end, e->n, e->ti_scalars, e->stream));
+    int32_t h[2];
+    CUCO_HIP(hipMemcpyAsync(h, e->ti_scalars, sizeof(h), hipMemcpyDeviceToHost, e->stream));
+    CUCO_HIP(hipStreamSynchronize(e->stream));
+    if (h[1]) {
+        set_error("a timestep left the integer timeline (timestep.c:126-133)");
+        return CUCO_ERR_STATE;
+    }
+    int32_t min_glob = h[0];
+    if (min_glob >= s.pm_ti_endstep) min_glob = s.pm_ti_endstep;
+    if (ti_stop > s.ti_current && min_glob >= ti_stop) {
+        TRY(launch_drift(e->pos, e->vel, e->bs,
+                         timeline_factor(e, e->drift_tab, s.ti_current, ti_stop), e->stream));
+        e->sorted_mode = -1;
+        s.ti_current = ti_stop;
+        s.time = timeline_time(e, s.ti_current);
+        stopped = true;
+        return CUCO_OK;
+    }
+    TRY(launch_drift(e->pos, e->vel, e->bs,
+                     timeline_factor(e, e->drift_tab, s.ti_current, min_glob), e->stream));
+    e->sorted_mode = -1;
+    s.ti_current = min_glob;
+    s.time = timeline_time(e, s.ti_current);
+
+    // Active particles of this step.
+    TRY(launch_fill_u32(reinterpret_cast<uint32_t *>(e->ti_scalars) + 2, 0u, 1, e->stream));
+    k_count_eq<<<grid_for(e->n, 256), 256, 0, e->stream>>>(e->ti_end, e->n, e->ti_scalars);
+    CUCO_LAUNCH_CHECK();
+
+    // set_softenings (gravlist.cpp:28-45) at the new time.
+    const double soft = softening_table(e->tl, s.time);
+    for (HostBox &b : e->box) {
+        b.prm.soft_h = 2.8 * soft;
+        set_pair_params(b);
+    }
+    TRY(update_pair_constants(e));
+    destroy_graph(e);
+
+    // compute_accelerations (accel.c:8-23)
+    const bool pm_step = s.pm_ti_endstep == s.ti_current;
+    TRY(begin_eval(e));
+    int rc = CUCO_OK;
+    if (pm_step) rc = do_pm(e, precision);
+    if (!rc) rc = do_pp(e, precision, nullptr, e->ti_end, s.ti_current);
+    end_eval(e);
+    if (rc) return rc;
+
+    // advance_and_find_timesteps (timestep.c:11-94)
+    const double a = s.time;
+    TimelineDev t{};
+    t.ti_current = s.ti_current;
+    t.log_begin = e->tl_log_begin;
+    t.log_max = e->tl_log_max;
+    t.interval = e->tl_interval;
+    t.fac1 = 1 / (a * a);
+    t.atime = a;
+    double ha = e->tl.omega0 / (a * a * a) + (1 - e->tl.omega0 - e->tl.omega_lambda) / (a * a) +
+                e->tl.omega_lambda;
+    t.hubble_a = e->tl.hubble * sqrt(ha);
+    t.err_tol = e->tl.err_tol_int_accuracy;
+    t.soft_table = soft;
+    t.dt_displacement = e->tl.max_size_timestep;
+    t.min_dt = e->tl.min_size_timestep;
+    const int fp32 = precision == CUCO_FP32;
+    const void *app = fp32 ? (const void *)e->acc_pp32 : (const void *)e->acc_pp64;
+    const void *apm = fp32 ? (const void *)e->acc_pm32 : (const void *)e->acc_pm64;
+    TRY(launch_advance(e->vel, app, apm, fp32, e->ti_beg, e->ti_end, e->n, t,
+                       e->kick_table, e->ti_scalars + 1, e->stream));
+    if (pm_step) {
+        int ti_step = kTimebase;
+        while (ti_step > (e->tl.max_size_timestep / e->tl_interval)) ti_step >>= 1;
+        if (ti_step > (s.pm_ti_endstep - s.pm_ti_begstep)) {
+            if (((kTimebase - s.pm_ti_endstep) % ti_step) > 0)
+                ti_step = s.pm_ti_endstep - s.pm_ti_begstep;
+        }
+        if (s.ti_current == kTimebase) ti_step = 0;
+        const int tstart = (s.pm_ti_begstep + s.pm_ti_endstep) / 2;
+        const int tend = s.pm_ti_endstep + ti_step / 2;
+        const double k = timeline_factor(e, e->kick_tab, tstart, tend);
+        s.pm_ti_begstep = s.pm_ti_endstep;
+        s.pm_ti_endstep = s.pm_ti_begstep + ti_step;
+        if (fp32) TRY(launch_kick32(e->vel, nullptr, e->acc_pm32, e->n, 0, k, e->stream));
+        else TRY(launch_kick64(e->vel, nullptr, e->acc_pm64, e->n, 0, k, e->stream));
+    }
+    int32_t cnt = 0;
+    CUCO_HIP(hipMemcpyAsync(&cnt, e->ti_scalars + 2, sizeof(cnt), hipMemcpyDeviceToHost, e->stream));
+    CUCO_HIP(hipStreamSynchronize(e->stream));
+    s.active = cnt;
+    s.steps++;
+    return CUCO_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cuco_timeline_init(cuco_engine *e, const cuco_timeline *t)
+{
+    CHECK_ENGINE(e);
+    TRY(check_ready(e));
+    if (!t || !(t->time_begin > 0) || !(t->time_max > t->time_begin) ||
+        !(t->hubble > 0) || !(t->err_tol_int_accuracy > 0) ||
+        !(t->max_size_timestep > 0) || !(t->min_size_timestep >= 0) ||
+        !(t->softening_halo > 0) || !(t->softening_halo_max_phys > 0)) {
+        set_error("timeline parameters out of range");
+        return CUCO_ERR_INVALID_ARG;
+    }
+    if (!e->ti_beg) {
+        TRY(dev_alloc(e->ti_beg, sizeof(int32_t) * e->capacity));
+        TRY(dev_alloc(e->ti_end, sizeof(int32_t) * e->capacity));
+        TRY(dev_alloc(e->ti_scalars, sizeof(int32_t) * 4));
+        TRY(dev_alloc(e->kick_table, sizeof(double) * kDriftTable));
+    }
+    e->tl = *t;
+    e->drift_tab.assign(kDriftTable, 0.0);
+    e->kick_tab.assign(kDriftTable, 0.0);
+    TRY(cuco_init_drift_table(t->time_begin, t->time_max, t->omega0, t->omega_lambda,
+                              t->hubble, e->drift_tab.data(), e->kick_tab.data()));
+    CUCO_HIP(hipMemcpy(e->kick_table, e->kick_tab.data(), sizeof(double) * kDriftTable,
+                       hipMemcpyHostToDevice));
+    e->tl_log_begin = log(t->time_begin);
+    e->tl_log_max = log(t->time_max);
+    e->tl_interval = (log(t->time_max) - log(t->time_begin)) / kTimebase;  // init.c:23
+    TRY(launch_fill_u32(reinterpret_cast<uint32_t *>(e->ti_beg), 0u, e->n, e->stream));
+    TRY(launch_fill_u32(reinterpret_cast<uint32_t *>(e->ti_end), 0u, e->n, e->stream));
+    TRY(launch_fill_u32(reinterpret_cast<uint32_t *>(e->ti_scalars), 0u, 4, e->stream));
+    CUCO_HIP(hipStreamSynchronize(e->stream));
+    cuco_timeline_state &s = e->tls;
+    s = cuco_timeline_state{};
+    s.time = t->time_begin;
+    s.timebase_interval = e->tl_interval;
+    e->tl_init = true;
+    return CUCO_OK;
+}
+
+int cuco_timeline_run(cuco_engine *e, int32_t precision, int32_t max_steps,
+                      int32_t ti_stop, cuco_timeline_state *state)
+{
+    CHECK_ENGINE(e);
+    TRY(check_prec(precision));
+    TRY(check_ready(e));
+    if (!e->tl_init) {
+        set_error("cuco_timeline_init has not been called");
+        return CUCO_ERR_STATE;
+    }
+    cuco_timeline_state &s = e->tls;
+    s.at_stop = 0;
+    for (int k = 0; k < max_steps && !s.finished; ++k) {
+        bool stopped = false;
+        TRY(timeline_step(e, precision, ti_stop, stopped));
+        if (stopped) {
+            s.at_stop = 1;
+            break;
+        }
+        // run.c:24
+        if (!(s.ti_current < kTimebase && s.time <= e->tl.time_max)) s.finished = 1;
+    }
+    int32_t flag = 0;
+    CUCO_HIP(hipMemcpyAsync(&flag, e->ti_scalars + 1, sizeof(flag), hipMemcpyDeviceToHost, e->stream));
+    CUCO_HIP(hipStreamSynchronize(e->stream));
+    if (state) *state = s;
+    if (flag) {
+        set_error("a timestep left the integer timeline (timestep.c:126-133)");
+        return CUCO_ERR_STATE;
+    }
+    return CUCO_OK;
+}
+
+int cuco_get_timesteps(cuco_engine *e, int32_t *ti_begstep, int32_t *ti_endstep)
+{
+    CHECK_ENGINE(e);
+    if (!e->tl_init) {
+        set_error("cuco_timeline_init has not been called");
+        return CUCO_ERR_STATE;
+    }
+    if (ti_begstep)
+        CUCO_HIP(hipMemcpyAsync(ti_begstep, e->ti_beg, sizeof(int32_t) * e->n,
+                                hipMemcpyDeviceToHost, e->stream));
+    if (ti_endstep)
+        CUCO_HIP(hipMemcpyAsync(ti_endstep, e->ti_end, sizeof(int32_t) * e->n,
+                                hipMemcpyDeviceToHost, e->stream));
+    CUCO_HIP(hipStreamSynchronize(e->stream));
+    return CUCO_OK;
+}
+
+int cuco_download_records(cuco_engine *e, cuco_particle *P)
+{
+    CHECK_ENGINE(e);
+    TRY(check_ready(e));
+    if (!P) {
+        set_error("null particle array");
+        return CUCO_ERR_INVALID_ARG;
+    }
+    const size_t n = e->n;
+    std::vector<double> pos(3 * n), vel(3 * n), app(3 * n, 0.0), apm(3 * n, 0.0);
+    std::vector<double4> hp(n);
+    CUCO_HIP(hipMemcpyAsync(hp.data(), e->pos, sizeof(double4) * n, hipMemcpyDeviceToHost, e->stream));
+    CUCO_HIP(hipStreamSynchronize(e->stream));
+    TRY(cuco_download_particles(e, pos.data(), vel.data()));
+    if (e->pp_prec >= 0) TRY(cuco_download_accel(e, app.data(), nullptr));
+    if (e->pm_prec >= 0) TRY(cuco_download_accel(e, nullptr, apm.data()));
+    std::vector<int32_t> tb(n, 0), te(n, 0);
+    if (e->tl_init) {
+        CUCO_HIP(hipMemcpyAsync(tb.data(), e->ti_beg, sizeof(int32_t) * n, hipMemcpyDeviceToHost, e->stream));
+        CUCO_HIP(hipMemcpyAsync(te.data(), e->ti_end, sizeof(int32_t) * n, hipMemcpyDeviceToHost, e->stream));
+        CUCO_HIP(hipStreamSynchronize(e->stream));
+    }
+    for (size_t i = 0; i < n; ++i) {
+        for (int d = 0; d < 3; ++d) {
+            P[i].Pos[d] = (float)pos[3 * i + d];
+            P[i].Vel[d] = (float)vel[3 * i + d];
+            P[i].GravAccel[d] = (float)app[3 * i + d];
+            P[i].GravPM[d] = (float)apm[3 * i + d];
+        }
+        P[i].Mass = (float)hp[i].w;
+        P[i].Ti_endstep = te[i];
+        P[i].Ti_begstep = tb[i];
+    }
+    return CUCO_OK;
+}
+
+int cuco_upload_records(cuco_engine *e, const cuco_particle *P, int64_t n)
+{
+    CHECK_ENGINE(e);
+    if (!P || n <= 0) {
+        set_error("null particle array or n <= 0");
+        return CUCO_ERR_INVALID_ARG;
+    }
+    std::vector<double> pos((size_t)n * 3), vel((size_t)n * 3), mass((size_t)n);
+    for (int64_t i = 0; i < n; ++i) {
+        for (int d = 0; d < 3; ++d) {
+            pos[3 * i + d] = P[i].Pos[d];
+            vel[3 * i + d] = P[i].Vel[d];
+        }
+        mass[i] = P[i].Mass;
+    }
+    return cuco_upload_particles(e, n, pos.data(), vel.data(), mass.data(), 0.0);
 }
 
 int cuco_pmforce_periodic(cuco_engine *e, cuco_particle *P, int64_t n, int32_t precision)
@@ -997,7 +1651,7 @@ int cuco_get_cell_table(cuco_engine *e, uint32_t *cell_start, uint32_t *cell_end
         set_error("cell table is available after an FP64 (reference order) evaluation");
         return CUCO_ERR_STATE;
     }
-    const uint32_t nc = e->gx.nkeys;
+    const uint32_t nc = e->nkeys[kKeyXMajor];
     uint32_t *ds = nullptr, *de = nullptr;
     TRY(dev_alloc(ds, sizeof(uint32_t) * nc));
     int rc = dev_alloc(de, sizeof(uint32_t) * nc);
@@ -1028,7 +1682,7 @@ int cuco_get_pm_grid(cuco_engine *e, int32_t which, double *out)
         return CUCO_ERR_STATE;
     }
     const int k = e->pm_prec == CUCO_FP64 ? 0 : 1;
-    const size_t M3 = (size_t)e->M * e->M * e->M;
+    const size_t M3 = e->grid_elems;  // every box's mesh, in box order
     void *src = which == 0 ? e->rho[k] : e->phi[k];
     CUCO_HIP(hipStreamSynchronize(e->stream));
     if (k == 0) {
@@ -1073,7 +1727,7 @@ int cuco_set_tuning(cuco_engine *e, const cuco_tuning *t)
 {
     CHECK_ENGINE(e);
     if (!t || t->pp_block_shift < e->min_block_shift || t->pp_block_shift > 4 ||
-        t->pp_block_shift >= e->gm.bits || t->pp_workgroups_per_cu < 1 ||
+        t->pp_block_shift >= e->min_bits || t->pp_workgroups_per_cu < 1 ||
         t->pp_workgroups_per_cu > 32) {
         set_error("tuning out of range");
         return CUCO_ERR_INVALID_ARG;
@@ -1082,6 +1736,7 @@ int cuco_set_tuning(cuco_engine *e, const cuco_tuning *t)
         e->tiles.block_shift = t->pp_block_shift;
         if (e->sorted_mode == kKeyMorton) e->sorted_mode = -1;  // rebuild tiles
         destroy_graph(e);
+        TRY(update_pair_constants(e));  // the table cut depends on the block size
     }
     if (t->pp_workgroups_per_cu != e->pp32_wg_per_cu) {
         e->pp32_wg_per_cu = t->pp_workgroups_per_cu;
@@ -1103,7 +1758,7 @@ int cuco_autotune(cuco_engine *e, cuco_tuning *chosen, double *best_ms)
     cuco_tuning best{e->tiles.block_shift, e->pp32_wg_per_cu, 0, 0};
     float best_t = 1e30f;
     int rc = CUCO_OK;
-    for (int shift = e->min_block_shift; shift <= std::min(4, e->gm.bits - 1) && !rc; ++shift) {
+    for (int shift = e->min_block_shift; shift <= std::min(4, e->min_bits - 1) && !rc; ++shift) {
         for (int wg : {4, 8, 16}) {
             cuco_tuning t{shift, wg, 0, 0};
             rc = cuco_set_tuning(e, &t);
@@ -1111,9 +1766,10 @@ int cuco_autotune(cuco_engine *e, cuco_tuning *chosen, double *best_ms)
             float ms = 1e30f;
             for (int rep = 0; rep < 2 && !rc; ++rep) {
                 if (hipEventRecord(a, e->stream) != hipSuccess) rc = CUCO_ERR_HIP;
-                if (!rc) rc = launch_pp32(e->spos32, e->skeys, e->perm, e->off, e->n, e->gm, e->pp,
-                                          e->table32, e->tiles, e->acc_pp32, nullptr,
-                                          e->pp32_workgroups, e->uniform_mass, e->stream);
+                if (!rc) rc = launch_pp32(e->spos32, e->skeys, e->perm, e->off, e->bs, e->p32,
+                                          e->table_cut, e->uniform_mass, e->table32, e->tiles,
+                                          e->acc_pp32, nullptr, e->pp32_workgroups,
+                                          nullptr, 0, e->stream);
                 if (!rc && (hipEventRecord(b, e->stream) != hipSuccess ||
                             hipEventSynchronize(b) != hipSuccess)) rc = CUCO_ERR_HIP;
                 float t_ms = 0;
@@ -1177,62 +1833,12 @@ int cuco_synchronize(cuco_engine *e)
 
 int cuco_fft3d_r2c(int32_t M, int32_t precision, const void *d_in, void *d_out)
 {
-    TRY(check_prec(precision));
-    if (!fft_size_supported(M) || !d_in || !d_out) {
-        set_error("unsupported FFT size %d or null pointer", M);
-        return CUCO_ERR_UNSUPPORTED;
-    }
-    cuco_engine tmp;
-    tmp.M = M;
-    tmp.prm.asmth = 1.0;
-    tmp.prm.box_size = 1.0;
-    PmTables tb;
-    int rc = precision == CUCO_FP64 ? make_pm_tables<double>(&tmp, tb) : make_pm_tables<float>(&tmp, tb);
-    if (!rc) {
-        if (precision == CUCO_FP64) {
-            rc = pm_fft_forward<double>(static_cast<const double *>(d_in), static_cast<Cx<double> *>(d_out), M, tb, nullptr);
-            if (!rc) rc = pm_fft_x<double>(static_cast<Cx<double> *>(d_out), M, false, nullptr, tb);
-        } else {
-            rc = pm_fft_forward<float>(static_cast<const float *>(d_in), static_cast<Cx<float> *>(d_out), M, tb, nullptr);
-            if (!rc) rc = pm_fft_x<float>(static_cast<Cx<float> *>(d_out), M, false, nullptr, tb);
-        }
-    }
-    if (!rc && hipDeviceSynchronize() != hipSuccess) {
-        set_error("FFT failed on the device");
-        rc = CUCO_ERR_HIP;
-    }
-    free_ptr(tb.tw); free_ptr(tb.gexp); free_ptr(tb.gsinc);
-    return rc;
+    return fft3d_single(M, precision, false, const_cast<void *>(d_in), d_out);
 }
 
 int cuco_fft3d_c2r(int32_t M, int32_t precision, void *d_in, void *d_out)
 {
-    TRY(check_prec(precision));
-    if (!fft_size_supported(M) || !d_in || !d_out) {
-        set_error("unsupported FFT size %d or null pointer", M);
-        return CUCO_ERR_UNSUPPORTED;
-    }
-    cuco_engine tmp;
-    tmp.M = M;
-    tmp.prm.asmth = 1.0;
-    tmp.prm.box_size = 1.0;
-    PmTables tb;
-    int rc = precision == CUCO_FP64 ? make_pm_tables<double>(&tmp, tb) : make_pm_tables<float>(&tmp, tb);
-    if (!rc) {
-        if (precision == CUCO_FP64) {
-            rc = pm_fft_x<double>(static_cast<Cx<double> *>(d_in), M, true, nullptr, tb);
-            if (!rc) rc = pm_fft_inverse<double>(static_cast<Cx<double> *>(d_in), static_cast<double *>(d_out), M, tb, nullptr);
-        } else {
-            rc = pm_fft_x<float>(static_cast<Cx<float> *>(d_in), M, true, nullptr, tb);
-            if (!rc) rc = pm_fft_inverse<float>(static_cast<Cx<float> *>(d_in), static_cast<float *>(d_out), M, tb, nullptr);
-        }
-    }
-    if (!rc && hipDeviceSynchronize() != hipSuccess) {
-        set_error("FFT failed on the device");
-        rc = CUCO_ERR_HIP;
-    }
-    free_ptr(tb.tw); free_ptr(tb.gexp); free_ptr(tb.gsinc);
-    return rc;
+    return fft3d_single(M, precision, true, d_in, d_out);
 }
 
 }  // extern "C"

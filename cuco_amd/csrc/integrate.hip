@@ -20,17 +20,23 @@ __device__ inline double wrap_coord(double x, double L)
 }
 
 __global__ void k_drift(double4 *__restrict__ pos, const double4 *__restrict__ vel,
-                        uint32_t n, double L, const double *__restrict__ factors,
+                        BoxView v, const double *__restrict__ factors,
                         const uint32_t *__restrict__ step, double drift)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    int bi;
+    uint32_t lb;
+    if (!box_block(v, bi, lb)) return;
+    const BoxDev &bx = v.box[bi];
+    const uint32_t il = lb * blockDim.x + threadIdx.x;
+    if (il >= bx.n) return;
+    const uint32_t i = bx.p0 + il;
+    const double L = bx.g.L;
     const double d = factors ? factors[3 * step[0] + 1] : drift;
     double4 p = pos[i];
-    const double4 v = vel[i];
-    p.x += v.x * d;
-    p.y += v.y * d;
-    p.z += v.z * d;
+    const double4 q = vel[i];
+    p.x += q.x * d;
+    p.y += q.y * d;
+    p.z += q.z * d;
     p.x = wrap_coord(p.x, L);
     p.y = wrap_coord(p.y, L);
     p.z = wrap_coord(p.z, L);
@@ -68,23 +74,119 @@ __global__ void k_kick(double4 *__restrict__ vel, const A *__restrict__ app,
 
 __global__ void k_step_advance(uint32_t *step) { step[0] += 1u; }
 
+// ------------------------------------------------------------ integer timeline
+// get_gravkick_factor (driftfac.c:96-126): the kick table interpolated
+// linearly in ln a at two integer times; the u < 1 branch as there.
+__device__ inline double table_at(const double *__restrict__ table, double u)
+{
+    int i = (int)u;
+    if (i >= kDriftTable) i = kDriftTable - 1;
+    if (i < 1) return u * table[0];
+    return table[i - 1] + (table[i] - table[i - 1]) * (u - i);
+}
+
+__device__ inline double kick_factor(const TimelineDev &t, const double *__restrict__ table,
+                                     int time0, int time1)
+{
+    const double a1 = t.log_begin + time0 * t.interval;
+    const double a2 = t.log_begin + time1 * t.interval;
+    const double u1 = (a1 - t.log_begin) / (t.log_max - t.log_begin) * kDriftTable;
+    const double u2 = (a2 - t.log_begin) / (t.log_max - t.log_begin) * kDriftTable;
+    return table_at(table, u2) - table_at(table, u1);
+}
+
+// advance_and_find_timesteps, particle part (timestep.c:30-70), with
+// get_timestep (timestep.c:96-138): for every particle whose step ends now,
+// a new power-of-two step from the acceleration criterion, then the kick
+// with its short-range acceleration from the middle of the old step to the
+// middle of the new one.  A step that falls off the integer timeline sets
+// bit 0 of flags and leaves the particle unchanged.
+template <typename A>
+__global__ void k_advance(double4 *__restrict__ vel, const A *__restrict__ app,
+                          const A *__restrict__ apm, int32_t *__restrict__ ti_beg,
+                          int32_t *__restrict__ ti_end, uint32_t n, TimelineDev t,
+                          const double *__restrict__ kick_table,
+                          int32_t *__restrict__ flags)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t end = ti_end[i];
+    if (end != t.ti_current) return;
+    const int32_t beg = ti_beg[i];
+    const A g = app[i];
+    const A pm = apm[i];
+    // get_timestep (timestep.c:96-138)
+    double ax = t.fac1 * g.x, ay = t.fac1 * g.y, az = t.fac1 * g.z;
+    ax += t.fac1 * pm.x;
+    ay += t.fac1 * pm.y;
+    az += t.fac1 * pm.z;
+    double ac = sqrt(ax * ax + ay * ay + az * az);
+    if (ac == 0) ac = 1.0e-30;
+    double dt = sqrt(2 * t.err_tol * t.atime * t.soft_table / ac);
+    dt *= t.hubble_a;
+    if (dt >= t.dt_displacement) dt = t.dt_displacement;
+    if (dt < t.min_dt) dt = t.min_dt;
+    const double steps = dt / t.interval;
+    if (!(steps > 0 && steps < (double)kTimebase)) {
+        atomicOr(flags, 1);
+        return;
+    }
+    int ti_step = (int)steps;
+    if (!(ti_step > 0 && ti_step < kTimebase)) {
+        atomicOr(flags, 1);
+        return;
+    }
+    // timestep.c:32-49
+    int ti_min = kTimebase;
+    while (ti_min > ti_step) ti_min >>= 1;
+    ti_step = ti_min;
+    if (ti_step > (end - beg)) {
+        if (((kTimebase - end) % ti_step) > 0) ti_step = end - beg;
+    }
+    if (t.ti_current == kTimebase) ti_step = 0;
+    if ((kTimebase - t.ti_current) < ti_step) ti_step = kTimebase - t.ti_current;
+    const int tstart = (beg + end) / 2;
+    const int tend = end + ti_step / 2;
+    const double k = kick_factor(t, kick_table, tstart, tend);
+    ti_beg[i] = end;
+    ti_end[i] = end + ti_step;
+    double4 v = vel[i];
+    v.x += (double)g.x * k;
+    v.y += (double)g.y * k;
+    v.z += (double)g.z * k;
+    vel[i] = v;
+}
+
+// Smallest Ti_endstep (find_next_sync_point_and_drift, run.c:35-45).
+__global__ void k_ti_min(const int32_t *__restrict__ ti_end, uint32_t n,
+                         int32_t *__restrict__ out)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    int32_t v = i < n ? ti_end[i] : 0x7fffffff;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+    if ((threadIdx.x & 63) == 0) atomicMin(out, v);
+}
+
 }  // namespace
 
-int launch_drift(double4 *pos, const double4 *vel, uint32_t n, double L,
+int launch_drift(double4 *pos, const double4 *vel, const BoxSet &bs,
                  double drift, hipStream_t s)
 {
-    if (n == 0) return CUCO_OK;
-    k_drift<<<grid_for(n, 256), 256, 0, s>>>(pos, vel, n, L, nullptr, nullptr, drift);
+    if (bs.total[kMapPart] == 0) return CUCO_OK;
+    k_drift<<<xcd_grid(bs.total[kMapPart]), 256, 0, s>>>(
+        pos, vel, bs.view(0, kMapPart), nullptr, nullptr, drift);
     CUCO_LAUNCH_CHECK();
     return CUCO_OK;
 }
 
-int launch_drift_table(double4 *pos, const double4 *vel, uint32_t n, double L,
+int launch_drift_table(double4 *pos, const double4 *vel, const BoxSet &bs,
                        const double *factors, const uint32_t *step,
                        hipStream_t s)
 {
-    if (n == 0) return CUCO_OK;
-    k_drift<<<grid_for(n, 256), 256, 0, s>>>(pos, vel, n, L, factors, step, 0.0);
+    if (bs.total[kMapPart] == 0) return CUCO_OK;
+    k_drift<<<xcd_grid(bs.total[kMapPart]), 256, 0, s>>>(
+        pos, vel, bs.view(0, kMapPart), factors, step, 0.0);
     CUCO_LAUNCH_CHECK();
     return CUCO_OK;
 }
@@ -129,6 +231,34 @@ int launch_kick_table(double4 *vel, const void *app, const void *apm, int fp32,
 int launch_step_advance(uint32_t *step, hipStream_t s)
 {
     k_step_advance<<<1, 1, 0, s>>>(step);
+    CUCO_LAUNCH_CHECK();
+    return CUCO_OK;
+}
+
+int launch_advance(double4 *vel, const void *app, const void *apm, int fp32,
+                   int32_t *ti_beg, int32_t *ti_end, uint32_t n,
+                   const TimelineDev &t, const double *kick_table,
+                   int32_t *flags, hipStream_t s)
+{
+    if (n == 0) return CUCO_OK;
+    if (fp32)
+        k_advance<float4><<<grid_for(n, 256), 256, 0, s>>>(
+            vel, static_cast<const float4 *>(app), static_cast<const float4 *>(apm),
+            ti_beg, ti_end, n, t, kick_table, flags);
+    else
+        k_advance<double4><<<grid_for(n, 256), 256, 0, s>>>(
+            vel, static_cast<const double4 *>(app), static_cast<const double4 *>(apm),
+            ti_beg, ti_end, n, t, kick_table, flags);
+    CUCO_LAUNCH_CHECK();
+    return CUCO_OK;
+}
+
+int launch_ti_min(const int32_t *ti_end, uint32_t n, int32_t *out, hipStream_t s)
+{
+    int rc = launch_fill_u32(reinterpret_cast<uint32_t *>(out), 0x7fffffffu, 1, s);
+    if (rc) return rc;
+    if (n == 0) return CUCO_OK;
+    k_ti_min<<<grid_for(n, 256), 256, 0, s>>>(ti_end, n, out);
     CUCO_LAUNCH_CHECK();
     return CUCO_OK;
 }

@@ -6,6 +6,37 @@
 
 namespace cuco {
 
+// ------------------------------------------------------------ boxes
+// Work units counted by the block maps (one prefix over boxes each).
+enum BoxMap : int {
+    kMapPart = 0,   // 256 particles per block
+    kMapGatherX,    // deposit gather bricks, x-major keys
+    kMapGatherM,    // deposit gather bricks, Morton keys
+    kMapRowsD,      // FFT z-row blocks, FP64
+    kMapRowsF,      // FFT z-row blocks, FP32
+    kMapLinesD,     // FFT y/x-line blocks, FP64
+    kMapLinesF,     // FFT y/x-line blocks, FP32
+    kMapCount
+};
+
+// Device-side description of the boxes of an engine plus the host copies
+// of every map's total (the launch sizes).
+struct BoxSet {
+    int nboxes = 0;
+    uint32_t capacity = 0;           // particle slots (deposit scratch)
+    uint32_t wave_slots = 0;         // deposit wave slots
+    BoxDev *dev[2] = {nullptr, nullptr};  // descriptors per KeyMode
+    uint32_t *maps = nullptr;        // kMapCount x (nboxes+1)
+    uint32_t total[kMapCount] = {};  // blocks per launch
+    uint32_t *tile_map = nullptr;    // nboxes+1: first pair-kernel tile per box
+    BoxView view(int mode, int map) const
+    {
+        return BoxView{dev[mode], maps + (size_t)map * (nboxes + 1), nboxes};
+    }
+};
+// Blocks of one box of mesh size M for a PM map (kMapGatherX..kMapLinesF).
+uint32_t pm_blocks(int M, int map);
+
 // ------------------------------------------------------------ sort
 struct SortScratch {
     uint32_t *keys_tmp = nullptr;
@@ -21,7 +52,7 @@ inline size_t sort_count_words(uint32_t n)
     return 256 * (size_t)((n + kSortTile - 1) / kSortTile + 1);
 }
 
-int launch_calc_keys(const double4 *pos, uint32_t n, const GridInfo &g,
+int launch_calc_keys(const double4 *pos, const BoxSet &bs, int mode,
                      uint32_t *keys, hipStream_t s);
 // Stable LSD sort by bits [0, key_bits).  With vals_identity the input
 // values are taken to be 0..n-1 and vals is only written.  On return
@@ -35,7 +66,7 @@ int launch_cell_offsets(const uint32_t *skeys, uint32_t n, uint32_t nkeys,
 int launch_reorder64(const double4 *pos, const uint32_t *perm, uint32_t n,
                      double4 *spos, hipStream_t s);
 int launch_reorder32(const double4 *pos, const uint32_t *perm,
-                     const uint32_t *skeys, uint32_t n, const GridInfo &g,
+                     const uint32_t *skeys, const BoxSet &bs, int mode,
                      float4 *spos, hipStream_t s);
 // Reference-format cellStart/cellEnd from the CSR offsets (x-major keys).
 int launch_cell_start_end(const uint32_t *off, uint32_t ncells,
@@ -49,9 +80,13 @@ int launch_fill_u32(uint32_t *p, uint32_t value, size_t n, hipStream_t s);
 int launch_copy16(void *dst, const void *src, size_t bytes, hipStream_t s);
 
 // ------------------------------------------------------------ PP
+// One box per launch: sorted particles [i0, i0+n), off at the box's first key.
+// With active_end, only particles whose active_end[original index] equals
+// ti_current get a new force (the active set of an integer-timeline step).
 int launch_pp64(const double4 *spos, const uint32_t *perm, const uint32_t *off,
-                uint32_t n, const PairParams &pp, const float *table,
-                double4 *acc, unsigned long long *pair_counter, hipStream_t s);
+                uint32_t i0, uint32_t n, const PairParams &pp, const float *table,
+                double4 *acc, unsigned long long *pair_counter,
+                const int32_t *active_end, int32_t ti_current, hipStream_t s);
 
 struct TileInfo {
     uint32_t *tile_start = nullptr;   // per tile: first sorted index
@@ -63,13 +98,34 @@ struct TileInfo {
     uint32_t nblocks = 0;
     int block_shift = 3;              // Morton block = 2^shift cells per dim
 };
-int launch_build_tiles(const uint32_t *off, uint32_t n, const GridInfo &g,
+
+// Constants of the FP32 pair kernel for one box.  All lengths are scaled by
+// s = asmthfac (table bins per length unit), so a pair's table bin is
+// simply (int)r.  A pair term in scaled units is dx_s * m * rinv_s^3 * T;
+// the physical acceleration is G * s^2 times the scaled sum.
+struct P32 {
+    float s;                      // asmthfac
+    float rcut, rcut2, filter2;   // scaled
+    float h, h_inv, h3_inv;       // scaled spline softening
+    float cellf, inv_cellf;       // scaled cell size
+    float eps;                    // added to r^2: keeps rsq finite at r = 0
+    float out_scale;              // G * s^2
+    float mass;                   // uniform particle mass of the box
+    int M;
+};
+P32 make_p32(const PairParams &pp, const GridInfo &g, double uniform_mass);
+// True when the zero tail of the LDS table can make the rcut cut alone.
+bool pp32_table_cut(const PairParams &pp, const GridInfo &g, int block_shift);
+
+// Builds the tile list of every box and bs.tile_map (Morton keys).
+int launch_build_tiles(const uint32_t *off, const BoxSet &bs, uint32_t nkeys,
                        TileInfo &t, hipStream_t s);
 int launch_pp32(const float4 *spos, const uint32_t *skeys,
-                const uint32_t *perm, const uint32_t *off, uint32_t n,
-                const GridInfo &g, const PairParams &pp, const float *table32,
-                const TileInfo &t, float4 *acc, unsigned long long *cand_counter,
-                int num_workgroups, double uniform_mass, hipStream_t s);
+                const uint32_t *perm, const uint32_t *off, const BoxSet &bs,
+                const P32 *p32s, bool table_cut, double uniform_mass,
+                const float *table32, const TileInfo &t, float4 *acc,
+                unsigned long long *cand_counter, int num_workgroups,
+                const int32_t *active_end, int32_t ti_current, hipStream_t s);
 
 // ------------------------------------------------------------ PM
 struct PmTables {
@@ -77,31 +133,31 @@ struct PmTables {
     void *gexp = nullptr;    // M: exp(-k^2 asmth2) with signed k
     void *gsinc = nullptr;   // M: 1/sinc(pi k/M)^4
 };
-// scratch: pm_deposit_scratch_bytes(n, fp64) bytes (per-cell corner sums
-// and per-wave partials).
+// scratch: pm_deposit_scratch_bytes(capacity, wave_slots, fp64) bytes
+// (per-cell corner sums and per-wave partials).
 template <typename T>
 int pm_deposit(const void *spos, const uint32_t *skeys, const uint32_t *off,
-               uint32_t n, const GridInfo &g, T *rho, void *scratch,
-               hipStream_t s);
-size_t pm_deposit_scratch_bytes(uint32_t n, int fp64);
+               const BoxSet &bs, int mode, T *rho, void *scratch, hipStream_t s);
+size_t pm_deposit_scratch_bytes(uint32_t capacity, uint32_t wave_slots, int fp64);
 template <typename T>
-int pm_fft_forward(const T *rho, Cx<T> *c, int M, const PmTables &tb,
+int pm_fft_forward(const T *rho, Cx<T> *c, const BoxSet &bs, const PmTables &tb,
                    hipStream_t s);
 template <typename T>
-int pm_green_x(Cx<T> *c, int M, const PmTables &tb, hipStream_t s);
+int pm_green_x(Cx<T> *c, const BoxSet &bs, const PmTables &tb, hipStream_t s);
 template <typename T>
-int pm_fft_inverse(Cx<T> *c, T *phi, int M, const PmTables &tb,
+int pm_fft_inverse(Cx<T> *c, T *phi, const BoxSet &bs, const PmTables &tb,
                    hipStream_t s);
 template <typename T>
 int pm_interp(const T *phi, const void *spos, const uint32_t *skeys,
-              const uint32_t *perm, uint32_t n, const GridInfo &g, double fac,
-              void *acc, hipStream_t s);
+              const uint32_t *perm, const BoxSet &bs, int mode, void *acc,
+              hipStream_t s);
 template <typename T>
-int pm_fft_x(Cx<T> *c, int M, bool inverse, hipStream_t s, const PmTables &tb);
+int pm_fft_x(Cx<T> *c, const BoxSet &bs, bool inverse, const PmTables &tb,
+             hipStream_t s);
 bool fft_size_supported(int M);
 
 // ------------------------------------------------------------ integrator
-int launch_drift(double4 *pos, const double4 *vel, uint32_t n, double L,
+int launch_drift(double4 *pos, const double4 *vel, const BoxSet &bs,
                  double drift, hipStream_t s);
 int launch_kick64(double4 *vel, const double4 *app, const double4 *apm,
                   uint32_t n, double kpp, double kpm, hipStream_t s);
@@ -110,9 +166,27 @@ int launch_kick32(double4 *vel, const float4 *app, const float4 *apm,
 int launch_kick_table(double4 *vel, const void *app, const void *apm,
                       int fp32, uint32_t n, const double *factors,
                       const uint32_t *step, int which, hipStream_t s);
-int launch_drift_table(double4 *pos, const double4 *vel, uint32_t n, double L,
+int launch_drift_table(double4 *pos, const double4 *vel, const BoxSet &bs,
                        const double *factors, const uint32_t *step,
                        hipStream_t s);
 int launch_step_advance(uint32_t *step, hipStream_t s);
+
+// Integer timeline (timestep.c, run.c): constants of one system step.
+constexpr int kTimebase = 1 << 28;   // allvars.h:11
+constexpr int kDriftTable = 1000;    // allvars.h:25
+struct TimelineDev {
+    int32_t ti_current;
+    int32_t pad0;
+    double log_begin, log_max, interval;  // ln TimeBegin, ln TimeMax, Timebase_interval
+    double fac1, atime, hubble_a;         // 1/a^2, a, H(a) (timestep.c:18-24)
+    double err_tol, soft_table;           // ErrTolIntAccuracy, SofteningTable
+    double dt_displacement, min_dt;       // MaxSizeTimestep, MinSizeTimestep
+};
+int launch_advance(double4 *vel, const void *app, const void *apm, int fp32,
+                   int32_t *ti_beg, int32_t *ti_end, uint32_t n,
+                   const TimelineDev &t, const double *kick_table,
+                   int32_t *flags, hipStream_t s);
+// out[0] = min(ti_end[0..n)) (out is cleared first).
+int launch_ti_min(const int32_t *ti_end, uint32_t n, int32_t *out, hipStream_t s);
 
 }  // namespace cuco

@@ -34,23 +34,6 @@ constexpr int kFftBytes = 65536;  // LDS tile per block: 8192 FP32 / 4096 FP64 c
 template <typename T>
 constexpr int fft_elems() { return kFftBytes / (int)sizeof(Cx<T>); }
 
-struct FftPlan {
-    int nstages;
-    int radix[16];
-};
-
-FftPlan make_plan(int n)
-{
-    FftPlan p{};
-    int m = n;
-    while (m % 4 == 0) { p.radix[p.nstages++] = 4; m /= 4; }
-    while (m % 2 == 0) { p.radix[p.nstages++] = 2; m /= 2; }
-    while (m % 3 == 0) { p.radix[p.nstages++] = 3; m /= 3; }
-    while (m % 5 == 0) { p.radix[p.nstages++] = 5; m /= 5; }
-    if (m != 1) p.nstages = -1;
-    return p;
-}
-
 // ------------------------------------------------------------- small DFTs
 template <typename T, int R, bool INV>
 __device__ inline void dft(Cx<T> *v)
@@ -189,16 +172,25 @@ __device__ inline Cx<T> *lds_tile()
 }
 
 // ------------------------------------------------------------- z rows
-// Real rows of length M -> M/2+1 complex (forward, unnormalized).
+// Real rows of length M -> M/2+1 complex (forward, unnormalized).  One
+// block holds B = (LDS tile)/(M/2) rows of one box.
 template <typename T>
 __global__ __launch_bounds__(kFftThreads) void k_rows_r2c(
-    const T *__restrict__ in, Cx<T> *__restrict__ out, int M, int nrows,
-    FftPlan plan, const Cx<T> *__restrict__ tw)
+    const T *__restrict__ in_all, Cx<T> *__restrict__ out_all, BoxView v,
+    const Cx<T> *__restrict__ tw_all)
 {
+    int bi;
+    uint32_t lb;
+    if (!box_block(v, bi, lb)) return;
+    const BoxDev &bx = v.box[bi];
+    const int M = bx.g.M;
+    const T *in = in_all + bx.grid0;
+    Cx<T> *out = out_all + bx.cgrid0;
+    const Cx<T> *tw = tw_all + bx.tab0;
     Cx<T> *lds = lds_tile<T>();
-    const int Nf = M / 2, Zc = M / 2 + 1;
+    const int Nf = M / 2, Zc = M / 2 + 1, nrows = M * M;
     const int B = fft_elems<T>() / Nf;
-    const int row0 = blockIdx.x * B;
+    const int row0 = (int)lb * B;
     const Cx<T> *inc = reinterpret_cast<const Cx<T> *>(in);
     for (int e = threadIdx.x; e < Nf * B; e += kFftThreads) {
         const int l = e / Nf, j = e % Nf;
@@ -206,7 +198,7 @@ __global__ __launch_bounds__(kFftThreads) void k_rows_r2c(
         lds[l * Nf + j] = row < nrows ? inc[(size_t)row * Nf + j] : Cx<T>{0, 0};
     }
     __syncthreads();
-    run_fft<T, true, false>(lds, Nf, B, Nf, plan, tw, 2);
+    run_fft<T, true, false>(lds, Nf, B, Nf, bx.half, tw, 2);
     for (int e = threadIdx.x; e < Zc * B; e += kFftThreads) {
         const int l = e / Zc, k = e % Zc;
         const int row = row0 + l;
@@ -225,13 +217,21 @@ __global__ __launch_bounds__(kFftThreads) void k_rows_r2c(
 // M/2+1 complex -> real rows of length M (inverse, unnormalized).
 template <typename T>
 __global__ __launch_bounds__(kFftThreads) void k_rows_c2r(
-    const Cx<T> *__restrict__ in, T *__restrict__ out, int M, int nrows,
-    FftPlan plan, const Cx<T> *__restrict__ tw)
+    const Cx<T> *__restrict__ in_all, T *__restrict__ out_all, BoxView v,
+    const Cx<T> *__restrict__ tw_all)
 {
+    int bi;
+    uint32_t lb;
+    if (!box_block(v, bi, lb)) return;
+    const BoxDev &bx = v.box[bi];
+    const int M = bx.g.M;
+    const Cx<T> *in = in_all + bx.cgrid0;
+    T *out = out_all + bx.grid0;
+    const Cx<T> *tw = tw_all + bx.tab0;
     Cx<T> *lds = lds_tile<T>();
-    const int Nf = M / 2, Zc = M / 2 + 1;
+    const int Nf = M / 2, Zc = M / 2 + 1, nrows = M * M;
     const int B = fft_elems<T>() / Nf;
-    const int row0 = blockIdx.x * B;
+    const int row0 = (int)lb * B;
     for (int e = threadIdx.x; e < Nf * B; e += kFftThreads) {
         const int l = e / Nf, k = e % Nf;
         const int row = row0 + l;
@@ -249,7 +249,7 @@ __global__ __launch_bounds__(kFftThreads) void k_rows_c2r(
         lds[l * Nf + k] = z;
     }
     __syncthreads();
-    run_fft<T, true, true>(lds, Nf, B, Nf, plan, tw, 2);
+    run_fft<T, true, true>(lds, Nf, B, Nf, bx.half, tw, 2);
     Cx<T> *outc = reinterpret_cast<Cx<T> *>(out);
     for (int e = threadIdx.x; e < Nf * B; e += kFftThreads) {
         const int l = e / Nf, j = e % Nf;
@@ -259,51 +259,77 @@ __global__ __launch_bounds__(kFftThreads) void k_rows_c2r(
 }
 
 // ------------------------------------------------------------- strided lines
-// Line L starts at (L / inner) * slab + (L % inner); elements are `stride`
-// apart.  Consecutive lines are consecutive in memory inside a slab.
-template <typename T, bool INV>
-__global__ __launch_bounds__(kFftThreads) void k_lines(
-    Cx<T> *__restrict__ data, int Nf, int nlines, int inner, size_t slab,
-    size_t stride, FftPlan plan, const Cx<T> *__restrict__ tw)
+// Complex mesh [x][y][z], z innermost, Zc = M/2+1.  y lines (XAXIS false):
+// line L = x*Zc + z, elements Zc apart.  x lines (XAXIS true): line
+// L = y*Zc + z, elements M*Zc apart.  Consecutive lines are consecutive in
+// memory, so each of the B lines of a block is one coalesced segment per
+// element.
+template <bool XAXIS>
+__device__ inline size_t line_base(int L, int M, int Zc)
 {
+    return XAXIS ? (size_t)L : (size_t)(L / Zc) * M * Zc + (L % Zc);
+}
+
+template <typename T, bool INV, bool XAXIS>
+__global__ __launch_bounds__(kFftThreads) void k_lines(
+    Cx<T> *__restrict__ data_all, BoxView v, const Cx<T> *__restrict__ tw_all)
+{
+    int bi;
+    uint32_t lb;
+    if (!box_block(v, bi, lb)) return;
+    const BoxDev &bx = v.box[bi];
+    const int M = bx.g.M, Zc = M / 2 + 1, Nf = M;
+    const int nlines = M * Zc;
+    const size_t stride = XAXIS ? (size_t)M * Zc : (size_t)Zc;
+    Cx<T> *data = data_all + bx.cgrid0;
+    const Cx<T> *tw = tw_all + bx.tab0;
     Cx<T> *lds = lds_tile<T>();
     const int B = fft_elems<T>() / Nf;
-    const int line0 = blockIdx.x * B;
+    const int line0 = (int)lb * B;
     for (int e = threadIdx.x; e < Nf * B; e += kFftThreads) {
         const int l = e % B, j = e / B;
         const int L = line0 + l;
-        Cx<T> v = {0, 0};
-        if (L < nlines) v = data[(size_t)(L / inner) * slab + (L % inner) + j * stride];
-        lds[j * B + l] = v;
+        Cx<T> x = {0, 0};
+        if (L < nlines) x = data[line_base<XAXIS>(L, M, Zc) + j * stride];
+        lds[j * B + l] = x;
     }
     __syncthreads();
-    run_fft<T, false, INV>(lds, Nf, B, B, plan, tw, 1);
+    run_fft<T, false, INV>(lds, Nf, B, B, bx.full, tw, 1);
     for (int e = threadIdx.x; e < Nf * B; e += kFftThreads) {
         const int l = e % B, j = e / B;
         const int L = line0 + l;
-        if (L < nlines) data[(size_t)(L / inner) * slab + (L % inner) + j * stride] = lds[j * B + l];
+        if (L < nlines) data[line_base<XAXIS>(L, M, Zc) + j * stride] = lds[j * B + l];
     }
 }
 
 // x lines: forward FFT, Green's function, inverse FFT.  Line L = y*Zc + z.
 template <typename T>
 __global__ __launch_bounds__(kFftThreads) void k_lines_green(
-    Cx<T> *__restrict__ data, int M, FftPlan plan, const Cx<T> *__restrict__ tw,
-    const T *__restrict__ gexp, const T *__restrict__ gsinc)
+    Cx<T> *__restrict__ data_all, BoxView v, const Cx<T> *__restrict__ tw_all,
+    const T *__restrict__ gexp_all, const T *__restrict__ gsinc_all)
 {
+    int bi;
+    uint32_t lb;
+    if (!box_block(v, bi, lb)) return;
+    const BoxDev &bx = v.box[bi];
+    const int M = bx.g.M;
+    Cx<T> *data = data_all + bx.cgrid0;
+    const Cx<T> *tw = tw_all + bx.tab0;
+    const T *gexp = gexp_all + bx.tab0;
+    const T *gsinc = gsinc_all + bx.tab0;
     Cx<T> *lds = lds_tile<T>();
     const int Nf = M, Zc = M / 2 + 1;
     const int nlines = M * Zc;
     const size_t stride = (size_t)M * Zc;
     const int B = fft_elems<T>() / Nf;
-    const int line0 = blockIdx.x * B;
+    const int line0 = (int)lb * B;
     for (int e = threadIdx.x; e < Nf * B; e += kFftThreads) {
         const int l = e % B, j = e / B;
         const int L = line0 + l;
         lds[j * B + l] = L < nlines ? data[(size_t)L + j * stride] : Cx<T>{0, 0};
     }
     __syncthreads();
-    run_fft<T, false, false>(lds, Nf, B, B, plan, tw, 1);
+    run_fft<T, false, false>(lds, Nf, B, B, bx.full, tw, 1);
     for (int e = threadIdx.x; e < Nf * B; e += kFftThreads) {
         const int l = e % B, j = e / B;
         const int L = line0 + l;
@@ -316,11 +342,11 @@ __global__ __launch_bounds__(kFftThreads) void k_lines_green(
         T g = 0;
         if (k2i > 0)
             g = -(gexp[j] * gexp[y] * gexp[z]) * (gsinc[j] * gsinc[y] * gsinc[z]) / (T)k2i;
-        Cx<T> v = lds[j * B + l];
-        lds[j * B + l] = {v.x * g, v.y * g};
+        Cx<T> x = lds[j * B + l];
+        lds[j * B + l] = {x.x * g, x.y * g};
     }
     __syncthreads();
-    run_fft<T, false, true>(lds, Nf, B, B, plan, tw, 1);
+    run_fft<T, false, true>(lds, Nf, B, B, bx.full, tw, 1);
     for (int e = threadIdx.x; e < Nf * B; e += kFftThreads) {
         const int l = e % B, j = e / B;
         const int L = line0 + l;
@@ -345,11 +371,9 @@ __global__ __launch_bounds__(kFftThreads) void k_lines_green(
 
 struct Src64 {
     const double4 *p;
-    __device__ inline void weights(uint32_t j, uint32_t key, const GridInfo &g,
-                                   double &dx, double &dy, double &dz,
-                                   double &m) const
+    __device__ inline void weights(uint32_t j, const GridInfo &g, double &dx,
+                                   double &dy, double &dz, double &m) const
     {
-        (void)key;
         const double4 q = p[j];
         // pm_periodic.c:99-118: d = fac*x - slab, with the clamped slab.
         dx = g.fac * q.x - cell_coord(g.fac, q.x, g.M);
@@ -361,12 +385,10 @@ struct Src64 {
 
 struct Src32 {
     const float4 *p;
-    float facf;
-    __device__ inline void weights(uint32_t j, uint32_t key, const GridInfo &,
-                                   float &dx, float &dy, float &dz,
-                                   float &m) const
+    __device__ inline void weights(uint32_t j, const GridInfo &g, float &dx,
+                                   float &dy, float &dz, float &m) const
     {
-        (void)key;
+        const float facf = (float)g.fac;
         const float4 q = p[j];
         dx = q.x * facf;
         dy = q.y * facf;
@@ -378,29 +400,37 @@ struct Src32 {
 template <typename T>
 __device__ inline T shfl_up_t(T v, int o) { return __shfl_up(v, o, 64); }
 
+// Waves are counted from each box's first particle, so a box's sums do not
+// depend on where the box sits in the particle arrays.
 template <typename T, typename V>
 __global__ __launch_bounds__(256) void k_cic_segments(
-    V src, const uint32_t *__restrict__ skeys, uint32_t n, GridInfo g,
+    V src, const uint32_t *__restrict__ skeys, BoxView v,
     T *__restrict__ cell_sums, T *__restrict__ first_part,
     T *__restrict__ last_part)
 {
-    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    int bi;
+    uint32_t lb;
+    if (!box_block(v, bi, lb)) return;
+    const BoxDev &bx = v.box[bi];
+    const uint32_t il = lb * 256 + threadIdx.x;
     const int lane = threadIdx.x & 63;
-    const uint32_t wave = i >> 6;
-    const uint32_t base = wave << 6;
-    const bool valid = i < n;
+    const uint32_t wl = il >> 6;
+    const uint32_t base = bx.p0 + (wl << 6);
+    const uint32_t nb = bx.n;
+    const bool valid = il < nb;
+    const uint32_t i = bx.p0 + il;
     const uint32_t key = valid ? skeys[i] : 0xffffffffu;
-    T v[8];
+    T val[8];
     if (valid) {
         T dx, dy, dz, m;
-        src.weights(i, key, g, dx, dy, dz, m);
+        src.weights(i, bx.g, dx, dy, dz, m);
         const T wx[2] = {(T)1.0 - dx, dx}, wy[2] = {(T)1.0 - dy, dy},
                 wz[2] = {(T)1.0 - dz, dz};
 #pragma unroll
-        for (int c = 0; c < 8; ++c) v[c] = m * wx[c >> 2] * wy[(c >> 1) & 1] * wz[c & 1];
+        for (int c = 0; c < 8; ++c) val[c] = m * wx[c >> 2] * wy[(c >> 1) & 1] * wz[c & 1];
     } else {
 #pragma unroll
-        for (int c = 0; c < 8; ++c) v[c] = 0;
+        for (int c = 0; c < 8; ++c) val[c] = 0;
     }
     // Segmented inclusive scan; keys are non-decreasing along the wave, so
     // lane-o holding the same key means every lane between does too.
@@ -410,8 +440,8 @@ __global__ __launch_bounds__(256) void k_cic_segments(
         const bool take = lane >= o && ko == key;
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
-            const T t = shfl_up_t(v[c], o);
-            v[c] += take ? t : (T)0;
+            const T t = shfl_up_t(val[c], o);
+            val[c] += take ? t : (T)0;
         }
     }
     // Both shuffles run on every lane: a shuffle inside a short-circuit
@@ -423,43 +453,70 @@ __global__ __launch_bounds__(256) void k_cic_segments(
     const uint64_t heads = __ballot(lane == 0 || kprev != key);
     const uint64_t below = heads & ((lane == 63) ? ~0ull : ((2ull << lane) - 1));
     const int start = 63 - __clzll(below);
-    const uint32_t prev_key = base > 0 ? skeys[base - 1] : 0xffffffffu;
-    const uint32_t next_key = base + 64 < n ? skeys[base + 64] : 0xffffffffu;
+    const uint32_t prev_key = wl > 0 ? skeys[base - 1] : 0xffffffffu;
+    const uint32_t next_key = (wl << 6) + 64 < nb ? skeys[base + 64] : 0xffffffffu;
     if (seg_end) {
         const bool open_left = start == 0 && prev_key == key;
         const bool open_right = lane == 63 && next_key == key;
+        const size_t slot = (size_t)bx.wave0 + wl;
         if (!open_left && !open_right) {
 #pragma unroll
-            for (int c = 0; c < 8; ++c) cell_sums[(size_t)(base + start) * 8 + c] = v[c];
+            for (int c = 0; c < 8; ++c) cell_sums[(size_t)(base + start) * 8 + c] = val[c];
         }
         if (start == 0) {
 #pragma unroll
-            for (int c = 0; c < 8; ++c) first_part[(size_t)wave * 8 + c] = v[c];
+            for (int c = 0; c < 8; ++c) first_part[slot * 8 + c] = val[c];
         }
         if (lane == 63) {
 #pragma unroll
-            for (int c = 0; c < 8; ++c) last_part[(size_t)wave * 8 + c] = v[c];
+            for (int c = 0; c < 8; ++c) last_part[slot * 8 + c] = val[c];
         }
     }
 }
 
-constexpr int kBx = 4, kBy = 4, kBz = 16;  // node brick, 256 threads
+constexpr int kBx = 4, kBy = 4, kBz = 16;  // x-major node brick, 256 threads
+
+// Blocks per box of the gather: x-major keys walk 4x4x16 bricks, Morton keys
+// 4x8x8 bricks whose waves are 4x4x4 Morton sub-cubes.
+inline uint32_t gather_blocks(int M, int key_mode)
+{
+    if (key_mode == kKeyMorton)
+        return (uint32_t)(((M + 3) / 4) * ((M + 7) / 8) * ((M + 7) / 8));
+    return (uint32_t)(((M + kBx - 1) / kBx) * ((M + kBy - 1) / kBy) * ((M + kBz - 1) / kBz));
+}
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_cic_gather(
-    const uint32_t *__restrict__ off, GridInfo g, const T *__restrict__ cell_sums,
+    const uint32_t *__restrict__ off, BoxView v, const T *__restrict__ cell_sums,
     const T *__restrict__ first_part, const T *__restrict__ last_part,
-    T *__restrict__ rho)
+    T *__restrict__ rho_all)
 {
+    int bi;
+    uint32_t lb;
+    if (!box_block(v, bi, lb)) return;
+    const BoxDev &bx = v.box[bi];
+    const GridInfo &g = bx.g;
     const int M = g.M;
-    const int nbz = (M + kBz - 1) / kBz, nby = (M + kBy - 1) / kBy;
-    const int b = blockIdx.x;
-    const int bz = b % nbz, by = (b / nbz) % nby, bx = b / (nbz * nby);
     const int t = threadIdx.x;
-    const int z = bz * kBz + (t % kBz);
-    const int y = by * kBy + ((t / kBz) % kBy);
-    const int x = bx * kBx + t / (kBz * kBy);
+    int x, y, z;
+    if (g.key_mode == kKeyMorton) {
+        // Each wave a 4x4x4 Morton sub-cube, so the cell-table entries a wave
+        // reads are a few contiguous Morton runs.
+        const int nb8 = (M + 7) / 8;
+        const int bz = (int)(lb % nb8), by = (int)((lb / nb8) % nb8), bxx = (int)(lb / (nb8 * nb8));
+        const int w = t >> 6, l = t & 63;
+        x = bxx * 4 + (int)compact3((uint32_t)l >> 2);
+        y = by * 8 + (w >> 1) * 4 + (int)compact3((uint32_t)l >> 1);
+        z = bz * 8 + (w & 1) * 4 + (int)compact3((uint32_t)l);
+    } else {
+        const int nbz = (M + kBz - 1) / kBz, nby = (M + kBy - 1) / kBy;
+        const int bz = (int)(lb % nbz), by = (int)((lb / nbz) % nby), bxx = (int)(lb / (nbz * nby));
+        z = bz * kBz + (t % kBz);
+        y = by * kBy + ((t / kBz) % kBy);
+        x = bxx * kBx + t / (kBz * kBy);
+    }
     if (x >= M || y >= M || z >= M) return;
+    const uint32_t p0 = bx.p0;
     // Corner c = (a, bb, cc): the node is the cell's upper neighbour along
     // the axes whose bit is set; cell = node - (a, bb, cc).
     uint32_t cs[8], ce[8];
@@ -468,36 +525,36 @@ __global__ __launch_bounds__(256) void k_cic_gather(
         const int cx = wrap_index(x - (c >> 2), M);
         const int cy = wrap_index(y - ((c >> 1) & 1), M);
         const int cz = wrap_index(z - (c & 1), M);
-        const uint32_t key = cell_key(g, cx, cy, cz);
+        const uint32_t key = bx.key_base + cell_key(g, cx, cy, cz);
         cs[c] = off[key];
         ce[c] = off[key + 1];
     }
     // Cells inside one wave: one load each, issued together (the index of an
     // empty or straddling cell is redirected to a valid slot and masked).
-    T v[8];
+    T val[8];
     bool straddle = false;
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
         const bool nonempty = ce[c] > cs[c];
-        const bool single = nonempty && (cs[c] >> 6) == ((ce[c] - 1) >> 6);
+        const bool single = nonempty && ((cs[c] - p0) >> 6) == ((ce[c] - 1 - p0) >> 6);
         straddle |= nonempty && !single;
-        const T t = cell_sums[(single ? (size_t)cs[c] * 8 : 0) + c];
-        v[c] = single ? t : (T)0;
+        const T tv = cell_sums[(single ? (size_t)cs[c] * 8 : 0) + c];
+        val[c] = single ? tv : (T)0;
     }
     double sum = 0;
 #pragma unroll
-    for (int c = 0; c < 8; ++c) sum += (double)v[c];
+    for (int c = 0; c < 8; ++c) sum += (double)val[c];
     if (straddle) {
         for (int c = 0; c < 8; ++c) {
             if (ce[c] <= cs[c]) continue;
-            const uint32_t w0 = cs[c] >> 6, w1 = (ce[c] - 1) >> 6;
+            const uint32_t w0 = (cs[c] - p0) >> 6, w1 = (ce[c] - 1 - p0) >> 6;
             if (w0 == w1) continue;
-            sum += (double)last_part[(size_t)w0 * 8 + c];
+            sum += (double)last_part[((size_t)bx.wave0 + w0) * 8 + c];
             for (uint32_t w = w0 + 1; w <= w1; ++w)
-                sum += (double)first_part[(size_t)w * 8 + c];
+                sum += (double)first_part[((size_t)bx.wave0 + w) * 8 + c];
         }
     }
-    rho[((size_t)x * M + y) * M + z] = (T)sum;
+    rho_all[bx.grid0 + ((size_t)x * M + y) * M + z] = (T)sum;
 }
 
 // ------------------------------------------------------------- interpolation
@@ -532,15 +589,23 @@ __device__ inline T interp_dim(const T *__restrict__ phi, int M, int sx, int sxx
     return a;
 }
 
-__global__ __launch_bounds__(256) void k_interp64(const double *__restrict__ phi,
+__global__ __launch_bounds__(256) void k_interp64(const double *__restrict__ phi_all,
                                                   const double4 *__restrict__ spos,
                                                   const uint32_t *__restrict__ perm,
-                                                  uint32_t n, GridInfo g, double fac,
+                                                  BoxView v,
                                                   double4 *__restrict__ acc)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    int bi;
+    uint32_t lb;
+    if (!box_block(v, bi, lb)) return;
+    const BoxDev &bx = v.box[bi];
+    const uint32_t il = lb * 256 + threadIdx.x;
+    if (il >= bx.n) return;
+    const uint32_t i = bx.p0 + il;
+    const GridInfo &g = bx.g;
     const int M = g.M;
+    const double fac = bx.pm_fac;
+    const double *phi = phi_all + bx.grid0;
     const double4 q = spos[i];
     const int sx = cell_coord(g.fac, q.x, M), sy = cell_coord(g.fac, q.y, M),
               sz = cell_coord(g.fac, q.z, M);
@@ -555,19 +620,27 @@ __global__ __launch_bounds__(256) void k_interp64(const double *__restrict__ phi
     acc[perm[i]] = a;
 }
 
-__global__ __launch_bounds__(256) void k_interp32(const float *__restrict__ phi,
+__global__ __launch_bounds__(256) void k_interp32(const float *__restrict__ phi_all,
                                                   const float4 *__restrict__ spos,
                                                   const uint32_t *__restrict__ skeys,
                                                   const uint32_t *__restrict__ perm,
-                                                  uint32_t n, GridInfo g, float facf,
-                                                  float fac, float4 *__restrict__ acc)
+                                                  BoxView v,
+                                                  float4 *__restrict__ acc)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    int bi;
+    uint32_t lb;
+    if (!box_block(v, bi, lb)) return;
+    const BoxDev &bx = v.box[bi];
+    const uint32_t il = lb * 256 + threadIdx.x;
+    if (il >= bx.n) return;
+    const uint32_t i = bx.p0 + il;
+    const GridInfo &g = bx.g;
     const int M = g.M;
+    const float facf = (float)g.fac, fac = (float)bx.pm_fac;
+    const float *phi = phi_all + bx.grid0;
     const float4 q = spos[i];
     int sx, sy, sz;
-    key_cell(g, skeys[i], sx, sy, sz);
+    key_cell(g, skeys[i] - bx.key_base, sx, sy, sz);
     const float dx = q.x * facf, dy = q.y * facf, dz = q.z * facf;
     const int sxx = wrap_index(sx + 1, M), syy = wrap_index(sy + 1, M),
               szz = wrap_index(sz + 1, M);
@@ -600,141 +673,142 @@ bool fft_size_supported(int M)
 
 template <typename T>
 int pm_deposit(const void *spos, const uint32_t *skeys, const uint32_t *off,
-               uint32_t n, const GridInfo &g, T *rho, void *scratch,
-               hipStream_t s)
+               const BoxSet &bs, int mode, T *rho, void *scratch, hipStream_t s)
 {
-    const int M = g.M;
     T *cell_sums = static_cast<T *>(scratch);
-    const size_t nwaves = ((size_t)n + 63) / 64;
-    T *first_part = cell_sums + (size_t)n * 8;
-    T *last_part = first_part + nwaves * 8;
-    if (n > 0) {
+    T *first_part = cell_sums + (size_t)bs.capacity * 8;
+    T *last_part = first_part + (size_t)bs.wave_slots * 8;
+    const BoxView vp = bs.view(mode, kMapPart);
+    if (bs.total[kMapPart] > 0) {
         if (sizeof(T) == 8) {
-            Src64 v{static_cast<const double4 *>(spos)};
-            k_cic_segments<double, Src64><<<grid_for(n, 256), 256, 0, s>>>(
-                v, skeys, n, g, reinterpret_cast<double *>(cell_sums),
+            Src64 src{static_cast<const double4 *>(spos)};
+            k_cic_segments<double, Src64><<<xcd_grid(bs.total[kMapPart]), 256, 0, s>>>(
+                src, skeys, vp, reinterpret_cast<double *>(cell_sums),
                 reinterpret_cast<double *>(first_part),
                 reinterpret_cast<double *>(last_part));
         } else {
-            Src32 v{static_cast<const float4 *>(spos), (float)g.fac};
-            k_cic_segments<float, Src32><<<grid_for(n, 256), 256, 0, s>>>(
-                v, skeys, n, g, reinterpret_cast<float *>(cell_sums),
+            Src32 src{static_cast<const float4 *>(spos)};
+            k_cic_segments<float, Src32><<<xcd_grid(bs.total[kMapPart]), 256, 0, s>>>(
+                src, skeys, vp, reinterpret_cast<float *>(cell_sums),
                 reinterpret_cast<float *>(first_part),
                 reinterpret_cast<float *>(last_part));
         }
         CUCO_LAUNCH_CHECK();
     }
-    const unsigned nb = (unsigned)(((M + kBx - 1) / kBx) * ((M + kBy - 1) / kBy) *
-                                   ((M + kBz - 1) / kBz));
-    k_cic_gather<T><<<nb, 256, 0, s>>>(off, g, cell_sums, first_part, last_part, rho);
+    const int map = mode == kKeyMorton ? kMapGatherM : kMapGatherX;
+    k_cic_gather<T><<<xcd_grid(bs.total[map]), 256, 0, s>>>(
+        off, bs.view(mode, map), cell_sums, first_part, last_part, rho);
     CUCO_LAUNCH_CHECK();
     return CUCO_OK;
 }
 
-size_t pm_deposit_scratch_bytes(uint32_t n, int fp64)
+size_t pm_deposit_scratch_bytes(uint32_t capacity, uint32_t wave_slots, int fp64)
 {
     const size_t elem = fp64 ? 8 : 4;
-    return elem * 8 * ((size_t)n + ((size_t)n + 63) / 64 * 2 + 64);
+    return elem * 8 * ((size_t)capacity + (size_t)wave_slots * 2 + 64);
+}
+
+uint32_t pm_blocks(int M, int map)
+{
+    const uint32_t Zc = (uint32_t)(M / 2 + 1);
+    switch (map) {
+    case kMapGatherX: return gather_blocks(M, kKeyXMajor);
+    case kMapGatherM: return gather_blocks(M, kKeyMorton);
+    case kMapRowsD: { const uint32_t B = fft_elems<double>() / (M / 2); return ((uint32_t)M * M + B - 1) / B; }
+    case kMapRowsF: { const uint32_t B = fft_elems<float>() / (M / 2); return ((uint32_t)M * M + B - 1) / B; }
+    case kMapLinesD: { const uint32_t B = fft_elems<double>() / M; return ((uint32_t)M * Zc + B - 1) / B; }
+    case kMapLinesF: { const uint32_t B = fft_elems<float>() / M; return ((uint32_t)M * Zc + B - 1) / B; }
+    default: return 0;
+    }
 }
 
 template <typename T>
-int pm_fft_forward(const T *rho, Cx<T> *c, int M, const PmTables &tb,
+int pm_fft_forward(const T *rho, Cx<T> *c, const BoxSet &bs, const PmTables &tb,
                    hipStream_t s)
 {
-    const FftPlan ph = make_plan(M / 2), pf = make_plan(M);
+    const bool d = sizeof(T) == 8;
+    const int mr = d ? kMapRowsD : kMapRowsF, ml = d ? kMapLinesD : kMapLinesF;
     const Cx<T> *tw = static_cast<const Cx<T> *>(tb.tw);
-    const int nrows = M * M, Zc = M / 2 + 1;
-    const int Brow = fft_elems<T>() / (M / 2);
-    int rc = launch_fft_kernel<T>(k_rows_r2c<T>, (unsigned)((nrows + Brow - 1) / Brow),
-                                  s, rho, c, M, nrows, ph, tw);
+    int rc = launch_fft_kernel<T>(k_rows_r2c<T>, xcd_grid(bs.total[mr]), s, rho, c,
+                                  bs.view(0, mr), tw);
     if (rc) return rc;
-    // y lines: L = x*Zc + z, slab M*Zc, stride Zc
-    const int nlines = M * Zc, B = fft_elems<T>() / M;
-    return launch_fft_kernel<T>(k_lines<T, false>, (unsigned)((nlines + B - 1) / B), s,
-                                c, M, nlines, Zc, (size_t)M * Zc, (size_t)Zc, pf, tw);
+    return launch_fft_kernel<T>(k_lines<T, false, false>, xcd_grid(bs.total[ml]), s, c,
+                                bs.view(0, ml), tw);
 }
 
 template <typename T>
-int pm_green_x(Cx<T> *c, int M, const PmTables &tb, hipStream_t s)
+int pm_green_x(Cx<T> *c, const BoxSet &bs, const PmTables &tb, hipStream_t s)
 {
-    const FftPlan pf = make_plan(M);
-    const int nlines = M * (M / 2 + 1), B = fft_elems<T>() / M;
-    return launch_fft_kernel<T>(k_lines_green<T>, (unsigned)((nlines + B - 1) / B), s,
-                                c, M, pf, static_cast<const Cx<T> *>(tb.tw),
+    const int ml = sizeof(T) == 8 ? kMapLinesD : kMapLinesF;
+    return launch_fft_kernel<T>(k_lines_green<T>, xcd_grid(bs.total[ml]), s, c,
+                                bs.view(0, ml), static_cast<const Cx<T> *>(tb.tw),
                                 static_cast<const T *>(tb.gexp),
                                 static_cast<const T *>(tb.gsinc));
 }
 
 template <typename T>
-int pm_fft_inverse(Cx<T> *c, T *phi, int M, const PmTables &tb, hipStream_t s)
+int pm_fft_inverse(Cx<T> *c, T *phi, const BoxSet &bs, const PmTables &tb,
+                   hipStream_t s)
 {
-    const FftPlan ph = make_plan(M / 2), pf = make_plan(M);
+    const bool d = sizeof(T) == 8;
+    const int mr = d ? kMapRowsD : kMapRowsF, ml = d ? kMapLinesD : kMapLinesF;
     const Cx<T> *tw = static_cast<const Cx<T> *>(tb.tw);
-    const int Zc = M / 2 + 1, nlines = M * Zc, B = fft_elems<T>() / M;
-    int rc = launch_fft_kernel<T>(k_lines<T, true>, (unsigned)((nlines + B - 1) / B), s,
-                                  c, M, nlines, Zc, (size_t)M * Zc, (size_t)Zc, pf, tw);
+    int rc = launch_fft_kernel<T>(k_lines<T, true, false>, xcd_grid(bs.total[ml]), s, c,
+                                  bs.view(0, ml), tw);
     if (rc) return rc;
-    const int nrows = M * M, Brow = fft_elems<T>() / (M / 2);
-    return launch_fft_kernel<T>(k_rows_c2r<T>, (unsigned)((nrows + Brow - 1) / Brow), s,
-                                (const Cx<T> *)c, phi, M, nrows, ph, tw);
+    return launch_fft_kernel<T>(k_rows_c2r<T>, xcd_grid(bs.total[mr]), s,
+                                (const Cx<T> *)c, phi, bs.view(0, mr), tw);
 }
 
 template <typename T>
 int pm_interp(const T *phi, const void *spos, const uint32_t *skeys,
-              const uint32_t *perm, uint32_t n, const GridInfo &g, double fac,
-              void *acc, hipStream_t s)
+              const uint32_t *perm, const BoxSet &bs, int mode, void *acc,
+              hipStream_t s)
 {
-    if (n == 0) return CUCO_OK;
+    if (bs.total[kMapPart] == 0) return CUCO_OK;
+    const BoxView vp = bs.view(mode, kMapPart);
     if (sizeof(T) == 8) {
-        k_interp64<<<grid_for(n, 256), 256, 0, s>>>(
+        k_interp64<<<xcd_grid(bs.total[kMapPart]), 256, 0, s>>>(
             reinterpret_cast<const double *>(phi), static_cast<const double4 *>(spos),
-            perm, n, g, fac, static_cast<double4 *>(acc));
+            perm, vp, static_cast<double4 *>(acc));
     } else {
-        k_interp32<<<grid_for(n, 256), 256, 0, s>>>(
+        k_interp32<<<xcd_grid(bs.total[kMapPart]), 256, 0, s>>>(
             reinterpret_cast<const float *>(phi), static_cast<const float4 *>(spos),
-            skeys, perm, n, g, (float)g.fac, (float)fac, static_cast<float4 *>(acc));
+            skeys, perm, vp, static_cast<float4 *>(acc));
     }
     CUCO_LAUNCH_CHECK();
     return CUCO_OK;
 }
 
 template <typename T>
-int pm_fft_x(Cx<T> *c, int M, bool inverse, hipStream_t s, const PmTables &tb)
+int pm_fft_x(Cx<T> *c, const BoxSet &bs, bool inverse, const PmTables &tb,
+             hipStream_t s)
 {
-    const FftPlan pf = make_plan(M);
+    const int ml = sizeof(T) == 8 ? kMapLinesD : kMapLinesF;
     const Cx<T> *tw = static_cast<const Cx<T> *>(tb.tw);
-    const int Zc = M / 2 + 1, nlines = M * Zc, B = fft_elems<T>() / M;
-    const size_t stride = (size_t)M * Zc;
     if (inverse)
-        return launch_fft_kernel<T>(k_lines<T, true>, (unsigned)((nlines + B - 1) / B), s,
-                                    c, M, nlines, nlines, (size_t)0, stride, pf, tw);
-    return launch_fft_kernel<T>(k_lines<T, false>, (unsigned)((nlines + B - 1) / B), s,
-                                c, M, nlines, nlines, (size_t)0, stride, pf, tw);
+        return launch_fft_kernel<T>(k_lines<T, true, true>, xcd_grid(bs.total[ml]), s, c,
+                                    bs.view(0, ml), tw);
+    return launch_fft_kernel<T>(k_lines<T, false, true>, xcd_grid(bs.total[ml]), s, c,
+                                bs.view(0, ml), tw);
 }
 
-template int pm_fft_x<double>(Cx<double> *, int, bool, hipStream_t, const PmTables &);
-template int pm_fft_x<float>(Cx<float> *, int, bool, hipStream_t, const PmTables &);
-template int pm_deposit<double>(const void *, const uint32_t *, const uint32_t *,
-                                uint32_t, const GridInfo &, double *, void *,
-                                hipStream_t);
-template int pm_deposit<float>(const void *, const uint32_t *, const uint32_t *,
-                               uint32_t, const GridInfo &, float *, void *,
-                               hipStream_t);
-template int pm_fft_forward<double>(const double *, Cx<double> *, int,
-                                    const PmTables &, hipStream_t);
-template int pm_fft_forward<float>(const float *, Cx<float> *, int,
-                                   const PmTables &, hipStream_t);
-template int pm_green_x<double>(Cx<double> *, int, const PmTables &, hipStream_t);
-template int pm_green_x<float>(Cx<float> *, int, const PmTables &, hipStream_t);
-template int pm_fft_inverse<double>(Cx<double> *, double *, int, const PmTables &,
-                                    hipStream_t);
-template int pm_fft_inverse<float>(Cx<float> *, float *, int, const PmTables &,
-                                   hipStream_t);
-template int pm_interp<double>(const double *, const void *, const uint32_t *,
-                               const uint32_t *, uint32_t, const GridInfo &,
-                               double, void *, hipStream_t);
-template int pm_interp<float>(const float *, const void *, const uint32_t *,
-                              const uint32_t *, uint32_t, const GridInfo &,
-                              double, void *, hipStream_t);
+#define CUCO_PM_INST(T)                                                              \
+    template int pm_fft_x<T>(Cx<T> *, const BoxSet &, bool, const PmTables &,        \
+                             hipStream_t);                                           \
+    template int pm_deposit<T>(const void *, const uint32_t *, const uint32_t *,     \
+                               const BoxSet &, int, T *, void *, hipStream_t);       \
+    template int pm_fft_forward<T>(const T *, Cx<T> *, const BoxSet &,               \
+                                   const PmTables &, hipStream_t);                   \
+    template int pm_green_x<T>(Cx<T> *, const BoxSet &, const PmTables &,            \
+                               hipStream_t);                                         \
+    template int pm_fft_inverse<T>(Cx<T> *, T *, const BoxSet &, const PmTables &,   \
+                                   hipStream_t);                                     \
+    template int pm_interp<T>(const T *, const void *, const uint32_t *,             \
+                              const uint32_t *, const BoxSet &, int, void *,         \
+                              hipStream_t);
+CUCO_PM_INST(double)
+CUCO_PM_INST(float)
+#undef CUCO_PM_INST
 
 }  // namespace cuco

@@ -34,21 +34,6 @@ constexpr int kRanges = 128;   // cell ranges per wave before the j stream
 constexpr int kJBuf = 128;     // staged j particles per wave
 constexpr int kTab = 4096;     // LDS erfc table; zero from the cut bin on
 
-// All lengths of the kernel are scaled by s = asmthfac (table bins per
-// length unit), so a pair's table bin is simply (int)r.  A pair term in
-// scaled units is dx_s * m * rinv_s^3 * T; the physical acceleration is
-// G * s^2 times the scaled sum.
-struct P32 {
-    float s;                      // asmthfac
-    float rcut, rcut2, filter2;   // scaled
-    float h, h_inv, h3_inv;       // scaled spline softening
-    float cellf, inv_cellf;       // scaled cell size
-    float eps;                    // added to r^2: keeps rsq finite at r = 0
-    float out_scale;              // G * s^2
-    float mass;                   // uniform particle mass (UNIFORM variant)
-    int M;
-};
-
 __device__ inline void wave_sync()
 {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -209,14 +194,18 @@ __device__ inline void kahan_add(float &sum, float &comp, float v)
     sum = t;
 }
 
+// Tiles of all boxes form one work list; tv.map holds each box's first
+// tile, p32s[b] the scaled constants of box b.
 template <bool GENERIC, bool UNIFORM>
 __global__ __launch_bounds__(kThreads) void k_pp32(
     const float4 *__restrict__ spos, const uint32_t *__restrict__ skeys,
     const uint32_t *__restrict__ perm, const uint32_t *__restrict__ off,
-    GridInfo g, P32 pp, const float *__restrict__ table32,
+    BoxView tv, const P32 *__restrict__ p32s, float mass,
+    const float *__restrict__ table32,
     const uint32_t *__restrict__ tstart, const uint32_t *__restrict__ tcount,
     uint32_t *__restrict__ counters, float4 *__restrict__ acc,
-    unsigned long long *__restrict__ cand)
+    unsigned long long *__restrict__ cand,
+    const int32_t *__restrict__ active_end, int32_t ti_current)
 {
     __shared__ float s_tab[kTab];
     __shared__ uint32_t s_rs[kWaves][kRanges];
@@ -225,7 +214,7 @@ __global__ __launch_bounds__(kThreads) void k_pp32(
     __shared__ float4 s_j[kWaves][kJBuf];
 
     for (int k = threadIdx.x; k < kTab; k += kThreads)
-        s_tab[k] = k < kTabPadded ? (UNIFORM ? pp.mass * table32[k] : table32[k]) : 0.0f;
+        s_tab[k] = k < kTabPadded ? (UNIFORM ? mass * table32[k] : table32[k]) : 0.0f;
     __syncthreads();
 
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -235,7 +224,6 @@ __global__ __launch_bounds__(kThreads) void k_pp32(
     float4 *sj = s_j[w];
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     const uint32_t ntiles = counters[0];
-    const int M = pp.M;
     const float inf = __builtin_huge_valf();
 
     for (;;) {
@@ -246,14 +234,24 @@ __global__ __launch_bounds__(kThreads) void k_pp32(
         t = __builtin_amdgcn_readlane(t, 0);
         if (t >= ntiles) break;
 
+        const int bi = find_box(tv.map, tv.nboxes, t);
+        const P32 pp = p32s[bi];
+        const GridInfo &g = tv.box[bi].g;
+        const uint32_t key_base = tv.box[bi].key_base;
+        const int M = pp.M;
         const uint32_t i0 = tstart[t];
         const uint32_t cnt = tcount[t];
         const bool valid = (uint32_t)lane < cnt;
         const uint32_t i = i0 + (valid ? (uint32_t)lane : 0u);
+        // Active set: a tile without an active target is skipped; in the
+        // others every lane computes and only active lanes store.
+        bool act = valid;
+        if (active_end) act = valid && active_end[perm[i]] == ti_current;
+        if (uni((uint32_t)(__ballot(act) != 0ull)) == 0u) continue;
 
         int c0x, c0y, c0z, cx, cy, cz;
-        key_cell(g, skeys[i0], c0x, c0y, c0z);
-        key_cell(g, skeys[i], cx, cy, cz);
+        key_cell(g, skeys[i0] - key_base, c0x, c0y, c0z);
+        key_cell(g, skeys[i] - key_base, cx, cy, cz);
         const float4 pi = spos[i];
         const float xi = pi.x * pp.s + (float)unwrap_delta(cx - c0x, M) * pp.cellf;
         const float yi = pi.y * pp.s + (float)unwrap_delta(cy - c0y, M) * pp.cellf;
@@ -304,7 +302,7 @@ __global__ __launch_bounds__(kThreads) void k_pp32(
                         ux += ux < 0 ? M : 0;
                         uy += uy < 0 ? M : 0;
                         uz += uz < 0 ? M : 0;
-                        const uint32_t key = cell_key(g, ux, uy, uz);
+                        const uint32_t key = key_base + cell_key(g, ux, uy, uz);
                         s = off[key];
                         c = off[key + 1] - s;
                         ok = c > 0;
@@ -389,7 +387,7 @@ __global__ __launch_bounds__(kThreads) void k_pp32(
             jc = 0;
         }
         wave_sync();
-        if (valid)
+        if (act)
             acc[perm[i]] = make_float4(ax * pp.out_scale, ay * pp.out_scale,
                                        az * pp.out_scale, 0.f);
         if (cand) atomicAdd(cand, lane == 0 ? ncand : 0ull);
@@ -423,13 +421,25 @@ __global__ void k_fill_tiles(const uint32_t *__restrict__ off,
     }
 }
 
+// First tile of each box: boxes start on Morton block boundaries, so it is
+// the scanned tile offset of the box's first block.
+__global__ void k_box_tiles(const uint32_t *__restrict__ tile_off,
+                            const BoxDev *__restrict__ boxes, int nboxes,
+                            int shift3, const uint32_t *__restrict__ counters,
+                            uint32_t *__restrict__ tile_map)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < nboxes) tile_map[b] = tile_off[boxes[b].key_base >> shift3];
+    if (b == nboxes) tile_map[b] = counters[0];
+}
+
 }  // namespace
 
-int launch_build_tiles(const uint32_t *off, uint32_t n, const GridInfo &g,
+int launch_build_tiles(const uint32_t *off, const BoxSet &bs, uint32_t nkeys,
                        TileInfo &t, hipStream_t s)
 {
     const int shift3 = 3 * t.block_shift;
-    const uint32_t nblocks = g.nkeys >> shift3;
+    const uint32_t nblocks = nkeys >> shift3;
     // t.nblocks is the capacity (blocks of the smallest tunable shift).
     if (nblocks == 0 || nblocks > t.nblocks) {
         set_error("tile build: %u Morton blocks exceed capacity %u", nblocks,
@@ -447,18 +457,14 @@ int launch_build_tiles(const uint32_t *off, uint32_t n, const GridInfo &g,
                                                         t.tile_start,
                                                         t.tile_count);
     CUCO_LAUNCH_CHECK();
-    (void)n;
+    k_box_tiles<<<grid_for(bs.nboxes + 1, 64), 64, 0, s>>>(
+        t.block_tiles, bs.dev[kKeyMorton], bs.nboxes, shift3, t.counters, bs.tile_map);
+    CUCO_LAUNCH_CHECK();
     return CUCO_OK;
 }
 
-int launch_pp32(const float4 *spos, const uint32_t *skeys,
-                const uint32_t *perm, const uint32_t *off, uint32_t n,
-                const GridInfo &g, const PairParams &pp, const float *table32,
-                const TileInfo &t, float4 *acc,
-                unsigned long long *cand_counter, int num_workgroups,
-                double uniform_mass, hipStream_t s)
+P32 make_p32(const PairParams &pp, const GridInfo &g, double uniform_mass)
 {
-    if (n == 0) return CUCO_OK;
     const double sc = pp.asmthfac;
     P32 p;
     p.s = (float)sc;
@@ -474,24 +480,43 @@ int launch_pp32(const float4 *spos, const uint32_t *skeys,
     p.out_scale = (float)(pp.G * sc * sc);
     p.mass = (float)uniform_mass;
     p.M = g.M;
+    return p;
+}
+
+bool pp32_table_cut(const PairParams &pp, const GridInfo &g, int block_shift)
+{
     // The table alone can make the cut when rcut sits on a bin edge and no
     // candidate can reach past the LDS table: a candidate lies within rcut
     // (x 1.0001) of a tile's box, and the box spans at most one Morton block
     // (plus the clamp of the last cell), so r <= rcut + sqrt(3)(2^shift+1) cell.
-    const double edge = pp.asmthfac * pp.rcut;
+    const double sc = pp.asmthfac;
+    const double edge = sc * pp.rcut;
     const bool aligned = fabs(edge - nearbyint(edge)) < 1e-9 * edge;
-    const double rmax = sc * (pp.rcut * 1.001 +
-                              sqrt(3.0) * ((1 << t.block_shift) + 1) * g.cell);
-    const bool generic = !(aligned && rmax < kTab - 2);
+    const double rmax = sc * (pp.rcut * 1.001 + sqrt(3.0) * ((1 << block_shift) + 1) * g.cell);
+    return aligned && rmax < kTab - 2;
+}
+
+int launch_pp32(const float4 *spos, const uint32_t *skeys,
+                const uint32_t *perm, const uint32_t *off, const BoxSet &bs,
+                const P32 *p32s, bool table_cut, double uniform_mass,
+                const float *table32, const TileInfo &t, float4 *acc,
+                unsigned long long *cand_counter, int num_workgroups,
+                const int32_t *active_end, int32_t ti_current, hipStream_t s)
+{
+    if (bs.total[kMapPart] == 0) return CUCO_OK;
+    const bool generic = !table_cut;
     const bool uniform = uniform_mass > 0;
+    const float mass = (float)uniform_mass;
+    const BoxView tv{bs.dev[kKeyMorton], bs.tile_map, bs.nboxes};
     int rc = launch_fill_u32(t.counters + 1, 0u, 1, s);
     if (rc) return rc;
     const int nwg = num_workgroups > 0 ? num_workgroups : 2048;
-#define CUCO_PP32(GENERIC, UNIFORM)                                             \
-    k_pp32<GENERIC, UNIFORM><<<nwg, kThreads, 0, s>>>(spos, skeys, perm, off, g, \
-                                                      p, table32, t.tile_start,  \
-                                                      t.tile_count, t.counters,  \
-                                                      acc, cand_counter)
+#define CUCO_PP32(GENERIC, UNIFORM)                                               \
+    k_pp32<GENERIC, UNIFORM><<<nwg, kThreads, 0, s>>>(spos, skeys, perm, off, tv,  \
+                                                      p32s, mass, table32,         \
+                                                      t.tile_start, t.tile_count,  \
+                                                      t.counters, acc, cand_counter, \
+                                                      active_end, ti_current)
     if (generic) {
         if (uniform) CUCO_PP32(true, true); else CUCO_PP32(true, false);
     } else {

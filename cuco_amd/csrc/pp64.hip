@@ -74,15 +74,18 @@ __device__ inline double gap_cells(double u, int c)
 template <bool COUNT>
 __global__ __launch_bounds__(kThreads) void k_pp64(
     const double4 *__restrict__ spos, const uint32_t *__restrict__ perm,
-    const uint32_t *__restrict__ off, uint32_t n, PairParams pp,
+    const uint32_t *__restrict__ off, uint32_t i0, uint32_t n, PairParams pp,
     const float *__restrict__ table, double4 *__restrict__ acc,
-    unsigned long long *__restrict__ counter)
+    unsigned long long *__restrict__ counter,
+    const int32_t *__restrict__ active_end, int32_t ti_current)
 {
     __shared__ double tab[kNTab];
     for (int k = threadIdx.x; k < kNTab; k += kThreads) tab[k] = (double)table[k];
     __syncthreads();
-    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
-    if (i >= n) return;
+    const uint32_t il = blockIdx.x * kThreads + threadIdx.x;
+    if (il >= n) return;
+    const uint32_t i = i0 + il;
+    if (active_end && active_end[perm[i]] != ti_current) return;
 
     const int M = pp.M;
     const double fac = pp.fac;
@@ -150,17 +153,22 @@ __global__ __launch_bounds__(kThreads) void k_pp64(
 
 }  // namespace
 
+// One box: its sorted particles are [i0, i0+n) and off points at the cell
+// table entry of the box's first key (entries are global sorted indices).
 int launch_pp64(const double4 *spos, const uint32_t *perm, const uint32_t *off,
-                uint32_t n, const PairParams &pp, const float *table,
-                double4 *acc, unsigned long long *pair_counter, hipStream_t s)
+                uint32_t i0, uint32_t n, const PairParams &pp, const float *table,
+                double4 *acc, unsigned long long *pair_counter,
+                const int32_t *active_end, int32_t ti_current, hipStream_t s)
 {
     if (n == 0) return CUCO_OK;
     if (pair_counter)
         k_pp64<true><<<grid_for(n, kThreads), kThreads, 0, s>>>(
-            spos, perm, off, n, pp, table, acc, pair_counter);
+            spos, perm, off, i0, n, pp, table, acc, pair_counter, active_end,
+            ti_current);
     else
         k_pp64<false><<<grid_for(n, kThreads), kThreads, 0, s>>>(
-            spos, perm, off, n, pp, table, acc, nullptr);
+            spos, perm, off, i0, n, pp, table, acc, nullptr, active_end,
+            ti_current);
     CUCO_LAUNCH_CHECK();
     return CUCO_OK;
 }

@@ -56,16 +56,22 @@ __device__ inline uint32_t block_excl_scan256(uint32_t v, uint32_t *sh,
     return off + inc - v;
 }
 
-__global__ void k_calc_keys(const double4 *__restrict__ pos, uint32_t n,
-                            GridInfo g, uint32_t *__restrict__ keys)
+__global__ void k_calc_keys(const double4 *__restrict__ pos, BoxView v,
+                            uint32_t *__restrict__ keys)
 {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    int bi;
+    uint32_t lb;
+    if (!box_block(v, bi, lb)) return;
+    const BoxDev &bx = v.box[bi];
+    const uint32_t il = lb * blockDim.x + threadIdx.x;
+    if (il >= bx.n) return;
+    const uint32_t i = bx.p0 + il;
+    const GridInfo &g = bx.g;
     double4 p = pos[i];
     int ix = cell_coord(g.fac, p.x, g.M);
     int iy = cell_coord(g.fac, p.y, g.M);
     int iz = cell_coord(g.fac, p.z, g.M);
-    keys[i] = cell_key(g, ix, iy, iz);
+    keys[i] = bx.key_base + cell_key(g, ix, iy, iz);
 }
 
 __global__ __launch_bounds__(kThreads) void k_rs_upsweep(
@@ -249,14 +255,20 @@ __global__ void k_reorder64(const double4 *__restrict__ pos,
 // Cell-relative FP32 coordinates: x - ix*cell computed in FP64, then rounded.
 __global__ void k_reorder32(const double4 *__restrict__ pos,
                             const uint32_t *__restrict__ perm,
-                            const uint32_t *__restrict__ skeys, uint32_t n,
-                            GridInfo g, float4 *__restrict__ spos)
+                            const uint32_t *__restrict__ skeys, BoxView v,
+                            float4 *__restrict__ spos)
 {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    int bi;
+    uint32_t lb;
+    if (!box_block(v, bi, lb)) return;
+    const BoxDev &bx = v.box[bi];
+    const uint32_t il = lb * blockDim.x + threadIdx.x;
+    if (il >= bx.n) return;
+    const uint32_t i = bx.p0 + il;
+    const GridInfo &g = bx.g;
     double4 p = pos[perm[i]];
     int ix, iy, iz;
-    key_cell(g, skeys[i], ix, iy, iz);
+    key_cell(g, skeys[i] - bx.key_base, ix, iy, iz);
     spos[i] = make_float4((float)(p.x - ix * g.cell), (float)(p.y - iy * g.cell),
                           (float)(p.z - iz * g.cell), (float)p.w);
 }
@@ -339,11 +351,12 @@ __global__ void k_fill_u32(uint32_t *p, uint32_t value, size_t n)
 
 }  // namespace
 
-int launch_calc_keys(const double4 *pos, uint32_t n, const GridInfo &g,
+int launch_calc_keys(const double4 *pos, const BoxSet &bs, int mode,
                      uint32_t *keys, hipStream_t s)
 {
-    if (n == 0) return CUCO_OK;
-    k_calc_keys<<<grid_for(n, 256), 256, 0, s>>>(pos, n, g, keys);
+    if (bs.total[kMapPart] == 0) return CUCO_OK;
+    k_calc_keys<<<xcd_grid(bs.total[kMapPart]), 256, 0, s>>>(
+        pos, bs.view(mode, kMapPart), keys);
     CUCO_LAUNCH_CHECK();
     return CUCO_OK;
 }
@@ -421,11 +434,12 @@ int launch_reorder64(const double4 *pos, const uint32_t *perm, uint32_t n,
 }
 
 int launch_reorder32(const double4 *pos, const uint32_t *perm,
-                     const uint32_t *skeys, uint32_t n, const GridInfo &g,
+                     const uint32_t *skeys, const BoxSet &bs, int mode,
                      float4 *spos, hipStream_t s)
 {
-    if (n == 0) return CUCO_OK;
-    k_reorder32<<<grid_for(n, 256), 256, 0, s>>>(pos, perm, skeys, n, g, spos);
+    if (bs.total[kMapPart] == 0) return CUCO_OK;
+    k_reorder32<<<xcd_grid(bs.total[kMapPart]), 256, 0, s>>>(
+        pos, perm, skeys, bs.view(mode, kMapPart), spos);
     CUCO_LAUNCH_CHECK();
     return CUCO_OK;
 }

@@ -45,7 +45,11 @@ EXPORTED = [
     "cuco_count_pairs", "cuco_set_timing", "cuco_get_stage_times",
     "cuco_stage_name", "cuco_synchronize", "cuco_init_drift_table",
     "cuco_table_factor", "cuco_kdk_factors", "cuco_copy_device_to_device",
-    "cuco_get_tuning", "cuco_set_tuning", "cuco_autotune",
+    "cuco_get_tuning", "cuco_set_tuning", "cuco_autotune", "cuco_set_params",
+    "cuco_ensemble_create", "cuco_ensemble_size", "cuco_write_snapshot",
+    "cuco_read_snapshot", "cuco_read_output_list", "cuco_find_next_output_time",
+    "cuco_download_records", "cuco_upload_records", "cuco_timeline_init",
+    "cuco_timeline_run", "cuco_get_timesteps",
 ]
 
 
@@ -85,6 +89,34 @@ class Particle(C.Structure):
     ]
 
 
+class Header(C.Structure):
+    """HEADER record of allvars.h:47-57 (32 bytes)."""
+    _fields_ = [
+        ("NumPart", C.c_int32), ("Mass", C.c_float), ("Time", C.c_float),
+        ("Redshift", C.c_float), ("BoxSize", C.c_float), ("Omega0", C.c_float),
+        ("OmegaLambda", C.c_float), ("HubbleParam", C.c_float),
+    ]
+
+
+class Timeline(C.Structure):
+    """Integrator parameters of begrun.c:33-56."""
+    _fields_ = [(f, C.c_double) for f in (
+        "time_begin", "time_max", "omega0", "omega_lambda", "hubble",
+        "err_tol_int_accuracy", "max_size_timestep", "min_size_timestep",
+        "softening_halo", "softening_halo_max_phys")]
+
+
+class TimelineState(C.Structure):
+    _fields_ = [("ti_current", C.c_int32), ("pm_ti_begstep", C.c_int32),
+                ("pm_ti_endstep", C.c_int32), ("steps", C.c_int32),
+                ("time", C.c_double), ("timebase_interval", C.c_double),
+                ("active", C.c_int64), ("finished", C.c_int32),
+                ("at_stop", C.c_int32)]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 PARTICLE_DTYPE = np.dtype([
     ("Pos", "<f4", 3), ("Mass", "<f4"), ("Vel", "<f4", 3),
     ("GravAccel", "<f4", 3), ("GravPM", "<f4", 3),
@@ -111,6 +143,20 @@ def lib():
         "cuco_create": (C.c_int, [C.POINTER(Params), i64, i32, C.POINTER(vp)]),
         "cuco_destroy": (C.c_int, [vp]),
         "cuco_set_softening": (C.c_int, [vp, C.c_double]),
+        "cuco_set_params": (C.c_int, [vp, i32, C.POINTER(Params)]),
+        "cuco_ensemble_create": (C.c_int, [C.POINTER(Params), C.POINTER(i64), i32,
+                                           i32, C.POINTER(vp)]),
+        "cuco_ensemble_size": (C.c_int, [vp, C.POINTER(i32), C.POINTER(i64)]),
+        "cuco_write_snapshot": (C.c_int, [C.c_char_p, C.POINTER(Header), vp]),
+        "cuco_read_snapshot": (C.c_int, [C.c_char_p, C.POINTER(Header), vp, i64]),
+        "cuco_read_output_list": (C.c_int, [C.c_char_p, dp, i32, C.POINTER(i32)]),
+        "cuco_find_next_output_time": (i32, [dp, i32, C.c_double, C.c_double,
+                                             C.c_double, i32]),
+        "cuco_download_records": (C.c_int, [vp, vp]),
+        "cuco_timeline_init": (C.c_int, [vp, C.POINTER(Timeline)]),
+        "cuco_timeline_run": (C.c_int, [vp, i32, i32, i32, C.POINTER(TimelineState)]),
+        "cuco_get_timesteps": (C.c_int, [vp, C.POINTER(i32), C.POINTER(i32)]),
+        "cuco_upload_records": (C.c_int, [vp, vp, i64]),
         "cuco_upload_particles": (C.c_int, [vp, i64, dp, dp, dp, C.c_double]),
         "cuco_download_particles": (C.c_int, [vp, dp, dp]),
         "cuco_download_accel": (C.c_int, [vp, dp, dp]),
@@ -199,6 +245,51 @@ def kdk_factors(drift, kick, a_begin, a_max, a_start, a_end, nsteps):
     _check(lib().cuco_kdk_factors(_dp(d), _dp(k), a_begin, a_max, a_start,
                                   a_end, nsteps, _dp(f)), "cuco_kdk_factors")
     return f
+
+
+TIMEBASE = 1 << 28  # allvars.h:11
+
+
+def write_snapshot(path: str, header: Header, P: np.ndarray) -> None:
+    """savepositions (io.c:9-40): HEADER + PARTICLE[NumPart]."""
+    P = np.ascontiguousarray(P, PARTICLE_DTYPE)
+    h = Header(*[getattr(header, f) for f, _ in Header._fields_])
+    h.NumPart = P.shape[0]
+    _check(lib().cuco_write_snapshot(os.fsencode(path), C.byref(h),
+                                     P.ctypes.data_as(C.c_void_p)),
+           "cuco_write_snapshot")
+
+
+def read_snapshot(path: str, header_only: bool = False):
+    """read_ic (read_ic.c:9-45): returns (Header, PARTICLE array or None)."""
+    h = Header()
+    _check(lib().cuco_read_snapshot(os.fsencode(path), C.byref(h), None, 0),
+           "cuco_read_snapshot")
+    if header_only:
+        return h, None
+    P = np.zeros(h.NumPart, PARTICLE_DTYPE)
+    _check(lib().cuco_read_snapshot(os.fsencode(path), C.byref(h),
+                                    P.ctypes.data_as(C.c_void_p), P.shape[0]),
+           "cuco_read_snapshot")
+    return h, P
+
+
+def read_output_list(path: str, max_times: int = 500) -> np.ndarray:
+    """read_outputlist (begrun.c:80-102); MAXLEN_OUTPUTLIST is 500."""
+    t = np.zeros(max_times)
+    n = C.c_int32(0)
+    _check(lib().cuco_read_output_list(os.fsencode(path), _dp(t), max_times,
+                                       C.byref(n)), "cuco_read_output_list")
+    return t[:n.value].copy()
+
+
+def find_next_output_time(times, time_begin, time_max, timebase_interval,
+                          ti_curr) -> int:
+    """find_next_outputtime (run.c:75-104)."""
+    t = np.ascontiguousarray(times, np.float64)
+    return int(lib().cuco_find_next_output_time(_dp(t), t.shape[0], time_begin,
+                                                time_max, timebase_interval,
+                                                ti_curr))
 
 
 class DeviceBuffer:
@@ -317,6 +408,18 @@ class Engine:
     def set_softening(self, soft_h: float) -> None:
         _check(lib().cuco_set_softening(self.h, soft_h), "cuco_set_softening")
 
+    def set_params(self, G: float, asmth: float, rcut: float, soft_h: float,
+                   box: int = -1) -> None:
+        """setSimParam: new G, split scale, cut and softening (box -1: all)."""
+        p = Params(self._grid_of(box), 0, self._size_of(box), G, asmth, rcut, soft_h)
+        _check(lib().cuco_set_params(self.h, box, C.byref(p)), "cuco_set_params")
+
+    def _grid_of(self, box: int) -> int:
+        return self.M
+
+    def _size_of(self, box: int) -> float:
+        return self.params.box_size
+
     # -------------------------------------------------------- state
     def upload(self, pos, vel=None, mass=None, uniform_mass: float = 0.0) -> None:
         pos = _xyz(pos)
@@ -369,6 +472,41 @@ class Engine:
         f = np.ascontiguousarray(factors, np.float64).reshape(-1, 3)
         _check(lib().cuco_kdk_steps(self.h, precision, f.shape[0], _dp(f),
                                     int(use_graph)), "cuco_kdk_steps")
+
+    def download_records(self) -> np.ndarray:
+        """The particles as PARTICLE records (savepositions' P[])."""
+        P = np.zeros(self.n, PARTICLE_DTYPE)
+        _check(lib().cuco_download_records(self.h, P.ctypes.data_as(C.c_void_p)),
+               "cuco_download_records")
+        return P
+
+    def upload_records(self, P: np.ndarray) -> None:
+        P = np.ascontiguousarray(P, PARTICLE_DTYPE)
+        _check(lib().cuco_upload_records(self.h, P.ctypes.data_as(C.c_void_p),
+                                         P.shape[0]), "cuco_upload_records")
+        self.n = P.shape[0]
+
+    # -------------------------------------------------------- integer timeline
+    def timeline_init(self, **kw) -> None:
+        """init + init_drift_table; keywords are the Timeline fields."""
+        t = Timeline(**kw)
+        _check(lib().cuco_timeline_init(self.h, C.byref(t)), "cuco_timeline_init")
+
+    def timeline_run(self, precision: int = FP64, max_steps: int = 1,
+                     ti_stop: int = -1) -> dict:
+        """Up to max_steps passes of run()'s body; returns the state."""
+        s = TimelineState()
+        _check(lib().cuco_timeline_run(self.h, precision, max_steps, ti_stop,
+                                       C.byref(s)), "cuco_timeline_run")
+        return s.as_dict()
+
+    def timesteps(self):
+        b = np.empty(self.n, np.int32)
+        e = np.empty(self.n, np.int32)
+        _check(lib().cuco_get_timesteps(self.h, b.ctypes.data_as(C.POINTER(C.c_int32)),
+                                        e.ctypes.data_as(C.POINTER(C.c_int32))),
+               "cuco_get_timesteps")
+        return b, e
 
     # -------------------------------------------------------- compat shims
     def pmforce_particles(self, P: np.ndarray, precision: int = FP64) -> None:
@@ -452,3 +590,75 @@ class Engine:
 
     def synchronize(self) -> None:
         _check(lib().cuco_synchronize(self.h), "cuco_synchronize")
+
+
+class Ensemble(Engine):
+    """Independent periodic boxes advanced together (cuco_ensemble_create).
+
+    ``boxes`` is a list of dicts with the keys of :class:`Engine`'s
+    parameters (pm_grid, box_size, G, asmth, rcut, soft_h); ``n_per_box``
+    the particle count of each box.  Particle arrays hold box 0's particles,
+    then box 1's, and so on.
+    """
+
+    def __init__(self, boxes, n_per_box, device: int = 0):
+        nb = len(boxes)
+        if nb == 0 or len(n_per_box) != nb:
+            raise ValueError("need one particle count per box")
+        arr = (Params * nb)()
+        for k, b in enumerate(boxes):
+            arr[k] = Params(int(b["pm_grid"]), 0, float(b["box_size"]), float(b["G"]),
+                            float(b["asmth"]), float(b["rcut"]), float(b["soft_h"]))
+        counts = (C.c_int64 * nb)(*[int(x) for x in n_per_box])
+        h = C.c_void_p()
+        _check(lib().cuco_ensemble_create(arr, counts, nb, device, C.byref(h)),
+               "cuco_ensemble_create")
+        self.h = h
+        self.box_params = arr
+        self.params = arr[0]
+        self.grids = [int(b["pm_grid"]) for b in boxes]
+        self.M = self.grids[0]
+        self.counts = [int(x) for x in n_per_box]
+        self.offsets = np.concatenate([[0], np.cumsum(self.counts)]).astype(np.int64)
+        self.n = 0
+
+    def _grid_of(self, box: int) -> int:
+        return self.grids[max(box, 0)]
+
+    def _size_of(self, box: int) -> float:
+        return self.box_params[max(box, 0)].box_size
+
+    def set_params(self, G, asmth, rcut, soft_h, box: int = -1) -> None:
+        boxes = range(len(self.grids)) if box < 0 else [box]
+        for b in boxes:
+            p = Params(self.grids[b], 0, self.box_params[b].box_size, G, asmth, rcut, soft_h)
+            _check(lib().cuco_set_params(self.h, b, C.byref(p)), "cuco_set_params")
+
+    def size(self):
+        nb = C.c_int32(0)
+        _check(lib().cuco_ensemble_size(self.h, C.byref(nb), None), "cuco_ensemble_size")
+        counts = (C.c_int64 * nb.value)()
+        _check(lib().cuco_ensemble_size(self.h, C.byref(nb), counts), "cuco_ensemble_size")
+        return nb.value, list(counts)
+
+    def box_slice(self, b: int) -> slice:
+        return slice(int(self.offsets[b]), int(self.offsets[b + 1]))
+
+    def cell_table(self):
+        nc = sum(m ** 3 for m in self.grids)
+        s = np.empty(nc, np.uint32)
+        e = np.empty(nc, np.uint32)
+        _check(lib().cuco_get_cell_table(self.h, s.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                         e.ctypes.data_as(C.POINTER(C.c_uint32))),
+               "cuco_get_cell_table")
+        return s, e
+
+    def pm_grid(self, which: int):
+        """List of each box's mesh."""
+        out = np.empty(sum(m ** 3 for m in self.grids))
+        _check(lib().cuco_get_pm_grid(self.h, which, _dp(out)), "cuco_get_pm_grid")
+        res, o = [], 0
+        for m in self.grids:
+            res.append(out[o:o + m ** 3].reshape(m, m, m))
+            o += m ** 3
+        return res

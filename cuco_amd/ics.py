@@ -67,7 +67,9 @@ class Box:
     @property
     def mass(self) -> float:
         rho_crit = 3 * HUBBLE_INTERNAL ** 2 / (8 * math.pi * G_INTERNAL)
-        return OMEGA_M * rho_crit * self.L ** 3 / self.N
+        # (L / n_side)^3 rather than L^3 / N: boxes of equal mean spacing get
+        # bit-identical masses.
+        return OMEGA_M * rho_crit * (self.L / self.n_side) ** 3
 
     @property
     def asmth(self) -> float:
@@ -144,7 +146,7 @@ def zeldovich(box: Box):
     delta[0, 0, 0] = 0.0
     psi = []
     for K in (KX, KY, KZ):
-        psi.append(np.fft.irfftn(1j * K / k2 * delta, s=(n, n, n)))
+        psi.append(np.fft.irfftn(1j * K / k2 * delta, s=(n, n, n), axes=(0, 1, 2)))
     psi = np.stack([p.reshape(-1) for p in psi], axis=1)
     spacing = box.L / n
     rms = math.sqrt(np.mean(np.sum(psi ** 2, axis=1)))

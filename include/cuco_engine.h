@@ -60,6 +60,19 @@ typedef struct cuco_particle {
     int32_t Ti_begstep;
 } cuco_particle;
 
+/* HEADER record of a snapshot / IC file, byte-compatible with
+ * allvars.h:47-57 (32 bytes). */
+typedef struct cuco_header {
+    int32_t NumPart;
+    float Mass;
+    float Time;
+    float Redshift;
+    float BoxSize;
+    float Omega0;
+    float OmegaLambda;
+    float HubbleParam;
+} cuco_header;
+
 typedef struct cuco_engine cuco_engine;
 
 /* ---------------------------------------------------------------- */
@@ -81,8 +94,35 @@ int cuco_create(const cuco_params *params, int64_t max_particles,
 /* Frees everything the handle owns (the reference never frees). */
 int cuco_destroy(cuco_engine *e);
 /* set_softenings + setSoftParam (gravlist.cpp:28-45, gravlist.cu:179-181):
- * re-sets the spline softening h (comoving), e.g. each step. */
+ * re-sets the spline softening h (comoving), e.g. each step, in every box. */
 int cuco_set_softening(cuco_engine *e, double soft_h);
+/* setSimParam (gravlist.cu:173-177) with linklist_init's derived constants
+ * (gravlist.cpp:61-70) and pm_init_periodic's split scale
+ * (pm_periodic.c:27-28): replaces G, asmth, rcut and soft_h of box `box`
+ * (-1: every box).  pm_grid and box_size must equal the values the handle
+ * was created with. */
+int cuco_set_params(cuco_engine *e, int32_t box, const cuco_params *params);
+
+/* ---------------------------------------------------------------- */
+/* Ensembles of independent boxes                                    */
+/* ---------------------------------------------------------------- */
+
+/* An ensemble advances nboxes independent periodic boxes together: each
+ * stage of a force evaluation or KDK step (keys, sort, cell table, pair
+ * forces, deposit, each FFT pass, interpolation, drift, kick) is one launch
+ * over all boxes, and cuco_kdk_steps captures the whole step as one graph.
+ * Box b has its own params[b] (mesh, box size, split scale, softening) and
+ * exactly n_per_box[b] particles; in every particle array of the handle
+ * (upload, download, accelerations) its particles follow those of boxes
+ * 0..b-1.  Every cuco_* call that takes a handle accepts an ensemble
+ * handle; cuco_upload_particles then takes exactly sum(n_per_box)
+ * particles.  Each box gives the same forces, bit for bit, as a
+ * single-box handle holding the same particles.  The reference runs one box
+ * per process; this has no counterpart there. */
+int cuco_ensemble_create(const cuco_params *params, const int64_t *n_per_box,
+                         int32_t nboxes, int32_t device, cuco_engine **out);
+/* Number of boxes and (if n_per_box is not NULL) the particles of each. */
+int cuco_ensemble_size(cuco_engine *e, int32_t *nboxes, int64_t *n_per_box);
 
 /* ---------------------------------------------------------------- */
 /* Particle state (device resident, FP64)                            */
@@ -192,6 +232,94 @@ int cuco_gravity_linklist(cuco_engine *e, cuco_particle *P, int64_t n,
                           int32_t precision);
 
 /* ---------------------------------------------------------------- */
+/* Integer timeline: individual power-of-two timesteps               */
+/* ---------------------------------------------------------------- */
+
+#define CUCO_TIMEBASE (1 << 28) /* allvars.h:11 */
+
+/* Run parameters of the reference's integrator (begrun.c:33-56). */
+typedef struct cuco_timeline {
+    double time_begin;              /* All.TimeBegin (scale factor) */
+    double time_max;                /* All.TimeMax */
+    double omega0;                  /* All.Omega0 */
+    double omega_lambda;            /* All.OmegaLambda */
+    double hubble;                  /* All.Hubble (internal units) */
+    double err_tol_int_accuracy;    /* All.ErrTolIntAccuracy */
+    double max_size_timestep;       /* All.MaxSizeTimestep (in ln a) */
+    double min_size_timestep;       /* All.MinSizeTimestep */
+    double softening_halo;          /* All.SofteningHalo (comoving) */
+    double softening_halo_max_phys; /* All.SofteningHaloMaxPhys */
+} cuco_timeline;
+
+/* Integer-timeline state after a cuco_timeline_run call. */
+typedef struct cuco_timeline_state {
+    int32_t ti_current;       /* All.Ti_Current */
+    int32_t pm_ti_begstep;    /* All.PM_Ti_begstep */
+    int32_t pm_ti_endstep;    /* All.PM_Ti_endstep */
+    int32_t steps;            /* All.NumCurrentTiStep */
+    double time;              /* TimeBegin * exp(Ti_Current * Timebase_interval) */
+    double timebase_interval; /* All.Timebase_interval */
+    int64_t active;           /* particles whose step ended in the last step */
+    int32_t finished;         /* Ti_Current reached TIMEBASE or Time > TimeMax */
+    int32_t at_stop;          /* the call returned at ti_stop */
+} cuco_timeline_state;
+
+/* init (init.c:20-48) and init_drift_table (driftfac.c:14-42): Time =
+ * TimeBegin, Ti_Current = 0, every Ti_begstep = Ti_endstep = 0 and
+ * PM_Ti_begstep = PM_Ti_endstep = 0; builds the drift and kick tables.
+ * Needs uploaded particles (internal velocities). */
+int cuco_timeline_init(cuco_engine *e, const cuco_timeline *t);
+/* Up to max_steps passes of run()'s body (run.c:9-25): find the next sync
+ * point and drift every particle to it (run.c:29-64); set the softening for
+ * the new time (gravlist.cpp:28-45); compute_accelerations (accel.c:8-23:
+ * PM on PM steps; the short-range force for the particles whose step ends
+ * now -- the reference computes it for every particle but kicks only
+ * these); advance_and_find_timesteps (timestep.c:11-94: new power-of-two
+ * steps from get_timestep, the short-range kick, the PM kick on PM steps).
+ * When the next sync point reaches ti_stop > Ti_Current (the next output
+ * time, cuco_find_next_output_time) the particles are drifted to ti_stop and
+ * the call returns with at_stop = 1, where the reference calls
+ * savepositions (run.c:47-56); the next call continues from there.
+ * ti_stop < 0: no stop.  A step that leaves the integer timeline
+ * (timestep.c:126-133) returns CUCO_ERR_STATE. */
+int cuco_timeline_run(cuco_engine *e, int32_t precision, int32_t max_steps,
+                      int32_t ti_stop, cuco_timeline_state *state);
+/* Ti_begstep / Ti_endstep of every particle (original order); either may
+ * be NULL. */
+int cuco_get_timesteps(cuco_engine *e, int32_t *ti_begstep, int32_t *ti_endstep);
+
+/* ---------------------------------------------------------------- */
+/* Snapshots and the output schedule (host side)                     */
+/* ---------------------------------------------------------------- */
+
+/* savepositions (io.c:9-40): writes HEADER then header->NumPart PARTICLE
+ * records, native byte order. */
+int cuco_write_snapshot(const char *path, const cuco_header *header,
+                        const cuco_particle *P);
+/* read_ic (read_ic.c:9-45): reads the HEADER and, when P is not NULL, the
+ * NumPart records into P (capacity records).  A file whose size does not
+ * match its HEADER is rejected. */
+int cuco_read_snapshot(const char *path, cuco_header *header, cuco_particle *P,
+                       int64_t capacity);
+/* read_outputlist (begrun.c:80-102): up to max_times scale factors (read
+ * as float, as the reference does) into times; *count receives how many. */
+int cuco_read_output_list(const char *path, double *times, int32_t max_times,
+                          int32_t *count);
+/* find_next_outputtime (run.c:75-104): smallest integer time of a listed
+ * scale factor in [time_begin, time_max] after ti_curr, else 2*TIMEBASE. */
+int32_t cuco_find_next_output_time(const double *times, int32_t count,
+                                   double time_begin, double time_max,
+                                   double timebase_interval, int32_t ti_curr);
+/* The handle's particles as PARTICLE records in original order (P[] of
+ * savepositions): Pos, Mass, internal Vel, the last GravAccel (PP) and
+ * GravPM, and the integer timeline's Ti_endstep/Ti_begstep (0 before
+ * cuco_timeline_init). */
+int cuco_download_records(cuco_engine *e, cuco_particle *P);
+/* Uploads Pos, Vel (internal velocities, as P[] holds them after init.c:
+ * 40-44) and Mass of n PARTICLE records; as cuco_upload_particles. */
+int cuco_upload_records(cuco_engine *e, const cuco_particle *P, int64_t n);
+
+/* ---------------------------------------------------------------- */
 /* Device-pointer API (mirrors proto.cuh:26-58)                      */
 /* ---------------------------------------------------------------- */
 
@@ -234,7 +362,8 @@ int cuco_get_cell_table(cuco_engine *e, uint32_t *cell_start,
 /* The 1000-entry short-range table uploaded to the device (gravlist.cpp:56-59). */
 int cuco_get_shortrange_table(cuco_engine *e, float *table);
 /* Mesh of the last PM evaluation: which = 0 density (after deposit),
- * 1 potential (after c2r).  PMGRID^3 doubles, x-major. */
+ * 1 potential (after c2r).  PMGRID^3 doubles, x-major (an ensemble: every
+ * box's mesh in box order). */
 int cuco_get_pm_grid(cuco_engine *e, int32_t which, double *out);
 /* Exact count of ordered pairs i != j with r^2 <= rcut^2 (FP64 test) and
  * of candidate pairs the FP32 kernel evaluates, for the current positions. */

@@ -57,6 +57,7 @@ def lib():
         L = C.CDLL(_LIB_PATH)
         dp, fp, up, i64p = (C.POINTER(C.c_double), C.POINTER(C.c_float),
                             C.POINTER(C.c_uint32), C.POINTER(C.c_int64))
+        i32p = C.POINTER(C.c_int32)
         P = C.POINTER(OrcParams)
         sigs = {
             "orc_shortrange_table": (None, [fp]),
@@ -80,6 +81,9 @@ def lib():
             "orc_table_factor": (C.c_double, [dp] + [C.c_double] * 4),
             "orc_kdk_run": (C.c_int, [P, dp, dp, dp, C.c_int64, C.c_int32, dp,
                                       dp, dp]),
+            "orc_timeline_run": (C.c_int, [P, dp, dp, dp, dp, dp, dp, C.c_int64,
+                                           i32p, i32p, i32p, dp, dp, dp,
+                                           C.c_int32, C.c_int32, i32p]),
         }
         for name, (res, args) in sigs.items():
             f = getattr(L, name)
@@ -261,3 +265,44 @@ def kdk_run(p, pos, vel, mass, factors):
     if rc != 0:
         raise RuntimeError(f"orc_kdk_run failed: {rc}")
     return pos, vel, app, apm
+
+
+TIMELINE_FIELDS = ("time_begin", "time_max", "omega0", "omega_lambda", "hubble",
+                   "err_tol_int_accuracy", "max_size_timestep",
+                   "min_size_timestep", "softening_halo",
+                   "softening_halo_max_phys")
+
+
+class Timeline:
+    """Integer-timeline state of the oracle run (arrays updated in place)."""
+
+    def __init__(self, p, tl: dict, drift, kick, pos, vel, mass):
+        self.p = p
+        self.tl = np.array([tl[k] for k in TIMELINE_FIELDS], np.float64)
+        self.drift = np.ascontiguousarray(drift, np.float64)
+        self.kick = np.ascontiguousarray(kick, np.float64)
+        self.pos = _pos(pos).copy()
+        self.vel = _pos(vel).copy()
+        self.mass = np.ascontiguousarray(mass, np.float64)
+        n = self.pos.shape[0]
+        self.ti_beg = np.zeros(n, np.int32)
+        self.ti_end = np.zeros(n, np.int32)
+        self.state = np.zeros(4, np.int32)
+        self.time = C.c_double(tl["time_begin"])
+        self.app = np.zeros((n, 3))
+        self.apm = np.zeros((n, 3))
+        self.status = np.zeros(3, np.int32)
+
+    def run(self, max_steps, ti_stop=-1):
+        i32 = lambda a: a.ctypes.data_as(C.POINTER(C.c_int32))
+        rc = lib().orc_timeline_run(
+            C.byref(self.p), _d(self.tl), _d(self.drift), _d(self.kick),
+            _d(self.pos), _d(self.vel), _d(self.mass), self.pos.shape[0],
+            i32(self.ti_beg), i32(self.ti_end), i32(self.state), C.byref(self.time),
+            _d(self.app), _d(self.apm), max_steps, ti_stop, i32(self.status))
+        if rc != 0:
+            raise RuntimeError(f"orc_timeline_run failed: {rc}")
+        return {"ti_current": int(self.state[0]), "pm_ti_begstep": int(self.state[1]),
+                "pm_ti_endstep": int(self.state[2]), "steps": int(self.state[3]),
+                "time": self.time.value, "finished": int(self.status[0]),
+                "at_stop": int(self.status[1]), "flags": int(self.status[2])}

@@ -756,3 +756,139 @@ int orc_kdk_run(const orc_params *p, double *pos, double *vel,
     }
     return rc;
 }
+
+/* ------------------------------------------------------------------ */
+/* Integer timeline (individual power-of-two timesteps)                */
+/* ------------------------------------------------------------------ */
+
+#define ORC_TIMEBASE (1 << 28) /* allvars.h:11 */
+
+/* get_drift_factor / get_gravkick_factor at two integer times
+ * (driftfac.c:66-126). */
+static double ti_factor(const double *table, double lb, double lm,
+                        double interval, int time0, int time1)
+{
+    double a1 = lb + time0 * interval;
+    double a2 = lb + time1 * interval;
+    return table_at(table, lb, lm, a2) - table_at(table, lb, lm, a1);
+}
+
+int orc_timeline_run(const orc_params *p0, const double *tl,
+                     const double *drift_tab, const double *kick_tab,
+                     double *pos, double *vel, const double *mass, int64_t n,
+                     int32_t *ti_beg, int32_t *ti_end, int32_t *state,
+                     double *time, double *acc_pp, double *acc_pm,
+                     int32_t max_steps, int32_t ti_stop, int32_t *status)
+{
+    const double time_begin = tl[0], time_max = tl[1], omega0 = tl[2];
+    const double omega_lambda = tl[3], hubble = tl[4], err_tol = tl[5];
+    const double max_dt = tl[6], min_dt = tl[7], soft_halo = tl[8];
+    const double soft_max_phys = tl[9];
+    const double lb = log(time_begin), lm = log(time_max);
+    const double interval = (log(time_max) - log(time_begin)) / ORC_TIMEBASE; /* init.c:23 */
+    orc_params p = *p0;
+    int32_t ti_current = state[0], pm_beg = state[1], pm_end = state[2];
+    int32_t steps = state[3];
+    int finished = 0, at_stop = 0, flags = 0, rc = 0;
+    int64_t *active = (int64_t *)malloc(sizeof(int64_t) * (size_t)(n > 0 ? n : 1));
+    if (!active) return -1;
+    for (int32_t k = 0; k < max_steps && !finished && rc == 0; ++k) {
+        /* find_next_sync_point_and_drift (run.c:29-64) */
+        int32_t min_glob = ti_end[0];
+        for (int64_t i = 1; i < n; ++i)
+            if (min_glob > ti_end[i]) min_glob = ti_end[i];
+        if (min_glob >= pm_end) min_glob = pm_end;
+        int32_t target = min_glob;
+        if (ti_stop > ti_current && min_glob >= ti_stop) target = ti_stop;
+        double d = ti_factor(drift_tab, lb, lm, interval, ti_current, target);
+        for (int64_t i = 0; i < 3 * n; ++i) pos[i] += vel[i] * d; /* predict.c:17-21 */
+        wrap_positions(pos, n, p.L);
+        ti_current = target;
+        *time = time_begin * exp(ti_current * interval);
+        if (target != min_glob) {
+            at_stop = 1;
+            break;
+        }
+        /* set_softenings (gravlist.cpp:28-45) */
+        double a = *time;
+        double soft = soft_halo * a > soft_max_phys ? soft_max_phys / a : soft_halo;
+        orc_set_params(&p, p0->M, p0->L, p0->G, p0->asmth, p0->rcut, 2.8 * soft);
+        p.nthreads = p0->nthreads;
+        /* compute_accelerations (accel.c:8-23): PM on PM steps; the
+         * short-range force of the particles that will be kicked. */
+        if (pm_end == ti_current) rc = orc_pm_force(&p, pos, mass, n, acc_pm);
+        int64_t na = 0;
+        for (int64_t i = 0; i < n; ++i)
+            if (ti_end[i] == ti_current) active[na++] = i;
+        if (rc == 0 && na > 0) {
+            double *tmp = (double *)malloc(sizeof(double) * 3 * (size_t)na);
+            if (!tmp) { rc = -1; break; }
+            rc = orc_pp_force(&p, pos, mass, n, active, na, tmp, NULL);
+            for (int64_t k2 = 0; k2 < na; ++k2)
+                for (int dd = 0; dd < 3; ++dd) acc_pp[3 * active[k2] + dd] = tmp[3 * k2 + dd];
+            free(tmp);
+        }
+        if (rc) break;
+        /* advance_and_find_timesteps (timestep.c:11-94) */
+        double fac1 = 1 / (a * a);
+        double hubble_a = omega0 / (a * a * a) + (1 - omega0 - omega_lambda) / (a * a) + omega_lambda;
+        hubble_a = hubble * sqrt(hubble_a);
+        for (int64_t k2 = 0; k2 < na; ++k2) {
+            int64_t i = active[k2];
+            const double *g = acc_pp + 3 * i, *pm = acc_pm + 3 * i;
+            /* get_timestep (timestep.c:96-138) */
+            double ax = fac1 * g[0], ay = fac1 * g[1], az = fac1 * g[2];
+            ax += fac1 * pm[0];
+            ay += fac1 * pm[1];
+            az += fac1 * pm[2];
+            double ac = sqrt(ax * ax + ay * ay + az * az);
+            if (ac == 0) ac = 1.0e-30;
+            double dt = sqrt(2 * err_tol * a * soft / ac);
+            dt *= hubble_a;
+            if (dt >= max_dt) dt = max_dt;
+            if (dt < min_dt) dt = min_dt;
+            double st = dt / interval;
+            if (!(st > 0 && st < (double)ORC_TIMEBASE)) { flags = 1; continue; }
+            int ti_step = (int)st;
+            int ti_min = ORC_TIMEBASE;
+            while (ti_min > ti_step) ti_min >>= 1;
+            ti_step = ti_min;
+            int end = ti_end[i], beg = ti_beg[i];
+            if (ti_step > (end - beg)) {
+                if (((ORC_TIMEBASE - end) % ti_step) > 0) ti_step = end - beg;
+            }
+            if (ti_current == ORC_TIMEBASE) ti_step = 0;
+            if ((ORC_TIMEBASE - ti_current) < ti_step) ti_step = ORC_TIMEBASE - ti_current;
+            int tstart = (beg + end) / 2, tend = end + ti_step / 2;
+            double kf = ti_factor(kick_tab, lb, lm, interval, tstart, tend);
+            ti_beg[i] = end;
+            ti_end[i] = end + ti_step;
+            for (int dd = 0; dd < 3; ++dd) vel[3 * i + dd] += g[dd] * kf;
+        }
+        if (pm_end == ti_current) {
+            int ti_step = ORC_TIMEBASE;
+            while (ti_step > (max_dt / interval)) ti_step >>= 1;
+            if (ti_step > (pm_end - pm_beg)) {
+                if (((ORC_TIMEBASE - pm_end) % ti_step) > 0) ti_step = pm_end - pm_beg;
+            }
+            if (ti_current == ORC_TIMEBASE) ti_step = 0;
+            int tstart = (pm_beg + pm_end) / 2, tend = pm_end + ti_step / 2;
+            double kf = ti_factor(kick_tab, lb, lm, interval, tstart, tend);
+            pm_beg = pm_end;
+            pm_end = pm_beg + ti_step;
+            for (int64_t i = 0; i < 3 * n; ++i) vel[i] += acc_pm[i] * kf;
+        }
+        steps++;
+        if (flags) break;
+        if (!(ti_current < ORC_TIMEBASE && *time <= time_max)) finished = 1; /* run.c:24 */
+    }
+    free(active);
+    state[0] = ti_current;
+    state[1] = pm_beg;
+    state[2] = pm_end;
+    state[3] = steps;
+    status[0] = finished;
+    status[1] = at_stop;
+    status[2] = flags;
+    return rc;
+}

@@ -160,6 +160,28 @@ int orc_kdk_run(const orc_params *p, double *pos, double *vel,
                 const double *mass, int64_t n, int32_t nsteps,
                 const double *factors, double *acc_pp, double *acc_pm);
 
+/* The reference's main loop with individual power-of-two timesteps:
+ * up to max_steps passes of run()'s body (run.c:9-25): next sync point and
+ * drift (run.c:29-64, with box wrapping), set_softenings (gravlist.cpp:
+ * 28-45), PM on PM steps and the short-range force of the particles whose
+ * step ends now (accel.c:8-23), advance_and_find_timesteps with get_timestep
+ * (timestep.c:11-138).  tl = {TimeBegin, TimeMax, Omega0, OmegaLambda,
+ * Hubble, ErrTolIntAccuracy, MaxSizeTimestep, MinSizeTimestep,
+ * SofteningHalo, SofteningHaloMaxPhys}; drift_tab/kick_tab as
+ * orc_drift_tables; state = {Ti_Current, PM_Ti_begstep, PM_Ti_endstep,
+ * NumCurrentTiStep} in/out; *time = TimeBegin exp(Ti_Current interval)
+ * (double, the reference stores float).  When the next sync point reaches
+ * ti_stop > Ti_Current the particles are drifted to ti_stop and the call
+ * returns (status[1] = 1).  status = {finished, at_stop, flags}; flags = 1
+ * when a step left the integer timeline (the reference exits there).
+ * acc_pp keeps its old value for particles that were not kicked. */
+int orc_timeline_run(const orc_params *p, const double *tl,
+                     const double *drift_tab, const double *kick_tab,
+                     double *pos, double *vel, const double *mass, int64_t n,
+                     int32_t *ti_beg, int32_t *ti_end, int32_t *state,
+                     double *time, double *acc_pp, double *acc_pm,
+                     int32_t max_steps, int32_t ti_stop, int32_t *status);
+
 #ifdef __cplusplus
 }
 #endif

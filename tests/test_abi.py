@@ -67,6 +67,19 @@ def test_create_rejects_bad_parameters_without_device(bad):
     with pytest.raises(E.CucoError) as ei:
         E.Engine(max_particles=100, **kw)
     assert ei.value.status in (1, 5)
+    good = dict(pm_grid=64, box_size=1000.0, G=1.0, asmth=19.53, rcut=87.9,
+                soft_h=1.0)
+    with pytest.raises(E.CucoError) as ei:
+        E.Ensemble([good, kw], [100, 100])   # one bad box fails the ensemble
+    assert ei.value.status in (1, 5)
+
+
+@pytest.mark.parametrize("counts", [[], [100, 0], [100, -5]])
+def test_ensemble_create_rejects_bad_counts_without_device(counts):
+    good = dict(pm_grid=64, box_size=1000.0, G=1.0, asmth=19.53, rcut=87.9,
+                soft_h=1.0)
+    with pytest.raises((E.CucoError, ValueError)):
+        E.Ensemble([good] * len(counts), counts)
 
 
 def test_drift_tables_einstein_de_sitter():

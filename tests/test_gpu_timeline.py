@@ -1,0 +1,130 @@
+"""Individual power-of-two timesteps on the device (run.c / timestep.c)
+against the oracle's restatement: the integer timeline (every particle's
+Ti_begstep/Ti_endstep, Ti_Current, the PM step) must match exactly, the
+trajectory to FP64 round-off; the active-set short-range force must equal a
+full evaluation for the kicked particles; the state round-trips through a
+snapshot file."""
+import math
+
+import numpy as np
+import pytest
+
+from cuco_amd import engine as E
+from cuco_amd import ics
+
+from helpers import halo_box, make_engine, oracle_params
+
+pytestmark = pytest.mark.gpu
+
+
+def setup(orc, seed=9):
+    box = halo_box(12, 24, 40.0, n_halos=4, seed=seed)
+    pos, vel, mass = ics.generate(box)
+    tl = dict(time_begin=0.2, time_max=0.3, omega0=0.3, omega_lambda=0.7,
+              hubble=ics.HUBBLE_INTERNAL, err_tol_int_accuracy=0.025,
+              max_size_timestep=math.log(1.5) / 8, min_size_timestep=0.0,
+              softening_halo=box.softening,
+              softening_halo_max_phys=box.softening * 0.25)
+    d, k = E.init_drift_table(tl["time_begin"], tl["time_max"], tl["omega0"],
+                              tl["omega_lambda"], tl["hubble"])
+    ref = orc.Timeline(oracle_params(box), tl, d, k, pos, vel, mass)
+    eng = make_engine(box)
+    eng.upload(pos, vel, mass)
+    eng.timeline_init(**tl)
+    return box, tl, ref, eng
+
+
+def compare(box, ref, eng, st, rs):
+    for key in ("ti_current", "pm_ti_begstep", "pm_ti_endstep", "steps",
+                "finished", "at_stop"):
+        assert st[key] == rs[key], key
+    assert st["time"] == pytest.approx(rs["time"], rel=1e-15)
+    b, e = eng.timesteps()
+    assert np.array_equal(b, ref.ti_beg) and np.array_equal(e, ref.ti_end)
+    pos, vel = eng.download_particles()
+    d = np.abs(pos - ref.pos)
+    d = np.minimum(d, box.L - d)
+    assert d.max() <= 1e-9 * box.L
+    assert np.abs(vel - ref.vel).max() <= 1e-9 * np.abs(ref.vel).max()
+
+
+def test_timeline_matches_oracle(orc):
+    box, tl, ref, eng = setup(orc)
+    interval = math.log(tl["time_max"] / tl["time_begin"]) / E.TIMEBASE
+    ti_out = E.find_next_output_time(np.array([0.22, 0.25]), tl["time_begin"],
+                                     tl["time_max"], interval, 0)
+    levels = set()
+    for chunk in range(4):
+        stop = ti_out if chunk == 1 else -1
+        st = eng.timeline_run(E.FP64, max_steps=6, ti_stop=stop)
+        rs = ref.run(6, stop)
+        assert rs["flags"] == 0
+        compare(box, ref, eng, st, rs)
+        b, e = eng.timesteps()
+        levels |= set(np.unique(e - b).tolist())
+        if chunk == 1:
+            assert st["at_stop"] == 1 and st["ti_current"] == ti_out
+    assert len(levels) >= 2                   # a real hierarchy was exercised
+
+
+def test_active_set_force_equals_full_evaluation(orc):
+    for precision in (E.FP64, E.FP32):
+        box, tl, ref, eng = setup(orc, seed=10)
+        eng.timeline_run(precision, max_steps=3)
+        st = eng.timeline_run(precision, max_steps=1)
+        b, _ = eng.timesteps()
+        active = b == st["ti_current"]        # kicked in this step: begstep moved here
+        assert 0 < active.sum()
+        app = eng.accelerations(pp=True, pm=False)
+        pos, _ = eng.download_particles()
+        full = make_engine(box)
+        full.upload(pos, None, ics.generate(box)[2])
+        full.set_softening(eng_soft(tl, st["time"]))
+        full.gravity_linklist(precision)
+        fa = full.accelerations(pp=True, pm=False)
+        assert np.array_equal(app[active], fa[active]), precision
+
+
+def eng_soft(tl, a):
+    s = tl["softening_halo"]
+    if s * a > tl["softening_halo_max_phys"]:
+        s = tl["softening_halo_max_phys"] / a
+    return 2.8 * s
+
+
+def test_fp32_timeline_tracks_fp64(orc):
+    box, tl, ref, e64 = setup(orc)
+    _, _, _, e32 = setup(orc)
+    s64 = e64.timeline_run(E.FP64, max_steps=8)
+    s32 = e32.timeline_run(E.FP32, max_steps=8)
+    assert s32["steps"] == 8 and s32["ti_current"] > 0
+    p64, _ = e64.download_particles()
+    p32, _ = e32.download_particles()
+    d = np.abs(p64 - p32)
+    d = np.minimum(d, box.L - d)
+    assert np.isfinite(p32).all()
+    assert np.median(d) <= 1e-6 * box.L
+    assert s64["steps"] == 8
+
+
+def test_records_and_snapshot_round_trip(orc, tmp_path):
+    box, tl, ref, eng = setup(orc)
+    st = eng.timeline_run(E.FP64, max_steps=4)
+    P = eng.download_records()
+    b, e = eng.timesteps()
+    assert np.array_equal(P["Ti_begstep"], b) and np.array_equal(P["Ti_endstep"], e)
+    pos, vel = eng.download_particles()
+    assert np.array_equal(P["Pos"], pos.astype(np.float32))
+    assert np.array_equal(P["Vel"], vel.astype(np.float32))
+    a = st["time"]
+    h = E.Header(P.shape[0], float(P["Mass"][0]), a, 1 / a - 1, box.L, 0.3, 0.7, 0.7)
+    path = str(tmp_path / "snap_000.cuco")
+    E.write_snapshot(path, h, P)
+    h2, Q = E.read_snapshot(path)
+    assert Q.tobytes() == P.tobytes() and h2.Time == np.float32(a)
+    # A second handle restarts from the records.
+    other = make_engine(box)
+    other.upload_records(Q)
+    p2, v2 = other.download_particles()
+    assert np.array_equal(p2, np.mod(Q["Pos"].astype(np.float64), box.L))
+    assert np.array_equal(v2, Q["Vel"].astype(np.float64))
