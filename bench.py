@@ -60,16 +60,24 @@ def cpu_baseline(box, pos, mass, n_targets, seed=5006):
     oracle.pm_force(p, pos, mass)
     t_pm = time.perf_counter() - t0
     value = box.N / (t_pp + t_pm)
+    # The faithful single-threaded structure: the same PP loop on one core.
+    p1 = oracle.params(box.M, box.L, ics.G_INTERNAL, box.asmth, box.rcut,
+                       box.soft_h, nthreads=1)
+    n1 = max(1000, n_targets // 20)
+    oracle.pp_force(p1, pos, mass, targets[:n1])
+    _, t_sample1 = oracle.pp_last_timing()
     return {
         "value": value,
         "unit": "particle force evals/s",
         "cores": threads,
         "kind": "port",
+        "single_thread_pp_evals_per_s": n1 / t_sample1,
         "sample": (f"FP64 oracle (OpenMP, {threads} threads): PP for {n_targets} "
                    f"random targets of the {box.N}-particle box, cell list over "
                    f"all particles (setup {t_setup:.2f}s, sample {t_sample:.2f}s, "
                    f"extrapolated PP {t_pp:.1f}s); full PM on the {box.M}^3 "
-                   f"mesh {t_pm:.1f}s"),
+                   f"mesh {t_pm:.1f}s; one thread: PP for {n1} targets in "
+                   f"{t_sample1:.2f}s"),
     }
 
 
@@ -140,6 +148,21 @@ def ensemble_bench(E, ics, nboxes, steps):
         "batched_evals_per_s": n_total * steps / t_batched,
         "bit_identical_to_looped": identical,
     }
+
+
+def pmc_traffic(kernel: str, workload: str):
+    """HBM bytes per dispatch of `kernel` on `workload` from the committed
+    rocprofv3 PMC summary (profiles/<kernel>_<workload>_pmc.md, the line
+    `traffic_bytes: <B>`), or None when there is none."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_pmc.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
 
 
 def main() -> int:
@@ -245,7 +268,7 @@ def main() -> int:
         "peak": PEAK_FP32_TFLOPS,
         "unit": "TFLOP/s",
         "frac": achieved / PEAK_FP32_TFLOPS,
-        "traffic": None,
+        "traffic": pmc_traffic("pp32", box.name),
         "pp_ms": pp_ms,
         "pairs_in": pairs_in,
         "candidates": candidates,

@@ -18,6 +18,7 @@ CSRC = os.path.join(HERE, "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 BUILD_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libcuco_engine.so")
+DRIVER = os.path.join(HERE, "cuco_driver")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
@@ -69,6 +70,13 @@ def build(verbose: bool = False) -> str:
     if res.returncode != 0:
         raise RuntimeError(f"link failed:\n{res.stderr}")
     os.replace(tmp, LIB)
+    # The C host program in the reference's shape, linked against the library.
+    cmd = [os.environ.get("CC", "gcc"), "-O2", "-std=c99", "-Wall", "-I", INCLUDE,
+           os.path.join(HERE, "host", "driver.c"), "-o", DRIVER, "-L", HERE,
+           "-lcuco_engine", "-Wl,-rpath,$ORIGIN", "-lm"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"driver build failed:\n{res.stderr}")
     if verbose:
         print(f"built {LIB}")
     return LIB
