@@ -178,6 +178,9 @@ def main() -> int:
     ap.add_argument("--ensemble-boxes", type=int, default=64,
                     help="boxes of the batched-vs-looped ensemble run (0: skip)")
     ap.add_argument("--ensemble-steps", type=int, default=20)
+    ap.add_argument("--pm-overlap", choices=["auto", "0", "1"], default="auto",
+                    help="PM chain on a second stream beside the pair force: "
+                         "the autotuner's choice, or forced off / on")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="counter pass: one FP32 evaluation of the workload and "
                          "a 1 GiB device copy of known size, nothing else")
@@ -229,6 +232,24 @@ def main() -> int:
     tuning, tune_ms = eng.autotune()
     log(f"[bench] autotune {tuning} best pp {tune_ms:.2f} ms "
         f"({time.perf_counter() - t0:.1f}s)")
+    other_ms = None
+    if args.pm_overlap != "auto":
+        tuning["pm_overlap"] = int(args.pm_overlap)
+        eng.set_tuning(**tuning)
+    else:
+        # The other PM/PP overlap setting, timed for comparison (not the value).
+        other = dict(tuning, pm_overlap=1 - tuning["pm_overlap"])
+        eng.set_tuning(**other)
+        for _ in range(2):
+            eng.compute_accelerations(E.FP32, True)
+        eng.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(max(2, args.steps // 2)):
+            eng.compute_accelerations(E.FP32, True)
+        eng.synchronize()
+        other_ms = (time.perf_counter() - t0) / max(2, args.steps // 2) * 1e3
+        eng.set_tuning(**tuning)
+        log(f"[bench] fp32 with pm_overlap={other['pm_overlap']}: {other_ms:.2f} ms/eval")
     for _ in range(args.warmup):
         eng.compute_accelerations(E.FP32, True)
     eng.synchronize()
@@ -302,6 +323,7 @@ def main() -> int:
         "fp32_over_fp64": t64 * 1e3 / ms32,
         "error_vs_fp64": errors,
         "tuning": tuning,
+        "fp32_ms_other_overlap": other_ms,
         "stage_ms_fp32": {k: v / max(nev, 1) for k, v in st32.items()},
         "stage_ms_fp64": {k: v / max(args.fp64_steps, 1) for k, v in st64.items()},
         "roofline": roofline,

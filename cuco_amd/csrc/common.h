@@ -231,8 +231,10 @@ __device__ inline bool box_block(const BoxView &v, int &bi, uint32_t &lb)
     const uint32_t total = v.map[v.nboxes];
     const long l = xcd_block(total);
     if (l < 0) return false;
-    bi = find_box(v.map, v.nboxes, (uint32_t)l);
-    lb = (uint32_t)l - v.map[bi];
+    // Box and block are uniform over the workgroup; in scalar registers the
+    // descriptor loads that depend on them can be scalar too.
+    bi = __builtin_amdgcn_readfirstlane(find_box(v.map, v.nboxes, (uint32_t)l));
+    lb = __builtin_amdgcn_readfirstlane((uint32_t)l - v.map[bi]);
     return true;
 }
 

@@ -84,6 +84,11 @@ struct HostBox {
 struct cuco_engine {
     int device = 0;
     hipStream_t stream = nullptr;
+    // Second stream for the PM chain when it overlaps the pair force, and
+    // the fork / join events between the two.
+    hipStream_t stream2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    bool pm_overlap = false;
     bool ensemble = false;  // box sizes fixed at creation
     std::vector<HostBox> box;
     uint32_t capacity = 0;
@@ -208,18 +213,18 @@ void end_eval(cuco_engine *e)
     }
 }
 
-int stage_start(cuco_engine *e, int s)
+int stage_start(cuco_engine *e, int s, hipStream_t st = nullptr)
 {
     if (e->cur_events) {
-        CUCO_HIP(hipEventRecord(e->cur_events->start[s], e->stream));
+        CUCO_HIP(hipEventRecord(e->cur_events->start[s], st ? st : e->stream));
         e->cur_events->used[s] = true;
     }
     return CUCO_OK;
 }
 
-int stage_stop(cuco_engine *e, int s)
+int stage_stop(cuco_engine *e, int s, hipStream_t st = nullptr)
 {
-    if (e->cur_events) CUCO_HIP(hipEventRecord(e->cur_events->stop[s], e->stream));
+    if (e->cur_events) CUCO_HIP(hipEventRecord(e->cur_events->stop[s], st ? st : e->stream));
     return CUCO_OK;
 }
 
@@ -498,7 +503,7 @@ int do_pp(cuco_engine *e, int precision, unsigned long long *counter,
 }
 
 template <typename T>
-int do_pm_t(cuco_engine *e, int precision)
+int do_pm_t(cuco_engine *e, int precision, hipStream_t st)
 {
     TRY(ensure_pm<T>(e));
     const int mode = mode_of(precision);
@@ -509,35 +514,59 @@ int do_pm_t(cuco_engine *e, int precision)
     T *rho = static_cast<T *>(e->rho[k]);
     T *phi = static_cast<T *>(e->phi[k]);
     Cx<T> *c = static_cast<Cx<T> *>(e->cgrid[k]);
-    TRY(stage_start(e, CUCO_STAGE_DEPOSIT));
-    TRY(pm_deposit<T>(spos, e->skeys, e->off, e->bs, mode, rho, e->dep_scratch[k], e->stream));
-    TRY(stage_stop(e, CUCO_STAGE_DEPOSIT));
-    TRY(stage_start(e, CUCO_STAGE_FFT_FWD));
-    TRY(pm_fft_forward<T>(rho, c, e->bs, e->tb[k], e->stream));
-    TRY(stage_stop(e, CUCO_STAGE_FFT_FWD));
-    TRY(stage_start(e, CUCO_STAGE_GREEN));
-    TRY(pm_green_x<T>(c, e->bs, e->tb[k], e->stream));
-    TRY(stage_stop(e, CUCO_STAGE_GREEN));
-    TRY(stage_start(e, CUCO_STAGE_FFT_INV));
-    TRY(pm_fft_inverse<T>(c, phi, e->bs, e->tb[k], e->stream));
-    TRY(stage_stop(e, CUCO_STAGE_FFT_INV));
-    TRY(stage_start(e, CUCO_STAGE_INTERP));
-    TRY(pm_interp<T>(phi, spos, e->skeys, e->perm, e->bs, mode, acc, e->stream));
-    TRY(stage_stop(e, CUCO_STAGE_INTERP));
+    TRY(stage_start(e, CUCO_STAGE_DEPOSIT, st));
+    TRY(pm_deposit<T>(spos, e->skeys, e->off, e->bs, mode, rho, e->dep_scratch[k], st));
+    TRY(stage_stop(e, CUCO_STAGE_DEPOSIT, st));
+    TRY(stage_start(e, CUCO_STAGE_FFT_FWD, st));
+    TRY(pm_fft_forward<T>(rho, c, e->bs, e->tb[k], st));
+    TRY(stage_stop(e, CUCO_STAGE_FFT_FWD, st));
+    TRY(stage_start(e, CUCO_STAGE_GREEN, st));
+    TRY(pm_green_x<T>(c, e->bs, e->tb[k], st));
+    TRY(stage_stop(e, CUCO_STAGE_GREEN, st));
+    TRY(stage_start(e, CUCO_STAGE_FFT_INV, st));
+    TRY(pm_fft_inverse<T>(c, phi, e->bs, e->tb[k], st));
+    TRY(stage_stop(e, CUCO_STAGE_FFT_INV, st));
+    TRY(stage_start(e, CUCO_STAGE_INTERP, st));
+    TRY(pm_interp<T>(phi, spos, e->skeys, e->perm, e->bs, mode, acc, st));
+    TRY(stage_stop(e, CUCO_STAGE_INTERP, st));
     e->pm_prec = precision;
     return CUCO_OK;
 }
 
-int do_pm(cuco_engine *e, int precision)
+int do_pm(cuco_engine *e, int precision, hipStream_t st = nullptr)
 {
-    return precision == CUCO_FP64 ? do_pm_t<double>(e, precision)
-                                  : do_pm_t<float>(e, precision);
+    if (!st) st = e->stream;
+    return precision == CUCO_FP64 ? do_pm_t<double>(e, precision, st)
+                                  : do_pm_t<float>(e, precision, st);
 }
 
-int do_forces(cuco_engine *e, int precision, int do_pm_flag)
+// accel.c:14-22: PM on PM steps, then PP.  With pm_overlap the PM chain is
+// issued first on the second stream (it waits for the sort) and the pair
+// force on the main stream; the main stream then waits for the PM chain.
+// The two touch disjoint outputs and only read the sorted arrays.
+int do_forces(cuco_engine *e, int precision, int do_pm_flag,
+              const int32_t *active_end = nullptr, int32_t ti_current = 0)
 {
-    if (do_pm_flag) TRY(do_pm(e, precision));  // accel.c:14-16
-    return do_pp(e, precision, nullptr);       // accel.c:22
+    if (!do_pm_flag) return do_pp(e, precision, nullptr, active_end, ti_current);
+    if (!e->pm_overlap) {
+        TRY(do_pm(e, precision));
+        return do_pp(e, precision, nullptr, active_end, ti_current);
+    }
+    TRY(precision == CUCO_FP64 ? ensure_pm<double>(e) : ensure_pm<float>(e));
+    TRY(sort_particles(e, mode_of(precision)));
+    CUCO_HIP(hipEventRecord(e->ev_fork, e->stream));
+    CUCO_HIP(hipStreamWaitEvent(e->stream2, e->ev_fork, 0));
+    int rc = do_pm(e, precision, e->stream2);
+    if (!rc) rc = do_pp(e, precision, nullptr, active_end, ti_current);
+    // Join even after an error so a capture never ends with a dangling fork.
+    hipError_t a = hipEventRecord(e->ev_join, e->stream2);
+    hipError_t b = hipStreamWaitEvent(e->stream, e->ev_join, 0);
+    if (rc) return rc;
+    if (a != hipSuccess || b != hipSuccess) {
+        set_error("stream join failed: %s", hipGetErrorString(a != hipSuccess ? a : b));
+        return CUCO_ERR_HIP;
+    }
+    return CUCO_OK;
 }
 
 int kick_current(cuco_engine *e, double kpp, double kpm)
@@ -806,9 +835,13 @@ int create_engine(const cuco_params *params, const uint32_t *caps, int nboxes,
     ALLOC(e->tiles.scan_scratch, sizeof(uint32_t) * scan_scratch_words(e->tiles.nblocks));
 #undef ALLOC
     if (rc == CUCO_OK) {
+        // Both streams keep the default priority.
         hipError_t err = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
+        if (err == hipSuccess) err = hipStreamCreateWithFlags(&e->stream2, hipStreamNonBlocking);
+        if (err == hipSuccess) err = hipEventCreateWithFlags(&e->ev_fork, hipEventDisableTiming);
+        if (err == hipSuccess) err = hipEventCreateWithFlags(&e->ev_join, hipEventDisableTiming);
         if (err != hipSuccess) {
-            set_error("hipStreamCreate failed: %s", hipGetErrorString(err));
+            set_error("stream / event creation failed: %s", hipGetErrorString(err));
             rc = CUCO_ERR_HIP;
         }
     }
@@ -919,9 +952,13 @@ int cuco_destroy(cuco_engine *e)
         free_pm_tables(e->tb[k]);
     }
     free_ptr(e->dfactors); free_ptr(e->dstep);
+    if (e->stream2) (void)hipStreamSynchronize(e->stream2);
     free_ptr(e->ti_beg); free_ptr(e->ti_end);
     free_ptr(e->ti_scalars); free_ptr(e->kick_table);
     if (e->stream) (void)hipStreamDestroy(e->stream);
+    if (e->stream2) (void)hipStreamDestroy(e->stream2);
+    if (e->ev_fork) (void)hipEventDestroy(e->ev_fork);
+    if (e->ev_join) (void)hipEventDestroy(e->ev_join);
     delete e;
     return CUCO_OK;
 }
@@ -1321,9 +1358,7 @@ int timeline_step(cuco_engine *e, int precision, int32_t ti_stop, bool &stopped)
     // compute_accelerations (accel.c:8-23)
     const bool pm_step = s.pm_ti_endstep == s.ti_current;
     TRY(begin_eval(e));
-    int rc = CUCO_OK;
-    if (pm_step) rc = do_pm(e, precision);
-    if (!rc) rc = do_pp(e, precision, nullptr, e->ti_end, s.ti_current);
+    int rc = do_forces(e, precision, pm_step, e->ti_end, s.ti_current);
     end_eval(e);
     if (rc) return rc;
 
@@ -1719,7 +1754,8 @@ int cuco_get_tuning(cuco_engine *e, cuco_tuning *t)
     }
     t->pp_block_shift = e->tiles.block_shift;
     t->pp_workgroups_per_cu = e->pp32_wg_per_cu;
-    t->reserved0 = t->reserved1 = 0;
+    t->pm_overlap = e->pm_overlap ? 1 : 0;
+    t->reserved1 = 0;
     return CUCO_OK;
 }
 
@@ -1728,7 +1764,7 @@ int cuco_set_tuning(cuco_engine *e, const cuco_tuning *t)
     CHECK_ENGINE(e);
     if (!t || t->pp_block_shift < e->min_block_shift || t->pp_block_shift > 4 ||
         t->pp_block_shift >= e->min_bits || t->pp_workgroups_per_cu < 1 ||
-        t->pp_workgroups_per_cu > 32) {
+        t->pp_workgroups_per_cu > 32 || t->pm_overlap < 0 || t->pm_overlap > 1) {
         set_error("tuning out of range");
         return CUCO_ERR_INVALID_ARG;
     }
@@ -1741,6 +1777,10 @@ int cuco_set_tuning(cuco_engine *e, const cuco_tuning *t)
     if (t->pp_workgroups_per_cu != e->pp32_wg_per_cu) {
         e->pp32_wg_per_cu = t->pp_workgroups_per_cu;
         e->pp32_workgroups = e->ncu * e->pp32_wg_per_cu;
+        destroy_graph(e);
+    }
+    if ((t->pm_overlap != 0) != e->pm_overlap) {
+        e->pm_overlap = t->pm_overlap != 0;
         destroy_graph(e);
     }
     return CUCO_OK;
@@ -1781,6 +1821,36 @@ int cuco_autotune(cuco_engine *e, cuco_tuning *chosen, double *best_ms)
             }
             if (rc) break;
         }
+    }
+    // Whole FP32 evaluations (sort, PM, PP) with and without the PM chain
+    // on its own stream; for each overlap setting, the pair kernel is also
+    // retimed at 4 and 8 workgroups per CU (fewer resident pair workgroups
+    // leave room for the PM kernels).
+    if (!rc) {
+        float best_eval = 1e30f;
+        cuco_tuning pick = best;
+        for (int ov = 0; ov < 2 && !rc; ++ov) {
+            for (int wg : {best.pp_workgroups_per_cu, 4}) {
+                if (ov == 0 && wg != best.pp_workgroups_per_cu) continue;
+                cuco_tuning t{best.pp_block_shift, wg, ov, 0};
+                rc = cuco_set_tuning(e, &t);
+                float ms = 1e30f;
+                for (int rep = 0; rep < 3 && !rc; ++rep) {
+                    e->sorted_mode = -1;
+                    if (hipEventRecord(a, e->stream) != hipSuccess) rc = CUCO_ERR_HIP;
+                    if (!rc) rc = do_forces(e, CUCO_FP32, 1);
+                    if (!rc && (hipEventRecord(b, e->stream) != hipSuccess ||
+                                hipEventSynchronize(b) != hipSuccess)) rc = CUCO_ERR_HIP;
+                    float t_ms = 0;
+                    if (!rc && hipEventElapsedTime(&t_ms, a, b) == hipSuccess) ms = std::min(ms, t_ms);
+                }
+                if (!rc && ms < best_eval * 0.98f) {
+                    best_eval = ms;
+                    pick = t;
+                }
+            }
+        }
+        best = pick;
     }
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);

@@ -28,11 +28,22 @@ namespace cuco {
 
 namespace {
 
-constexpr int kFftThreads = 256;
+constexpr int kFftThreads = 512;
 constexpr int kFftBytes = 65536;  // LDS tile per block: 8192 FP32 / 4096 FP64 complex
 
 template <typename T>
 constexpr int fft_elems() { return kFftBytes / (int)sizeof(Cx<T>); }
+
+// Lines per block of the strided (y, x) passes: the largest power of two
+// that fits the LDS tile, so a block's element -> (line, position) split is
+// a mask and a shift.
+template <typename T>
+__host__ __device__ inline int lines_log2(int Nf)
+{
+    int lg = 0;
+    while ((2 << lg) * Nf <= fft_elems<T>()) ++lg;
+    return lg;
+}
 
 // ------------------------------------------------------------- small DFTs
 template <typename T, int R, bool INV>
@@ -101,28 +112,43 @@ __device__ inline int lidx(int l, int j, int ld)
     return ROWS ? l * ld + j : j * ld + l;
 }
 
-// One Stockham stage over all B lines of length Nf held in LDS.
+// q = n / d and r = n % d for 0 <= n < 2^20 through a float reciprocal
+// (inv = 1/d rounded): the estimate is off by at most one and corrected.
+__device__ inline void divmod(int n, int d, float inv, int &q, int &r)
+{
+    q = (int)((float)n * inv);
+    r = n - q * d;
+    if (r < 0) { --q; r += d; }
+    else if (r >= d) { ++q; r -= d; }
+}
+
+// One Stockham stage over all B lines of length Nf held in LDS.  Every
+// thread reads its butterflies, the block synchronises, and every thread
+// writes them to their output positions (kept from the read phase).
 template <typename T, int R, bool ROWS, bool INV>
-__device__ void stage(Cx<T> *lds, int Nf, int B, int ld, int Ns,
+__device__ void stage(Cx<T> *lds, int Nf, int B, int lgB, int ld, int Ns,
                       const Cx<T> *__restrict__ tw, int twstride)
 {
     constexpr int MAXB = (fft_elems<T>() / R + kFftThreads - 1) / kFftThreads;
     const int span = Nf / R;
     const int nb = span * B;
+    const float inv_span = 1.0f / (float)span, inv_ns = 1.0f / (float)Ns;
+    const int tstep = (Nf / (Ns * R)) * twstride;
+    const int ostride = ROWS ? Ns : Ns * ld;  // output element stride r -> r+1
     Cx<T> v[MAXB][R];
-    int jl[MAXB];
+    int out[MAXB];
 #pragma unroll
     for (int q = 0; q < MAXB; ++q) {
         const int b = threadIdx.x + q * kFftThreads;
         if (b < nb) {
             int l, j;
-            if (ROWS) { j = b % span; l = b / span; } else { l = b % B; j = b / B; }
-            jl[q] = j;
+            if (ROWS) divmod(b, span, inv_span, l, j);
+            else { l = b & (B - 1); j = b >> lgB; }
 #pragma unroll
             for (int r = 0; r < R; ++r) v[q][r] = lds[lidx<ROWS>(l, j + r * span, ld)];
-            const int k = j % Ns;
+            int jq, k;
+            divmod(j, Ns, inv_ns, jq, k);
             if (Ns > 1) {
-                const int tstep = (Nf / (Ns * R)) * twstride;
 #pragma unroll
                 for (int r = 1; r < R; ++r) {
                     Cx<T> w = tw[k * r * tstep];
@@ -131,6 +157,7 @@ __device__ void stage(Cx<T> *lds, int Nf, int B, int ld, int Ns,
                 }
             }
             dft<T, R, INV>(v[q]);
+            out[q] = lidx<ROWS>(l, jq * Ns * R + k, ld);
         }
     }
     __syncthreads();
@@ -138,28 +165,26 @@ __device__ void stage(Cx<T> *lds, int Nf, int B, int ld, int Ns,
     for (int q = 0; q < MAXB; ++q) {
         const int b = threadIdx.x + q * kFftThreads;
         if (b < nb) {
-            const int l = ROWS ? b / span : b % B;
-            const int j = jl[q];
-            const int k = j % Ns;
-            const int d = (j / Ns) * Ns * R + k;
 #pragma unroll
-            for (int r = 0; r < R; ++r) lds[lidx<ROWS>(l, d + r * Ns, ld)] = v[q][r];
+            for (int r = 0; r < R; ++r) lds[out[q] + r * ostride] = v[q][r];
         }
     }
     __syncthreads();
 }
 
+// lgB: log2 of B for the strided passes (B is a power of two there).
 template <typename T, bool ROWS, bool INV>
-__device__ void run_fft(Cx<T> *lds, int Nf, int B, int ld, const FftPlan &plan,
+__device__ void run_fft(Cx<T> *lds, int Nf, int B, int lgB, int ld, const FftPlan &plan,
                         const Cx<T> *__restrict__ tw, int twstride)
 {
     int Ns = 1;
-    for (int s = 0; s < plan.nstages; ++s) {
-        const int R = plan.radix[s];
-        if (R == 4) stage<T, 4, ROWS, INV>(lds, Nf, B, ld, Ns, tw, twstride);
-        else if (R == 2) stage<T, 2, ROWS, INV>(lds, Nf, B, ld, Ns, tw, twstride);
-        else if (R == 3) stage<T, 3, ROWS, INV>(lds, Nf, B, ld, Ns, tw, twstride);
-        else stage<T, 5, ROWS, INV>(lds, Nf, B, ld, Ns, tw, twstride);
+    const int ns = __builtin_amdgcn_readfirstlane(plan.nstages);
+    for (int s = 0; s < ns; ++s) {
+        const int R = __builtin_amdgcn_readfirstlane(plan.radix[s]);
+        if (R == 4) stage<T, 4, ROWS, INV>(lds, Nf, B, lgB, ld, Ns, tw, twstride);
+        else if (R == 2) stage<T, 2, ROWS, INV>(lds, Nf, B, lgB, ld, Ns, tw, twstride);
+        else if (R == 3) stage<T, 3, ROWS, INV>(lds, Nf, B, lgB, ld, Ns, tw, twstride);
+        else stage<T, 5, ROWS, INV>(lds, Nf, B, lgB, ld, Ns, tw, twstride);
         Ns *= R;
     }
 }
@@ -198,7 +223,7 @@ __global__ __launch_bounds__(kFftThreads) void k_rows_r2c(
         lds[l * Nf + j] = row < nrows ? inc[(size_t)row * Nf + j] : Cx<T>{0, 0};
     }
     __syncthreads();
-    run_fft<T, true, false>(lds, Nf, B, Nf, bx.half, tw, 2);
+    run_fft<T, true, false>(lds, Nf, B, 0, Nf, bx.half, tw, 2);
     for (int e = threadIdx.x; e < Zc * B; e += kFftThreads) {
         const int l = e / Zc, k = e % Zc;
         const int row = row0 + l;
@@ -249,7 +274,7 @@ __global__ __launch_bounds__(kFftThreads) void k_rows_c2r(
         lds[l * Nf + k] = z;
     }
     __syncthreads();
-    run_fft<T, true, true>(lds, Nf, B, Nf, bx.half, tw, 2);
+    run_fft<T, true, true>(lds, Nf, B, 0, Nf, bx.half, tw, 2);
     Cx<T> *outc = reinterpret_cast<Cx<T> *>(out);
     for (int e = threadIdx.x; e < Nf * B; e += kFftThreads) {
         const int l = e / Nf, j = e % Nf;
@@ -284,19 +309,19 @@ __global__ __launch_bounds__(kFftThreads) void k_lines(
     Cx<T> *data = data_all + bx.cgrid0;
     const Cx<T> *tw = tw_all + bx.tab0;
     Cx<T> *lds = lds_tile<T>();
-    const int B = fft_elems<T>() / Nf;
+    const int lgB = lines_log2<T>(Nf), B = 1 << lgB;
     const int line0 = (int)lb * B;
     for (int e = threadIdx.x; e < Nf * B; e += kFftThreads) {
-        const int l = e % B, j = e / B;
+        const int l = e & (B - 1), j = e >> lgB;
         const int L = line0 + l;
         Cx<T> x = {0, 0};
         if (L < nlines) x = data[line_base<XAXIS>(L, M, Zc) + j * stride];
         lds[j * B + l] = x;
     }
     __syncthreads();
-    run_fft<T, false, INV>(lds, Nf, B, B, bx.full, tw, 1);
+    run_fft<T, false, INV>(lds, Nf, B, lgB, B, bx.full, tw, 1);
     for (int e = threadIdx.x; e < Nf * B; e += kFftThreads) {
-        const int l = e % B, j = e / B;
+        const int l = e & (B - 1), j = e >> lgB;
         const int L = line0 + l;
         if (L < nlines) data[line_base<XAXIS>(L, M, Zc) + j * stride] = lds[j * B + l];
     }
@@ -321,17 +346,17 @@ __global__ __launch_bounds__(kFftThreads) void k_lines_green(
     const int Nf = M, Zc = M / 2 + 1;
     const int nlines = M * Zc;
     const size_t stride = (size_t)M * Zc;
-    const int B = fft_elems<T>() / Nf;
+    const int lgB = lines_log2<T>(Nf), B = 1 << lgB;
     const int line0 = (int)lb * B;
     for (int e = threadIdx.x; e < Nf * B; e += kFftThreads) {
-        const int l = e % B, j = e / B;
+        const int l = e & (B - 1), j = e >> lgB;
         const int L = line0 + l;
         lds[j * B + l] = L < nlines ? data[(size_t)L + j * stride] : Cx<T>{0, 0};
     }
     __syncthreads();
-    run_fft<T, false, false>(lds, Nf, B, B, bx.full, tw, 1);
+    run_fft<T, false, false>(lds, Nf, B, lgB, B, bx.full, tw, 1);
     for (int e = threadIdx.x; e < Nf * B; e += kFftThreads) {
-        const int l = e % B, j = e / B;
+        const int l = e & (B - 1), j = e >> lgB;
         const int L = line0 + l;
         if (L >= nlines) continue;
         const int y = L / Zc, z = L % Zc;
@@ -346,9 +371,9 @@ __global__ __launch_bounds__(kFftThreads) void k_lines_green(
         lds[j * B + l] = {x.x * g, x.y * g};
     }
     __syncthreads();
-    run_fft<T, false, true>(lds, Nf, B, B, bx.full, tw, 1);
+    run_fft<T, false, true>(lds, Nf, B, lgB, B, bx.full, tw, 1);
     for (int e = threadIdx.x; e < Nf * B; e += kFftThreads) {
-        const int l = e % B, j = e / B;
+        const int l = e & (B - 1), j = e >> lgB;
         const int L = line0 + l;
         if (L < nlines) data[(size_t)L + j * stride] = lds[j * B + l];
     }
@@ -716,8 +741,8 @@ uint32_t pm_blocks(int M, int map)
     case kMapGatherM: return gather_blocks(M, kKeyMorton);
     case kMapRowsD: { const uint32_t B = fft_elems<double>() / (M / 2); return ((uint32_t)M * M + B - 1) / B; }
     case kMapRowsF: { const uint32_t B = fft_elems<float>() / (M / 2); return ((uint32_t)M * M + B - 1) / B; }
-    case kMapLinesD: { const uint32_t B = fft_elems<double>() / M; return ((uint32_t)M * Zc + B - 1) / B; }
-    case kMapLinesF: { const uint32_t B = fft_elems<float>() / M; return ((uint32_t)M * Zc + B - 1) / B; }
+    case kMapLinesD: { const uint32_t B = 1u << lines_log2<double>(M); return ((uint32_t)M * Zc + B - 1) / B; }
+    case kMapLinesF: { const uint32_t B = 1u << lines_log2<float>(M); return ((uint32_t)M * Zc + B - 1) / B; }
     default: return 0;
     }
 }

@@ -167,15 +167,18 @@ typedef struct cuco_tuning {
                                      2^shift cells per dimension (2..4) */
     int32_t pp_workgroups_per_cu; /* persistent FP32 pair-kernel workgroups
                                      per compute unit (1..32) */
-    int32_t reserved0;
+    int32_t pm_overlap;           /* 1: the PM chain runs on a second stream,
+                                     concurrently with the pair force */
     int32_t reserved1;
 } cuco_tuning;
 
 int cuco_get_tuning(cuco_engine *e, cuco_tuning *t);
 int cuco_set_tuning(cuco_engine *e, const cuco_tuning *t);
 /* Times the FP32 pair force on the current particles for every candidate
- * tuning, keeps the fastest and returns it; *best_ms (optional) receives its
- * kernel time.  Leaves the accelerations of the last trial. */
+ * block size and workgroup count, then a whole FP32 evaluation with and
+ * without PM/PP overlap; keeps the fastest and returns it; *best_ms
+ * (optional) receives the best pair-kernel time.  Leaves the accelerations
+ * of the last trial. */
 int cuco_autotune(cuco_engine *e, cuco_tuning *chosen, double *best_ms);
 
 /* ---------------------------------------------------------------- */

@@ -99,3 +99,30 @@ def test_every_tuning_gives_the_same_forces():
     assert np.abs(eng.accelerations()[0] - ref).max() <= 1e-5 * scale
     with pytest.raises(E.CucoError):
         eng.set_tuning(9, 4)
+
+
+@pytest.mark.parametrize("precision", [E.FP64, E.FP32])
+def test_pm_overlap_gives_identical_forces_and_graphs(precision):
+    """The PM chain on a second stream, concurrent with the pair force:
+    same forces, and a captured KDK step (fork/join inside the graph)
+    replays to the same trajectory as eager steps without overlap."""
+    box = halo_box(seed=13)
+    pos, vel, mass = ics.generate(box)
+    lat = ics.WORKLOADS["lattice32"]
+    d, k = E.init_drift_table(lat.a_start, lat.a_end, ics.OMEGA_M, ics.OMEGA_L,
+                              ics.HUBBLE_INTERNAL)
+    f = E.kdk_factors(d, k, lat.a_start, lat.a_end, lat.a_start, lat.a_end, 10)[:3]
+    runs = []
+    for overlap in (0, 1):
+        eng = make_engine(box)
+        eng.set_tuning(3, 4, overlap)
+        eng.upload(pos, vel, mass)
+        eng.compute_accelerations(precision, True)
+        a = eng.accelerations()
+        eng.kdk_steps(f[:1], precision, use_graph=bool(overlap))
+        eng.kdk_steps(f[1:], precision, use_graph=bool(overlap))
+        runs.append((a, eng.download_particles(), eng.accelerations()))
+    (a0, s0, b0), (a1, s1, b1) = runs
+    assert np.array_equal(a0[0], a1[0]) and np.array_equal(a0[1], a1[1])
+    assert np.array_equal(s0[0], s1[0]) and np.array_equal(s0[1], s1[1])
+    assert np.array_equal(b0[0], b1[0]) and np.array_equal(b0[1], b1[1])
