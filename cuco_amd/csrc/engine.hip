@@ -124,6 +124,7 @@ struct cuco_engine {
     float *table32 = nullptr;  // 1024 entries for the FP32 kernel
     float htable[kNTab];
     TileInfo tiles;
+    PairSym psym;  // each-pair-once variant of the FP32 pair force
     int pp32_workgroups = 0;
     int pp32_wg_per_cu = 8;
     int ncu = 256;
@@ -290,7 +291,7 @@ int update_pair_constants(cuco_engine *e)
         const double edge = b.pp.asmthfac * b.pp.rcut;
         same_edge = same_edge && fabs(edge - edge0) <= 1e-9 * edge0;
         cut = cut && pp32_table_cut(b.pp, b.g[kKeyMorton], e->tiles.block_shift);
-        h.push_back(make_p32(b.pp, b.g[kKeyMorton], e->uniform_mass));
+        h.push_back(make_p32(b.pp, b.g[kKeyMorton], e->uniform_mass, b.cap));
     }
     const bool aligned = fabs(edge0 - nearbyint(edge0)) < 1e-9 * edge0;
     e->table_cut = cut && same_edge;
@@ -492,10 +493,11 @@ int do_pp(cuco_engine *e, int precision, unsigned long long *counter,
                             active_end, ti_current, e->stream));
         }
     } else {
+        e->psym.n_sorted = e->n;
         TRY(launch_pp32(e->spos32, e->skeys, e->perm, e->off, e->bs, e->p32,
                         e->table_cut, e->uniform_mass, e->table32, e->tiles,
                         e->acc_pp32, counter, e->pp32_workgroups, active_end,
-                        ti_current, e->stream));
+                        ti_current, e->psym, e->stream));
     }
     TRY(stage_stop(e, CUCO_STAGE_PP));
     e->pp_prec = precision;
@@ -941,6 +943,7 @@ int cuco_destroy(cuco_engine *e)
     free_ptr(e->off); free_ptr(e->spos64); free_ptr(e->spos32);
     free_ptr(e->table); free_ptr(e->table32); free_ptr(e->dcount);
     free_ptr(e->p32);
+    free_ptr(e->psym.iacc); free_ptr(e->psym.jacc);
     free_ptr(e->bs.dev[0]); free_ptr(e->bs.dev[1]);
     free_ptr(e->bs.maps); free_ptr(e->bs.tile_map);
     free_ptr(e->tiles.tile_start); free_ptr(e->tiles.tile_count);
@@ -1755,7 +1758,7 @@ int cuco_get_tuning(cuco_engine *e, cuco_tuning *t)
     t->pp_block_shift = e->tiles.block_shift;
     t->pp_workgroups_per_cu = e->pp32_wg_per_cu;
     t->pm_overlap = e->pm_overlap ? 1 : 0;
-    t->reserved1 = 0;
+    t->pp_symmetric = e->psym.on ? 1 : 0;
     return CUCO_OK;
 }
 
@@ -1764,7 +1767,8 @@ int cuco_set_tuning(cuco_engine *e, const cuco_tuning *t)
     CHECK_ENGINE(e);
     if (!t || t->pp_block_shift < e->min_block_shift || t->pp_block_shift > 4 ||
         t->pp_block_shift >= e->min_bits || t->pp_workgroups_per_cu < 1 ||
-        t->pp_workgroups_per_cu > 32 || t->pm_overlap < 0 || t->pm_overlap > 1) {
+        t->pp_workgroups_per_cu > 32 || t->pm_overlap < 0 || t->pm_overlap > 1 ||
+        t->pp_symmetric < 0 || t->pp_symmetric > 1) {
         set_error("tuning out of range");
         return CUCO_ERR_INVALID_ARG;
     }
@@ -1783,6 +1787,14 @@ int cuco_set_tuning(cuco_engine *e, const cuco_tuning *t)
         e->pm_overlap = t->pm_overlap != 0;
         destroy_graph(e);
     }
+    if ((t->pp_symmetric != 0) != e->psym.on) {
+        if (t->pp_symmetric && !e->psym.iacc) {
+            TRY(dev_alloc(e->psym.iacc, sizeof(float4) * e->capacity));
+            TRY(dev_alloc(e->psym.jacc, sizeof(unsigned long long) * 3 * (size_t)e->capacity));
+        }
+        e->psym.on = t->pp_symmetric != 0;
+        destroy_graph(e);
+    }
     return CUCO_OK;
 }
 
@@ -1799,17 +1811,19 @@ int cuco_autotune(cuco_engine *e, cuco_tuning *chosen, double *best_ms)
     float best_t = 1e30f;
     int rc = CUCO_OK;
     for (int shift = e->min_block_shift; shift <= std::min(4, e->min_bits - 1) && !rc; ++shift) {
-        for (int wg : {4, 8, 16}) {
-            cuco_tuning t{shift, wg, 0, 0};
+        for (int cfg = 0; cfg < 6; ++cfg) {
+            const int wg = cfg % 3 == 0 ? 4 : cfg % 3 == 1 ? 8 : 16;
+            cuco_tuning t{shift, wg, 0, cfg / 3};
             rc = cuco_set_tuning(e, &t);
             if (!rc) rc = sort_particles(e, kKeyMorton);
             float ms = 1e30f;
             for (int rep = 0; rep < 2 && !rc; ++rep) {
                 if (hipEventRecord(a, e->stream) != hipSuccess) rc = CUCO_ERR_HIP;
+                e->psym.n_sorted = e->n;
                 if (!rc) rc = launch_pp32(e->spos32, e->skeys, e->perm, e->off, e->bs, e->p32,
                                           e->table_cut, e->uniform_mass, e->table32, e->tiles,
                                           e->acc_pp32, nullptr, e->pp32_workgroups,
-                                          nullptr, 0, e->stream);
+                                          nullptr, 0, e->psym, e->stream);
                 if (!rc && (hipEventRecord(b, e->stream) != hipSuccess ||
                             hipEventSynchronize(b) != hipSuccess)) rc = CUCO_ERR_HIP;
                 float t_ms = 0;
@@ -1832,7 +1846,7 @@ int cuco_autotune(cuco_engine *e, cuco_tuning *chosen, double *best_ms)
         for (int ov = 0; ov < 2 && !rc; ++ov) {
             for (int wg : {best.pp_workgroups_per_cu, 4}) {
                 if (ov == 0 && wg != best.pp_workgroups_per_cu) continue;
-                cuco_tuning t{best.pp_block_shift, wg, ov, 0};
+                cuco_tuning t{best.pp_block_shift, wg, ov, best.pp_symmetric};
                 rc = cuco_set_tuning(e, &t);
                 float ms = 1e30f;
                 for (int rep = 0; rep < 3 && !rc; ++rep) {

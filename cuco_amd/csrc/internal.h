@@ -112,20 +112,31 @@ struct P32 {
     float out_scale;              // G * s^2
     float mass;                   // uniform particle mass of the box
     int M;
+    float jscale, jinv;           // fixed-point scale of the each-pair-once j sums
 };
-P32 make_p32(const PairParams &pp, const GridInfo &g, double uniform_mass);
+P32 make_p32(const PairParams &pp, const GridInfo &g, double uniform_mass,
+             uint32_t n_box);
 // True when the zero tail of the LDS table can make the rcut cut alone.
 bool pp32_table_cut(const PairParams &pp, const GridInfo &g, int block_shift);
 
 // Builds the tile list of every box and bs.tile_map (Morton keys).
 int launch_build_tiles(const uint32_t *off, const BoxSet &bs, uint32_t nkeys,
                        TileInfo &t, hipStream_t s);
+// Buffers of the each-pair-once variant: own sums (sorted order) and the
+// j sums in 64-bit fixed point (3 planes of n_sorted).
+struct PairSym {
+    bool on = false;
+    float4 *iacc = nullptr;
+    unsigned long long *jacc = nullptr;
+    uint32_t n_sorted = 0;
+};
 int launch_pp32(const float4 *spos, const uint32_t *skeys,
                 const uint32_t *perm, const uint32_t *off, const BoxSet &bs,
                 const P32 *p32s, bool table_cut, double uniform_mass,
                 const float *table32, const TileInfo &t, float4 *acc,
                 unsigned long long *cand_counter, int num_workgroups,
-                const int32_t *active_end, int32_t ti_current, hipStream_t s);
+                const int32_t *active_end, int32_t ti_current,
+                const PairSym &sym, hipStream_t s);
 
 // ------------------------------------------------------------ PM
 struct PmTables {

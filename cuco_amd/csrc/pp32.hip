@@ -22,6 +22,9 @@
 //    its total with Kahan compensation.
 //  * A persistent grid pulls tiles from an atomic work counter, so dense
 //    halo tiles and void tiles balance across the chip.
+#include <algorithm>
+#include <cmath>
+
 #include "internal.h"
 
 namespace cuco {
@@ -194,9 +197,88 @@ __device__ inline void kahan_add(float &sum, float &comp, float v)
     sum = t;
 }
 
+// ---------------------------------------------------------------------------
+// Each pair once (Newton's third law).  A tile takes only the j with a larger
+// sorted index than its first particle, and inside the tile only j > i; the
+// force on j is minus the pair term, summed over the wave's lanes with DPP
+// and added to j's total in 64-bit fixed point (an integer sum, so the result
+// does not depend on the order in which tiles run).
+
+// The j side rides along with j: at step s lane L evaluates the pair with
+// staged j number (L + s) mod 64 and subtracts the pair term from its
+// accumulator, which then moves one lane down (lane L takes lane L+1's), so
+// the accumulator of j is always in the lane that meets j next; after 64
+// steps lane L holds the j-side sum of staged j number L.
+__device__ inline float rotate_down(float v)
+{
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x134, 0xF, 0xF, false));
+}
+
+// Pair sums of one lane against the 64 staged j (uniform mass folded into
+// the table, cut by the table).  With MASKED a pair counts only when j's
+// sorted index exceeds i's (iid = 0xffffffff for lanes without a target;
+// unused slots carry index 0 and a position inside the tile).
+// rot / rotid hold the 64 staged j twice (slots 64..127 repeat 0..63), so
+// lane L reads slot L + s with no wrap.
+template <bool MASKED>
+__device__ inline void interact_sym(const float4 *__restrict__ rot,
+                                    const uint32_t *__restrict__ rotid,
+                                    const P32 &pp, const float *__restrict__ tab,
+                                    float xi, float yi, float zi, uint32_t iid,
+                                    float &sx, float &sy, float &sz, float &jx,
+                                    float &jy, float &jz)
+{
+    const int lane = threadIdx.x & 63;
+    const float4 *rl = rot + lane;
+    const uint32_t *il = rotid + lane;
+    for (int s = 0; s < 64; s += 4) {
+        float dx[4], dy[4], dz[4], r[4], fac[4], t[4];
+        bool keep[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const float4 p = rl[s + q];
+            if (MASKED) keep[q] = il[s + q] > iid;
+            dx[q] = p.x - xi;
+            dy[q] = p.y - yi;
+            dz[q] = p.z - zi;
+            const float r2 = fmaf(dx[q], dx[q], fmaf(dy[q], dy[q], fmaf(dz[q], dz[q], pp.eps)));
+            const float rinv = __builtin_amdgcn_rsqf(r2);
+            r[q] = r2 * rinv;
+            fac[q] = rinv * rinv * rinv;
+            t[q] = tab[(int)r[q]];
+        }
+        const bool close = (r[0] < pp.h) | (r[1] < pp.h) | (r[2] < pp.h) | (r[3] < pp.h);
+        if (close) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) fac[q] = spline_fac<true>(r[q], fac[q], 1.0f, pp);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            float f = fac[q] * t[q];
+            if (MASKED) f = keep[q] ? f : 0.0f;
+            sx = fmaf(dx[q], f, sx);
+            sy = fmaf(dy[q], f, sy);
+            sz = fmaf(dz[q], f, sz);
+            jx = rotate_down(fmaf(-dx[q], f, jx));
+            jy = rotate_down(fmaf(-dy[q], f, jy));
+            jz = rotate_down(fmaf(-dz[q], f, jz));
+        }
+    }
+}
+
+__device__ inline uint32_t wave_min_u32(uint32_t v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, o, 64));
+    return v;
+}
+
 // Tiles of all boxes form one work list; tv.map holds each box's first
 // tile, p32s[b] the scaled constants of box b.
-template <bool GENERIC, bool UNIFORM>
+// SYM: each pair once (section above); the tile's own sums go to iacc
+// (sorted order, unscaled) and the j side to jacc (three 64-bit fixed-point
+// planes of n_sorted entries); a second kernel combines them.
+template <bool GENERIC, bool UNIFORM, bool SYM>
 __global__ __launch_bounds__(kThreads) void k_pp32(
     const float4 *__restrict__ spos, const uint32_t *__restrict__ skeys,
     const uint32_t *__restrict__ perm, const uint32_t *__restrict__ off,
@@ -205,13 +287,20 @@ __global__ __launch_bounds__(kThreads) void k_pp32(
     const uint32_t *__restrict__ tstart, const uint32_t *__restrict__ tcount,
     uint32_t *__restrict__ counters, float4 *__restrict__ acc,
     unsigned long long *__restrict__ cand,
-    const int32_t *__restrict__ active_end, int32_t ti_current)
+    const int32_t *__restrict__ active_end, int32_t ti_current,
+    float4 *__restrict__ iacc, unsigned long long *__restrict__ jacc,
+    uint32_t n_sorted)
 {
+    // SYM trades half of the range list for the doubled rotation buffer.
+    constexpr int kR = SYM ? kRanges / 2 : kRanges;
     __shared__ float s_tab[kTab];
-    __shared__ uint32_t s_rs[kWaves][kRanges];
-    __shared__ uint32_t s_rp[kWaves][kRanges + 1];
-    __shared__ uint32_t s_rq[kWaves][kRanges];
+    __shared__ uint32_t s_rs[kWaves][kR];
+    __shared__ uint32_t s_rp[kWaves][kR + 1];
+    __shared__ uint32_t s_rq[kWaves][kR];
     __shared__ float4 s_j[kWaves][kJBuf];
+    __shared__ uint32_t s_jid[kWaves][SYM ? kJBuf : 1];
+    __shared__ float4 s_rot[kWaves][SYM ? 128 : 1];
+    __shared__ uint32_t s_rotid[kWaves][SYM ? 128 : 1];
 
     for (int k = threadIdx.x; k < kTab; k += kThreads)
         s_tab[k] = k < kTabPadded ? (UNIFORM ? mass * table32[k] : table32[k]) : 0.0f;
@@ -222,6 +311,9 @@ __global__ __launch_bounds__(kThreads) void k_pp32(
     uint32_t *rp = s_rp[w];
     uint32_t *rq = s_rq[w];
     float4 *sj = s_j[w];
+    uint32_t *sjid = s_jid[w];
+    float4 *rot = s_rot[w];
+    uint32_t *rotid = s_rotid[w];
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     const uint32_t ntiles = counters[0];
     const float inf = __builtin_huge_valf();
@@ -277,6 +369,38 @@ __global__ __launch_bounds__(kThreads) void k_pp32(
         const uint32_t total_q = uni(sane ? (uint32_t)nqx * nqy * nqz : 0u);
 
         float ax = 0.f, ay = 0.f, az = 0.f, kx = 0.f, ky = 0.f, kz = 0.f;
+        // SYM: the j side of one staged chunk, added to jacc (fixed point).
+        const uint32_t iid = valid ? i : 0xffffffffu;
+        auto eval_sym = [&](int nj, float &sx_, float &sy_, float &sz_) {
+            float jx = 0.f, jy = 0.f, jz = 0.f;
+            // Copy the chunk twice into the rotation buffer; unused slots of a
+            // partial chunk get index 0 (always masked) and a position inside
+            // the tile (a bounded table bin).
+            float4 mine = make_float4(bminx, bminy, bminz, 0.f);
+            uint32_t mid = 0u;
+            if (lane < nj) {
+                mine = sj[lane];
+                mid = sjid[lane];
+            }
+            rot[lane] = mine;
+            rot[lane + 64] = mine;
+            rotid[lane] = mid;
+            rotid[lane + 64] = mid;
+            wave_sync();
+            const uint32_t mn = uni(wave_min_u32((uint32_t)lane < (uint32_t)nj ? mid : 0xffffffffu));
+            // Pairs inside the tile, lanes without a target, unused slots:
+            // mask by index.
+            if (nj < 64 || cnt < 64u || mn < i0 + cnt)
+                interact_sym<true>(rot, rotid, pp, s_tab, xi, yi, zi, iid, sx_, sy_, sz_, jx, jy, jz);
+            else
+                interact_sym<false>(rot, rotid, pp, s_tab, xi, yi, zi, iid, sx_, sy_, sz_, jx, jy, jz);
+            if ((uint32_t)lane < (uint32_t)nj) {
+                const uint32_t j = mid;
+                atomicAdd(&jacc[j], (unsigned long long)__double2ll_rn((double)jx * pp.jscale));
+                atomicAdd(&jacc[(size_t)n_sorted + j], (unsigned long long)__double2ll_rn((double)jy * pp.jscale));
+                atomicAdd(&jacc[2 * (size_t)n_sorted + j], (unsigned long long)__double2ll_rn((double)jz * pp.jscale));
+            }
+        };
         uint32_t jc = 0;
         uint32_t cursor = 0;
         unsigned long long ncand = 0;
@@ -284,7 +408,7 @@ __global__ __launch_bounds__(kThreads) void k_pp32(
         while (cursor < total_q) {
             // Enumerate cells into the range list.
             uint32_t nr = 0;
-            while (cursor < total_q && nr + 64 <= (uint32_t)kRanges) {
+            while (cursor < total_q && nr + 64 <= (uint32_t)kR) {
                 const uint32_t q = cursor + lane;
                 bool ok = q < total_q;
                 uint32_t s = 0, c = 0, packed = 0;
@@ -305,6 +429,12 @@ __global__ __launch_bounds__(kThreads) void k_pp32(
                         const uint32_t key = key_base + cell_key(g, ux, uy, uz);
                         s = off[key];
                         c = off[key + 1] - s;
+                        if (SYM) {
+                            // only j after the tile's first particle
+                            const uint32_t e = s + c, lo = i0 + 1;
+                            s = max(s, lo);
+                            c = e > s ? e - s : 0u;
+                        }
                         ok = c > 0;
                         packed = pack_q(qx, qy, qz);
                     }
@@ -338,6 +468,7 @@ __global__ __launch_bounds__(kThreads) void k_pp32(
                 const uint32_t f = f0 + lane;
                 bool ok = f < total_j;
                 float4 qj = make_float4(0.f, 0.f, 0.f, 0.f);
+                uint32_t jsorted = 0;
                 if (ok) {
                     uint32_t lo = 0, hi = nr;
                     while (hi - lo > 1) {
@@ -345,6 +476,7 @@ __global__ __launch_bounds__(kThreads) void k_pp32(
                         if (rp[mid] <= f) lo = mid; else hi = mid;
                     }
                     const uint32_t j = rs[lo] + (f - rp[lo]);
+                    jsorted = j;
                     int qx, qy, qz;
                     unpack_q(rq[lo], qx, qy, qz);
                     const float4 pj = spos[j];
@@ -357,21 +489,36 @@ __global__ __launch_bounds__(kThreads) void k_pp32(
                     ok = gx * gx + gy * gy + gz * gz <= pp.filter2;
                 }
                 const uint64_t bal = __ballot(ok);
-                if (ok) sj[jc + (uint32_t)__popcll(bal & lt)] = qj;
+                if (ok) {
+                    const uint32_t slot = jc + (uint32_t)__popcll(bal & lt);
+                    sj[slot] = qj;
+                    if (SYM) sjid[slot] = jsorted;
+                }
                 jc = uni(jc + (uint32_t)__popcll(bal));
                 wave_sync();
                 if (jc >= 64) {
                     float sx = 0.f, sy = 0.f, sz = 0.f;
-                    interact<GENERIC, UNIFORM>(sj, 64, pp, s_tab, xi, yi, zi, sx, sy, sz);
+                    if (SYM) {
+                        eval_sym(64, sx, sy, sz);
+                    } else {
+                        interact<GENERIC, UNIFORM>(sj, 64, pp, s_tab, xi, yi, zi, sx, sy, sz);
+                    }
                     kahan_add(ax, kx, sx);
                     kahan_add(ay, ky, sy);
                     kahan_add(az, kz, sz);
                     if (cand) ncand += 64ull * cnt;
                     const uint32_t rem = jc - 64;
                     float4 tmp = make_float4(0.f, 0.f, 0.f, 0.f);
-                    if ((uint32_t)lane < rem) tmp = sj[64 + lane];
+                    uint32_t tid = 0;
+                    if ((uint32_t)lane < rem) {
+                        tmp = sj[64 + lane];
+                        if (SYM) tid = sjid[64 + lane];
+                    }
                     wave_sync();
-                    if ((uint32_t)lane < rem) sj[lane] = tmp;
+                    if ((uint32_t)lane < rem) {
+                        sj[lane] = tmp;
+                        if (SYM) sjid[lane] = tid;
+                    }
                     jc = rem;
                     wave_sync();
                 }
@@ -379,7 +526,11 @@ __global__ __launch_bounds__(kThreads) void k_pp32(
         }
         if (jc > 0) {
             float sx = 0.f, sy = 0.f, sz = 0.f;
-            interact<GENERIC, UNIFORM>(sj, (int)jc, pp, s_tab, xi, yi, zi, sx, sy, sz);
+            if (SYM) {
+                eval_sym((int)jc, sx, sy, sz);
+            } else {
+                interact<GENERIC, UNIFORM>(sj, (int)jc, pp, s_tab, xi, yi, zi, sx, sy, sz);
+            }
             kahan_add(ax, kx, sx);
             kahan_add(ay, ky, sy);
             kahan_add(az, kz, sz);
@@ -387,9 +538,12 @@ __global__ __launch_bounds__(kThreads) void k_pp32(
             jc = 0;
         }
         wave_sync();
-        if (act)
+        if (SYM) {
+            if (valid) iacc[i] = make_float4(ax, ay, az, 0.f);
+        } else if (act) {
             acc[perm[i]] = make_float4(ax * pp.out_scale, ay * pp.out_scale,
                                        az * pp.out_scale, 0.f);
+        }
         if (cand) atomicAdd(cand, lane == 0 ? ncand : 0ull);
     }
 }
@@ -433,6 +587,30 @@ __global__ void k_box_tiles(const uint32_t *__restrict__ tile_off,
     if (b == nboxes) tile_map[b] = counters[0];
 }
 
+// Each-pair-once result: own sums plus the fixed-point j sums, scaled to
+// accelerations and scattered to original order.
+__global__ void k_pp32_combine(const float4 *__restrict__ iacc,
+                               const unsigned long long *__restrict__ jacc,
+                               uint32_t n_sorted, const uint32_t *__restrict__ perm,
+                               BoxView v, const P32 *__restrict__ p32s,
+                               float4 *__restrict__ acc)
+{
+    int bi;
+    uint32_t lb;
+    if (!box_block(v, bi, lb)) return;
+    const BoxDev &bx = v.box[bi];
+    const uint32_t il = lb * 256 + threadIdx.x;
+    if (il >= bx.n) return;
+    const uint32_t i = bx.p0 + il;
+    const double inv = p32s[bi].jinv;
+    const float sc = p32s[bi].out_scale;
+    const float4 a = iacc[i];
+    const float x = a.x + (float)((double)(long long)jacc[i] * inv);
+    const float y = a.y + (float)((double)(long long)jacc[(size_t)n_sorted + i] * inv);
+    const float z = a.z + (float)((double)(long long)jacc[2 * (size_t)n_sorted + i] * inv);
+    acc[perm[i]] = make_float4(x * sc, y * sc, z * sc, 0.f);
+}
+
 }  // namespace
 
 int launch_build_tiles(const uint32_t *off, const BoxSet &bs, uint32_t nkeys,
@@ -463,7 +641,8 @@ int launch_build_tiles(const uint32_t *off, const BoxSet &bs, uint32_t nkeys,
     return CUCO_OK;
 }
 
-P32 make_p32(const PairParams &pp, const GridInfo &g, double uniform_mass)
+P32 make_p32(const PairParams &pp, const GridInfo &g, double uniform_mass,
+             uint32_t n_box)
 {
     const double sc = pp.asmthfac;
     P32 p;
@@ -480,6 +659,15 @@ P32 make_p32(const PairParams &pp, const GridInfo &g, double uniform_mass)
     p.out_scale = (float)(pp.G * sc * sc);
     p.mass = (float)uniform_mass;
     p.M = g.M;
+    // Fixed-point scale of the each-pair-once j sums: a pair term is at most
+    // m * 10.67 / h^2 (scaled units; the spline's peak, the table is <= 1),
+    // so n such terms stay below 2^62 / scale.
+    const double pair_max = std::max(uniform_mass, 1e-30) * 10.67 / ((double)p.h * p.h);
+    const double bound = std::max(1.0, (double)n_box) * pair_max;
+    int e2 = 0;
+    std::frexp(std::ldexp(1.0, 62) / bound, &e2);
+    p.jscale = (float)std::ldexp(1.0, std::min(e2 - 1, 126));
+    p.jinv = 1.0f / p.jscale;
     return p;
 }
 
@@ -501,7 +689,8 @@ int launch_pp32(const float4 *spos, const uint32_t *skeys,
                 const P32 *p32s, bool table_cut, double uniform_mass,
                 const float *table32, const TileInfo &t, float4 *acc,
                 unsigned long long *cand_counter, int num_workgroups,
-                const int32_t *active_end, int32_t ti_current, hipStream_t s)
+                const int32_t *active_end, int32_t ti_current,
+                const PairSym &sym, hipStream_t s)
 {
     if (bs.total[kMapPart] == 0) return CUCO_OK;
     const bool generic = !table_cut;
@@ -511,12 +700,26 @@ int launch_pp32(const float4 *spos, const uint32_t *skeys,
     int rc = launch_fill_u32(t.counters + 1, 0u, 1, s);
     if (rc) return rc;
     const int nwg = num_workgroups > 0 ? num_workgroups : 2048;
-#define CUCO_PP32(GENERIC, UNIFORM)                                               \
-    k_pp32<GENERIC, UNIFORM><<<nwg, kThreads, 0, s>>>(spos, skeys, perm, off, tv,  \
-                                                      p32s, mass, table32,         \
-                                                      t.tile_start, t.tile_count,  \
-                                                      t.counters, acc, cand_counter, \
-                                                      active_end, ti_current)
+    // Each pair once: the uniform-mass, table-cut configuration, all targets.
+    const bool symmetric = sym.on && sym.iacc && sym.jacc && !generic && uniform && !active_end;
+    if (symmetric) {
+        rc = launch_fill_u32(reinterpret_cast<uint32_t *>(sym.jacc), 0u,
+                             (size_t)6 * sym.n_sorted, s);
+        if (rc) return rc;
+        k_pp32<false, true, true><<<nwg, kThreads, 0, s>>>(
+            spos, skeys, perm, off, tv, p32s, mass, table32, t.tile_start,
+            t.tile_count, t.counters, acc, cand_counter, nullptr, 0, sym.iacc,
+            sym.jacc, sym.n_sorted);
+        CUCO_LAUNCH_CHECK();
+        k_pp32_combine<<<xcd_grid(bs.total[kMapPart]), 256, 0, s>>>(
+            sym.iacc, sym.jacc, sym.n_sorted, perm, bs.view(kKeyMorton, kMapPart), p32s, acc);
+        CUCO_LAUNCH_CHECK();
+        return CUCO_OK;
+    }
+#define CUCO_PP32(GENERIC, UNIFORM)                                                   \
+    k_pp32<GENERIC, UNIFORM, false><<<nwg, kThreads, 0, s>>>(                         \
+        spos, skeys, perm, off, tv, p32s, mass, table32, t.tile_start, t.tile_count,  \
+        t.counters, acc, cand_counter, active_end, ti_current, nullptr, nullptr, 0u)
     if (generic) {
         if (uniform) CUCO_PP32(true, true); else CUCO_PP32(true, false);
     } else {

@@ -55,7 +55,7 @@ EXPORTED = [
 
 class Tuning(C.Structure):
     _fields_ = [("pp_block_shift", C.c_int32), ("pp_workgroups_per_cu", C.c_int32),
-                ("pm_overlap", C.c_int32), ("reserved1", C.c_int32)]
+                ("pm_overlap", C.c_int32), ("pp_symmetric", C.c_int32)]
 
 
 class CucoError(RuntimeError):
@@ -564,11 +564,12 @@ class Engine:
         _check(lib().cuco_get_tuning(self.h, C.byref(t)), "cuco_get_tuning")
         return {"pp_block_shift": t.pp_block_shift,
                 "pp_workgroups_per_cu": t.pp_workgroups_per_cu,
-                "pm_overlap": t.pm_overlap}
+                "pm_overlap": t.pm_overlap, "pp_symmetric": t.pp_symmetric}
 
     def set_tuning(self, pp_block_shift: int, pp_workgroups_per_cu: int,
-                   pm_overlap: int = 0) -> None:
-        t = Tuning(pp_block_shift, pp_workgroups_per_cu, int(pm_overlap), 0)
+                   pm_overlap: int = 0, pp_symmetric: int = 0) -> None:
+        t = Tuning(pp_block_shift, pp_workgroups_per_cu, int(pm_overlap),
+                   int(pp_symmetric))
         _check(lib().cuco_set_tuning(self.h, C.byref(t)), "cuco_set_tuning")
 
     def autotune(self):
@@ -578,7 +579,8 @@ class Engine:
         _check(lib().cuco_autotune(self.h, C.byref(t), C.byref(ms)), "cuco_autotune")
         return ({"pp_block_shift": t.pp_block_shift,
                  "pp_workgroups_per_cu": t.pp_workgroups_per_cu,
-                 "pm_overlap": t.pm_overlap}, ms.value)
+                 "pm_overlap": t.pm_overlap, "pp_symmetric": t.pp_symmetric},
+                ms.value)
 
     def set_timing(self, on: bool) -> None:
         _check(lib().cuco_set_timing(self.h, int(on)), "cuco_set_timing")
