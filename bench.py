@@ -150,15 +150,15 @@ def ensemble_bench(E, ics, nboxes, steps):
     }
 
 
-def pmc_traffic(kernel: str, workload: str):
+def pmc_traffic(kernel: str, workload: str, key: str = "traffic_bytes"):
     """HBM bytes per dispatch of `kernel` on `workload` from the committed
     rocprofv3 PMC summary (profiles/<kernel>_<workload>_pmc.md, the line
-    `traffic_bytes: <B>`), or None when there is none."""
+    `<key>: <B>`), or None when there is none."""
     path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_pmc.md")
     try:
         with open(path) as f:
             for line in f:
-                if line.startswith("traffic_bytes:"):
+                if line.startswith(key + ":"):
                     return float(line.split(":", 1)[1])
     except OSError:
         pass
@@ -184,6 +184,12 @@ def main() -> int:
     ap.add_argument("--pmc-probe", action="store_true",
                     help="counter pass: one FP32 evaluation of the workload and "
                          "a 1 GiB device copy of known size, nothing else")
+    ap.add_argument("--tuning", default=None,
+                    help="skip the autotuner and use SHIFT,WG,OVERLAP,SYMMETRIC "
+                         "(e.g. 3,8,0,1): a run for a clean kernel trace")
+    ap.add_argument("--pp-symmetric", choices=["auto", "0", "1"], default="auto",
+                    help="FP32 pair force evaluating each pair once: the "
+                         "autotuner's choice, or forced off / on")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -204,6 +210,10 @@ def main() -> int:
     eng.upload(pos, vel, mass)
     log("[bench] uploaded")
     if args.pmc_probe:
+        if args.pp_symmetric != "auto":
+            t = eng.tuning()
+            t["pp_symmetric"] = int(args.pp_symmetric)
+            eng.set_tuning(**t)
         eng.compute_accelerations(E.FP32, True)
         eng.synchronize()
         nbytes = 1 << 30
@@ -229,10 +239,22 @@ def main() -> int:
 
     # FP32 chosen path: autotune (outside the timed region), warmup, steps.
     t0 = time.perf_counter()
-    tuning, tune_ms = eng.autotune()
-    log(f"[bench] autotune {tuning} best pp {tune_ms:.2f} ms "
-        f"({time.perf_counter() - t0:.1f}s)")
+    if args.tuning:
+        v = [int(x) for x in args.tuning.split(",")]
+        tuning = dict(pp_block_shift=v[0], pp_workgroups_per_cu=v[1],
+                      pm_overlap=v[2], pp_symmetric=v[3])
+        eng.set_tuning(**tuning)
+        log(f"[bench] tuning {tuning} (given)")
+    else:
+        tuning, tune_ms = eng.autotune()
+        log(f"[bench] autotune {tuning} best pp {tune_ms:.2f} ms "
+            f"({time.perf_counter() - t0:.1f}s)")
     other_ms = None
+    if args.tuning:
+        args.pm_overlap = str(tuning["pm_overlap"])
+    if args.pp_symmetric != "auto":
+        tuning["pp_symmetric"] = int(args.pp_symmetric)
+        eng.set_tuning(**tuning)
     if args.pm_overlap != "auto":
         tuning["pm_overlap"] = int(args.pm_overlap)
         eng.set_tuning(**tuning)
@@ -276,7 +298,16 @@ def main() -> int:
     log(f"[bench] errors {errors['total']}")
 
     pairs_in, candidates = eng.count_pairs()
-    pp_ms = st32["pp"] / max(nev, 1)
+    # The pair kernel alone (no PM running beside it), HIP events on the
+    # engine's stream.
+    eng.set_timing(True)
+    eng.stage_times(reset=True)
+    for _ in range(3):
+        eng.gravity_linklist(E.FP32)
+    eng.synchronize()
+    st_pp, n_pp = eng.stage_times(reset=True)
+    eng.set_timing(False)
+    pp_ms = st_pp["pp"] / max(n_pp, 1)
     achieved = FLOPS_PER_INTERACTION * pairs_in / (pp_ms * 1e-3) / 1e12
     roofline = {
         "bound": "mfma",
@@ -284,12 +315,16 @@ def main() -> int:
                  "vector ALUs (a pair force is no dense contraction, so no MFMA) "
                  "against MI355X's published FP32 peak; packed FP32 is disabled "
                  "in every kernel, which caps this kernel at half of that peak"),
-        "kernel": "k_pp32 (FP32 pair force)",
+        "kernel": ("k_pp32 (FP32 pair force, each pair once for both particles) + "
+                   "k_pp32_combine" if tuning.get("pp_symmetric")
+                   else "k_pp32 (FP32 pair force)"),
         "achieved": achieved,
         "peak": PEAK_FP32_TFLOPS,
         "unit": "TFLOP/s",
         "frac": achieved / PEAK_FP32_TFLOPS,
-        "traffic": pmc_traffic("pp32", box.name),
+        "traffic": pmc_traffic("pp32", box.name,
+                               "traffic_bytes_symmetric" if tuning.get("pp_symmetric")
+                               else "traffic_bytes"),
         "pp_ms": pp_ms,
         "pairs_in": pairs_in,
         "candidates": candidates,
