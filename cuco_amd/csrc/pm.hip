@@ -29,7 +29,7 @@ namespace cuco {
 namespace {
 
 constexpr int kFftThreads = 512;
-constexpr int kFftBytes = 65536;  // LDS tile per block: 8192 FP32 / 4096 FP64 complex
+constexpr int kFftBytes = 32768;  // LDS tile per block: 4096 FP32 / 2048 FP64 complex
 
 template <typename T>
 constexpr int fft_elems() { return kFftBytes / (int)sizeof(Cx<T>); }
@@ -583,35 +583,65 @@ __global__ __launch_bounds__(256) void k_cic_gather(
 }
 
 // ------------------------------------------------------------- interpolation
+// The 4-point stencil (pm_periodic.c:260-267) at the node s+a of a line of
+// potential values v[k] = phi(s-2+k), k = 0..5.
 template <typename T>
-__device__ inline T stencil(const T *__restrict__ phi, int M, int x, int y, int z,
-                            int dim, T fac)
+__device__ inline T stencil_line(const T *v, int a, T fac)
 {
-    int xl = x, xr = x, xll = x, xrr = x, yl = y, yr = y, yll = y, yrr = y;
-    int zl = z, zr = z, zll = z, zrr = z;
-    if (dim == 0) { xr = wrap_index(x + 1, M); xrr = wrap_index(x + 2, M); xl = wrap_index(x - 1, M); xll = wrap_index(x - 2, M); }
-    if (dim == 1) { yr = wrap_index(y + 1, M); yrr = wrap_index(y + 2, M); yl = wrap_index(y - 1, M); yll = wrap_index(y - 2, M); }
-    if (dim == 2) { zr = wrap_index(z + 1, M); zrr = wrap_index(z + 2, M); zl = wrap_index(z - 1, M); zll = wrap_index(z - 2, M); }
-    const size_t MM = (size_t)M;
-    return fac * (((T)4.0 / (T)3) * (phi[((size_t)xl * MM + yl) * MM + zl] - phi[((size_t)xr * MM + yr) * MM + zr]) -
-                  ((T)1.0 / (T)6) * (phi[((size_t)xll * MM + yll) * MM + zll] - phi[((size_t)xrr * MM + yrr) * MM + zrr]));
+    return fac * (((T)4.0 / (T)3) * (v[a + 1] - v[a + 3]) - ((T)1.0 / (T)6) * (v[a] - v[a + 4]));
 }
 
+// The 72 potential values the 8 CIC nodes' stencils read, each once: along
+// x for the 4 (y, z) node pairs, along y for the 4 (x, z), along z for the
+// 4 (x, y).
 template <typename T>
-__device__ inline T interp_dim(const T *__restrict__ phi, int M, int sx, int sxx,
-                               int sy, int syy, int sz, int szz, T dx, T dy,
-                               T dz, int dim, T fac)
+struct Lines {
+    T x[2][2][6], y[2][2][6], z[2][2][6];
+};
+
+template <typename T>
+__device__ inline void load_lines(const T *__restrict__ phi, int M, int sx, int sy,
+                                  int sz, Lines<T> &L)
 {
-    // pm_periodic.c:297-306, same term order.
-    T a = stencil(phi, M, sx, sy, sz, dim, fac) * ((T)1.0 - dx) * ((T)1.0 - dy) * ((T)1.0 - dz);
-    a += stencil(phi, M, sx, syy, sz, dim, fac) * ((T)1.0 - dx) * dy * ((T)1.0 - dz);
-    a += stencil(phi, M, sx, sy, szz, dim, fac) * ((T)1.0 - dx) * ((T)1.0 - dy) * dz;
-    a += stencil(phi, M, sx, syy, szz, dim, fac) * ((T)1.0 - dx) * dy * dz;
-    a += stencil(phi, M, sxx, sy, sz, dim, fac) * (dx) * ((T)1.0 - dy) * ((T)1.0 - dz);
-    a += stencil(phi, M, sxx, syy, sz, dim, fac) * (dx) * dy * ((T)1.0 - dz);
-    a += stencil(phi, M, sxx, sy, szz, dim, fac) * (dx) * ((T)1.0 - dy) * dz;
-    a += stencil(phi, M, sxx, syy, szz, dim, fac) * (dx) * dy * dz;
-    return a;
+    const size_t MM = (size_t)M;
+    int xs[6], ys[6], zs[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        xs[k] = wrap_index(sx - 2 + k, M);
+        ys[k] = wrap_index(sy - 2 + k, M);
+        zs[k] = wrap_index(sz - 2 + k, M);
+    }
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int k = 0; k < 6; ++k) {
+                L.x[a][b][k] = phi[((size_t)xs[k] * MM + ys[2 + a]) * MM + zs[2 + b]];
+                L.y[a][b][k] = phi[((size_t)xs[2 + a] * MM + ys[k]) * MM + zs[2 + b]];
+                L.z[a][b][k] = phi[((size_t)xs[2 + a] * MM + ys[2 + b]) * MM + zs[k]];
+            }
+}
+
+// pm_periodic.c:297-306: the stencil at the 8 nodes with the CIC weights,
+// same term order.  Node (a, b, c) = (sx+a, sy+b, sz+c).
+template <typename T, int DIM>
+__device__ inline T interp_dim(const Lines<T> &L, T dx, T dy, T dz, T fac)
+{
+    auto st = [&](int a, int b, int c) -> T {
+        if (DIM == 0) return stencil_line(L.x[b][c], a, fac);
+        if (DIM == 1) return stencil_line(L.y[a][c], b, fac);
+        return stencil_line(L.z[a][b], c, fac);
+    };
+    T acc = st(0, 0, 0) * ((T)1.0 - dx) * ((T)1.0 - dy) * ((T)1.0 - dz);
+    acc += st(0, 1, 0) * ((T)1.0 - dx) * dy * ((T)1.0 - dz);
+    acc += st(0, 0, 1) * ((T)1.0 - dx) * ((T)1.0 - dy) * dz;
+    acc += st(0, 1, 1) * ((T)1.0 - dx) * dy * dz;
+    acc += st(1, 0, 0) * (dx) * ((T)1.0 - dy) * ((T)1.0 - dz);
+    acc += st(1, 1, 0) * (dx) * dy * ((T)1.0 - dz);
+    acc += st(1, 0, 1) * (dx) * ((T)1.0 - dy) * dz;
+    acc += st(1, 1, 1) * (dx) * dy * dz;
+    return acc;
 }
 
 __global__ __launch_bounds__(256) void k_interp64(const double *__restrict__ phi_all,
@@ -635,12 +665,12 @@ __global__ __launch_bounds__(256) void k_interp64(const double *__restrict__ phi
     const int sx = cell_coord(g.fac, q.x, M), sy = cell_coord(g.fac, q.y, M),
               sz = cell_coord(g.fac, q.z, M);
     const double dx = g.fac * q.x - sx, dy = g.fac * q.y - sy, dz = g.fac * q.z - sz;
-    const int sxx = wrap_index(sx + 1, M), syy = wrap_index(sy + 1, M),
-              szz = wrap_index(sz + 1, M);
+    Lines<double> L;
+    load_lines(phi, M, sx, sy, sz, L);
     double4 a;
-    a.x = interp_dim(phi, M, sx, sxx, sy, syy, sz, szz, dx, dy, dz, 0, fac);
-    a.y = interp_dim(phi, M, sx, sxx, sy, syy, sz, szz, dx, dy, dz, 1, fac);
-    a.z = interp_dim(phi, M, sx, sxx, sy, syy, sz, szz, dx, dy, dz, 2, fac);
+    a.x = interp_dim<double, 0>(L, dx, dy, dz, fac);
+    a.y = interp_dim<double, 1>(L, dx, dy, dz, fac);
+    a.z = interp_dim<double, 2>(L, dx, dy, dz, fac);
     a.w = 0.0;
     acc[perm[i]] = a;
 }
@@ -667,12 +697,12 @@ __global__ __launch_bounds__(256) void k_interp32(const float *__restrict__ phi_
     int sx, sy, sz;
     key_cell(g, skeys[i] - bx.key_base, sx, sy, sz);
     const float dx = q.x * facf, dy = q.y * facf, dz = q.z * facf;
-    const int sxx = wrap_index(sx + 1, M), syy = wrap_index(sy + 1, M),
-              szz = wrap_index(sz + 1, M);
+    Lines<float> L;
+    load_lines(phi, M, sx, sy, sz, L);
     float4 a;
-    a.x = interp_dim(phi, M, sx, sxx, sy, syy, sz, szz, dx, dy, dz, 0, fac);
-    a.y = interp_dim(phi, M, sx, sxx, sy, syy, sz, szz, dx, dy, dz, 1, fac);
-    a.z = interp_dim(phi, M, sx, sxx, sy, syy, sz, szz, dx, dy, dz, 2, fac);
+    a.x = interp_dim<float, 0>(L, dx, dy, dz, fac);
+    a.y = interp_dim<float, 1>(L, dx, dy, dz, fac);
+    a.z = interp_dim<float, 2>(L, dx, dy, dz, fac);
     a.w = 0.f;
     acc[perm[i]] = a;
 }

@@ -36,6 +36,7 @@ constexpr int kThreads = 64 * kWaves;
 constexpr int kRanges = 128;   // cell ranges per wave before the j stream
 constexpr int kJBuf = 128;     // staged j particles per wave
 constexpr int kTab = 4096;     // LDS erfc table; zero from the cut bin on
+constexpr int kTabSym = 1024;  // its copy in the each-pair-once kernel (index clamped)
 
 __device__ inline void wave_sync()
 {
@@ -245,7 +246,9 @@ __device__ inline void interact_sym(const float4 *__restrict__ rot,
             const float rinv = __builtin_amdgcn_rsqf(r2);
             r[q] = r2 * rinv;
             fac[q] = rinv * rinv * rinv;
-            t[q] = tab[(int)r[q]];
+            // The table's zero tail from the cut bin on: bins past the
+            // 1024-entry copy read its last (zero) entry.
+            t[q] = tab[min((int)r[q], kTabSym - 1)];
         }
         const bool close = (r[0] < pp.h) | (r[1] < pp.h) | (r[2] < pp.h) | (r[3] < pp.h);
         if (close) {
@@ -279,7 +282,7 @@ __device__ inline uint32_t wave_min_u32(uint32_t v)
 // (sorted order, unscaled) and the j side to jacc (three 64-bit fixed-point
 // planes of n_sorted entries); a second kernel combines them.
 template <bool GENERIC, bool UNIFORM, bool SYM>
-__global__ __launch_bounds__(kThreads) void k_pp32(
+__global__ __launch_bounds__(kThreads, SYM ? 5 : 1) void k_pp32(
     const float4 *__restrict__ spos, const uint32_t *__restrict__ skeys,
     const uint32_t *__restrict__ perm, const uint32_t *__restrict__ off,
     BoxView tv, const P32 *__restrict__ p32s, float mass,
@@ -293,7 +296,8 @@ __global__ __launch_bounds__(kThreads) void k_pp32(
 {
     // SYM trades half of the range list for the doubled rotation buffer.
     constexpr int kR = SYM ? kRanges / 2 : kRanges;
-    __shared__ float s_tab[kTab];
+    constexpr int kT = SYM ? kTabSym : kTab;
+    __shared__ float s_tab[kT];
     __shared__ uint32_t s_rs[kWaves][kR];
     __shared__ uint32_t s_rp[kWaves][kR + 1];
     __shared__ uint32_t s_rq[kWaves][kR];
@@ -302,7 +306,7 @@ __global__ __launch_bounds__(kThreads) void k_pp32(
     __shared__ float4 s_rot[kWaves][SYM ? 128 : 1];
     __shared__ uint32_t s_rotid[kWaves][SYM ? 128 : 1];
 
-    for (int k = threadIdx.x; k < kTab; k += kThreads)
+    for (int k = threadIdx.x; k < kT; k += kThreads)
         s_tab[k] = k < kTabPadded ? (UNIFORM ? mass * table32[k] : table32[k]) : 0.0f;
     __syncthreads();
 
