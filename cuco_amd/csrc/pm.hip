@@ -29,10 +29,14 @@ namespace cuco {
 namespace {
 
 constexpr int kFftThreads = 512;
-constexpr int kFftBytes = 32768;  // LDS tile per block: 4096 FP32 / 2048 FP64 complex
+// LDS tile per block: 16 KB (2048 complex) in FP32, 32 KB (2048 complex) in
+// FP64.  Small tiles keep the per-thread butterfly state small, so three
+// 512-thread blocks (24 waves) fit on a CU.
+template <typename T>
+constexpr int fft_bytes() { return sizeof(T) == 4 ? 16384 : 32768; }
 
 template <typename T>
-constexpr int fft_elems() { return kFftBytes / (int)sizeof(Cx<T>); }
+constexpr int fft_elems() { return fft_bytes<T>() / (int)sizeof(Cx<T>); }
 
 // Lines per block of the strided (y, x) passes: the largest power of two
 // that fits the LDS tile, so a block's element -> (line, position) split is
@@ -708,7 +712,7 @@ __global__ __launch_bounds__(256) void k_interp32(const float *__restrict__ phi_
 }
 
 template <typename T>
-size_t lds_bytes() { return (size_t)kFftBytes; }
+size_t lds_bytes() { return (size_t)fft_bytes<T>(); }
 
 template <typename T, typename K, typename... Args>
 int launch_fft_kernel(K kernel, unsigned grid, hipStream_t s, Args... args)

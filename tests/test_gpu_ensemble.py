@@ -68,6 +68,21 @@ def test_forces_match_single_boxes(precision, specs, jitter):
         assert np.array_equal(apm[sl], sm), (b, "pm")
 
 
+def test_each_pair_once_matches_single_boxes():
+    """The each-pair-once pair kernel (fixed-point j sums, one per-box scale)
+    gives every box of an ensemble exactly what a single-box handle gives."""
+    boxes, data = make_boxes(UNIFORM, False)
+    ens = make_ensemble(boxes, data)
+    ens.set_tuning(3, 8, 0, 1)
+    ens.compute_accelerations(E.FP32, True)
+    app = ens.accelerations(pp=True, pm=False)
+    for b, eng in enumerate(singles(boxes, data)):
+        eng.set_tuning(3, 8, 0, 1)
+        eng.compute_accelerations(E.FP32, True)
+        sl = ens.box_slice(b)
+        assert np.array_equal(app[sl], eng.accelerations(pp=True, pm=False)), b
+
+
 def test_sorted_order_and_meshes_match_single_boxes():
     boxes, data = make_boxes(MIXED, True)
     ens = make_ensemble(boxes, data)
