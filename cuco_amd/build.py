@@ -37,9 +37,10 @@ COMMON = [
 ]
 
 # FP64 parity needs every multiply and add rounded on its own, exactly as the
-# C reference writes them; only the FP32 pair kernel may fuse.
-CONTRACT = {"pp32.hip": "-ffp-contract=fast"}
-SOURCES = ["engine.hip", "sort.hip", "pp64.hip", "pp32.hip", "pm.hip",
+# C reference writes them; only the FP32 pair kernel and the FP32 register
+# FFTs may fuse.
+CONTRACT = {"pp32.hip": "-ffp-contract=fast", "fft32.hip": "-ffp-contract=fast"}
+SOURCES = ["engine.hip", "sort.hip", "pp64.hip", "pp32.hip", "pm.hip", "fft32.hip",
            "integrate.hip", "driftfac.cpp", "snapshot.cpp"]
 CXX = os.environ.get("CXX", "g++")
 

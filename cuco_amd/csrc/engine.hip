@@ -347,6 +347,8 @@ int update_boxes(cuco_engine *e)
         for (int b = 0; b < nb; ++b) mp[b + 1] += mp[b];
         e->bs.total[map] = mp[nb];
     }
+    e->bs.fft32_mask = 0;
+    for (const HostBox &h : e->box) e->bs.fft32_mask |= fft32_mask_bit(h.prm.pm_grid);
     for (int m = 0; m < 2; ++m)
         CUCO_HIP(hipMemcpy(e->bs.dev[m], e->hdev[m].data(), sizeof(BoxDev) * nb,
                            hipMemcpyHostToDevice));
@@ -358,7 +360,7 @@ int update_boxes(cuco_engine *e)
 // Twiddles and Green's function factors of every box, concatenated in box
 // order (box b at BoxDev::tab0).
 template <typename T>
-void fill_pm_tables(int M, double asmth, double L, Cx<T> *tw, T *ge, T *gs)
+void fill_pm_tables(int M, double asmth, double L, Cx<T> *tw, T *ge, T *gs, T *gp)
 {
     const double asmth2 = std::pow(2 * M_PI * asmth / L, 2);
     for (int k = 0; k < M; ++k) {
@@ -373,6 +375,7 @@ void fill_pm_tables(int M, double asmth, double L, Cx<T> *tw, T *ge, T *gs)
         }
         const double ff = 1.0 / f;
         gs[k] = (T)(ff * ff * ff * ff);
+        gp[k] = (T)(exp(-kk * kk * asmth2) * (ff * ff * ff * ff));
     }
 }
 
@@ -383,23 +386,27 @@ int upload_pm_tables(const std::vector<int> &Ms, const std::vector<double> &asmt
     size_t total = 0;
     for (int M : Ms) total += M;
     std::vector<Cx<T>> tw(total);
-    std::vector<T> ge(total), gs(total);
+    std::vector<T> ge(total), gs(total), gp(total);
     size_t o = 0;
     for (size_t b = 0; b < Ms.size(); ++b) {
-        fill_pm_tables<T>(Ms[b], asmth[b], L[b], tw.data() + o, ge.data() + o, gs.data() + o);
+        fill_pm_tables<T>(Ms[b], asmth[b], L[b], tw.data() + o, ge.data() + o, gs.data() + o,
+                          gp.data() + o);
         o += Ms[b];
     }
     Cx<T> *dtw = nullptr;
-    T *dge = nullptr, *dgs = nullptr;
+    T *dge = nullptr, *dgs = nullptr, *dgp = nullptr;
     TRY(dev_alloc(dtw, sizeof(Cx<T>) * total));
     tb.tw = dtw;
     TRY(dev_alloc(dge, sizeof(T) * total));
     tb.gexp = dge;
     TRY(dev_alloc(dgs, sizeof(T) * total));
     tb.gsinc = dgs;
+    TRY(dev_alloc(dgp, sizeof(T) * total));
+    tb.gprod = dgp;
     CUCO_HIP(hipMemcpy(dtw, tw.data(), sizeof(Cx<T>) * total, hipMemcpyHostToDevice));
     CUCO_HIP(hipMemcpy(dge, ge.data(), sizeof(T) * total, hipMemcpyHostToDevice));
     CUCO_HIP(hipMemcpy(dgs, gs.data(), sizeof(T) * total, hipMemcpyHostToDevice));
+    CUCO_HIP(hipMemcpy(dgp, gp.data(), sizeof(T) * total, hipMemcpyHostToDevice));
     return CUCO_OK;
 }
 
@@ -408,6 +415,7 @@ void free_pm_tables(PmTables &tb)
     free_ptr(tb.tw);
     free_ptr(tb.gexp);
     free_ptr(tb.gsinc);
+    free_ptr(tb.gprod);
 }
 
 template <typename T>
@@ -644,6 +652,7 @@ int make_fft_box(int M, FftBox &f)
     std::vector<uint32_t> maps(kMapCount * 2, 0u);
     for (int map = kMapGatherX; map < kMapCount; ++map) maps[map * 2 + 1] = pm_blocks(M, map);
     for (int map = 0; map < kMapCount; ++map) f.bs.total[map] = maps[map * 2 + 1];
+    f.bs.fft32_mask = fft32_mask_bit(M);
     TRY(dev_alloc(f.bs.dev[0], sizeof(BoxDev)));
     f.bs.dev[1] = f.bs.dev[0];
     TRY(dev_alloc(f.bs.maps, sizeof(uint32_t) * maps.size()));

@@ -16,6 +16,8 @@ enum BoxMap : int {
     kMapRowsF,      // FFT z-row blocks, FP32
     kMapLinesD,     // FFT y/x-line blocks, FP64
     kMapLinesF,     // FFT y/x-line blocks, FP32
+    kMapRowsR,      // register-FFT z-row blocks (FP32, power-of-two meshes)
+    kMapLinesR,     // register-FFT y/x-line blocks
     kMapCount
 };
 
@@ -29,12 +31,13 @@ struct BoxSet {
     uint32_t *maps = nullptr;        // kMapCount x (nboxes+1)
     uint32_t total[kMapCount] = {};  // blocks per launch
     uint32_t *tile_map = nullptr;    // nboxes+1: first pair-kernel tile per box
+    uint32_t fft32_mask = 0;         // bit i: a box has mesh 32 << i (register FFTs)
     BoxView view(int mode, int map) const
     {
         return BoxView{dev[mode], maps + (size_t)map * (nboxes + 1), nboxes};
     }
 };
-// Blocks of one box of mesh size M for a PM map (kMapGatherX..kMapLinesF).
+// Blocks of one box of mesh size M for a PM map (kMapGatherX..kMapLinesR).
 uint32_t pm_blocks(int M, int map);
 
 // ------------------------------------------------------------ sort
@@ -143,6 +146,7 @@ struct PmTables {
     void *tw = nullptr;      // M complex twiddles exp(-2 pi i k/M), precision T
     void *gexp = nullptr;    // M: exp(-k^2 asmth2) with signed k
     void *gsinc = nullptr;   // M: 1/sinc(pi k/M)^4
+    void *gprod = nullptr;   // M: gexp * gsinc (one load per element, FP32 kernels)
 };
 // scratch: pm_deposit_scratch_bytes(capacity, wave_slots, fp64) bytes
 // (per-cell corner sums and per-wave partials).
@@ -166,6 +170,19 @@ template <typename T>
 int pm_fft_x(Cx<T> *c, const BoxSet &bs, bool inverse, const PmTables &tb,
              hipStream_t s);
 bool fft_size_supported(int M);
+// FP32 register-resident FFTs of the boxes whose mesh is a power of two in
+// 32..1024 (fft32_supported); the generic FP32 maps count no blocks for
+// those boxes, so every box takes the same kernels whatever set it is in.
+// One launch per mesh size present (bs.fft32_mask).
+bool fft32_supported(int M);
+uint32_t fft32_mask_bit(int M);
+uint32_t fft32_blocks(int M, bool rows);
+int fft32_forward(const float *rho, Cx<float> *c, const BoxSet &bs, const PmTables &tb,
+                  hipStream_t s);
+int fft32_green_x(Cx<float> *c, const BoxSet &bs, const PmTables &tb, hipStream_t s);
+int fft32_inverse(Cx<float> *c, float *phi, const BoxSet &bs, const PmTables &tb,
+                  hipStream_t s);
+int fft32_x(Cx<float> *c, const BoxSet &bs, bool inverse, const PmTables &tb, hipStream_t s);
 
 // ------------------------------------------------------------ integrator
 int launch_drift(double4 *pos, const double4 *vel, const BoxSet &bs,

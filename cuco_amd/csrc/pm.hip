@@ -717,6 +717,7 @@ size_t lds_bytes() { return (size_t)fft_bytes<T>(); }
 template <typename T, typename K, typename... Args>
 int launch_fft_kernel(K kernel, unsigned grid, hipStream_t s, Args... args)
 {
+    if (grid == 0) return CUCO_OK;
     kernel<<<grid, kFftThreads, lds_bytes<T>(), s>>>(args...);
     CUCO_LAUNCH_CHECK();
     return CUCO_OK;
@@ -774,9 +775,19 @@ uint32_t pm_blocks(int M, int map)
     case kMapGatherX: return gather_blocks(M, kKeyXMajor);
     case kMapGatherM: return gather_blocks(M, kKeyMorton);
     case kMapRowsD: { const uint32_t B = fft_elems<double>() / (M / 2); return ((uint32_t)M * M + B - 1) / B; }
-    case kMapRowsF: { const uint32_t B = fft_elems<float>() / (M / 2); return ((uint32_t)M * M + B - 1) / B; }
+    case kMapRowsF: {
+        if (fft32_supported(M)) return 0;
+        const uint32_t B = fft_elems<float>() / (M / 2);
+        return ((uint32_t)M * M + B - 1) / B;
+    }
     case kMapLinesD: { const uint32_t B = 1u << lines_log2<double>(M); return ((uint32_t)M * Zc + B - 1) / B; }
-    case kMapLinesF: { const uint32_t B = 1u << lines_log2<float>(M); return ((uint32_t)M * Zc + B - 1) / B; }
+    case kMapLinesF: {
+        if (fft32_supported(M)) return 0;
+        const uint32_t B = 1u << lines_log2<float>(M);
+        return ((uint32_t)M * Zc + B - 1) / B;
+    }
+    case kMapRowsR: return fft32_blocks(M, true);
+    case kMapLinesR: return fft32_blocks(M, false);
     default: return 0;
     }
 }
@@ -791,18 +802,24 @@ int pm_fft_forward(const T *rho, Cx<T> *c, const BoxSet &bs, const PmTables &tb,
     int rc = launch_fft_kernel<T>(k_rows_r2c<T>, xcd_grid(bs.total[mr]), s, rho, c,
                                   bs.view(0, mr), tw);
     if (rc) return rc;
-    return launch_fft_kernel<T>(k_lines<T, false, false>, xcd_grid(bs.total[ml]), s, c,
-                                bs.view(0, ml), tw);
+    rc = launch_fft_kernel<T>(k_lines<T, false, false>, xcd_grid(bs.total[ml]), s, c,
+                              bs.view(0, ml), tw);
+    if (rc) return rc;
+    if constexpr (sizeof(T) == 4) return fft32_forward(rho, c, bs, tb, s);
+    return CUCO_OK;
 }
 
 template <typename T>
 int pm_green_x(Cx<T> *c, const BoxSet &bs, const PmTables &tb, hipStream_t s)
 {
     const int ml = sizeof(T) == 8 ? kMapLinesD : kMapLinesF;
-    return launch_fft_kernel<T>(k_lines_green<T>, xcd_grid(bs.total[ml]), s, c,
-                                bs.view(0, ml), static_cast<const Cx<T> *>(tb.tw),
-                                static_cast<const T *>(tb.gexp),
-                                static_cast<const T *>(tb.gsinc));
+    const int rc = launch_fft_kernel<T>(k_lines_green<T>, xcd_grid(bs.total[ml]), s, c,
+                                        bs.view(0, ml), static_cast<const Cx<T> *>(tb.tw),
+                                        static_cast<const T *>(tb.gexp),
+                                        static_cast<const T *>(tb.gsinc));
+    if (rc) return rc;
+    if constexpr (sizeof(T) == 4) return fft32_green_x(c, bs, tb, s);
+    return CUCO_OK;
 }
 
 template <typename T>
@@ -815,8 +832,11 @@ int pm_fft_inverse(Cx<T> *c, T *phi, const BoxSet &bs, const PmTables &tb,
     int rc = launch_fft_kernel<T>(k_lines<T, true, false>, xcd_grid(bs.total[ml]), s, c,
                                   bs.view(0, ml), tw);
     if (rc) return rc;
-    return launch_fft_kernel<T>(k_rows_c2r<T>, xcd_grid(bs.total[mr]), s,
-                                (const Cx<T> *)c, phi, bs.view(0, mr), tw);
+    rc = launch_fft_kernel<T>(k_rows_c2r<T>, xcd_grid(bs.total[mr]), s,
+                              (const Cx<T> *)c, phi, bs.view(0, mr), tw);
+    if (rc) return rc;
+    if constexpr (sizeof(T) == 4) return fft32_inverse(c, phi, bs, tb, s);
+    return CUCO_OK;
 }
 
 template <typename T>
@@ -845,11 +865,14 @@ int pm_fft_x(Cx<T> *c, const BoxSet &bs, bool inverse, const PmTables &tb,
 {
     const int ml = sizeof(T) == 8 ? kMapLinesD : kMapLinesF;
     const Cx<T> *tw = static_cast<const Cx<T> *>(tb.tw);
-    if (inverse)
-        return launch_fft_kernel<T>(k_lines<T, true, true>, xcd_grid(bs.total[ml]), s, c,
-                                    bs.view(0, ml), tw);
-    return launch_fft_kernel<T>(k_lines<T, false, true>, xcd_grid(bs.total[ml]), s, c,
-                                bs.view(0, ml), tw);
+    const int rc = inverse
+                       ? launch_fft_kernel<T>(k_lines<T, true, true>, xcd_grid(bs.total[ml]), s,
+                                              c, bs.view(0, ml), tw)
+                       : launch_fft_kernel<T>(k_lines<T, false, true>, xcd_grid(bs.total[ml]), s,
+                                              c, bs.view(0, ml), tw);
+    if (rc) return rc;
+    if constexpr (sizeof(T) == 4) return fft32_x(c, bs, inverse, tb, s);
+    return CUCO_OK;
 }
 
 #define CUCO_PM_INST(T)                                                              \
