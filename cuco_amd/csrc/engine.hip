@@ -537,7 +537,7 @@ int do_pm_t(cuco_engine *e, int precision, hipStream_t st)
     TRY(pm_fft_inverse<T>(c, phi, e->bs, e->tb[k], st));
     TRY(stage_stop(e, CUCO_STAGE_FFT_INV, st));
     TRY(stage_start(e, CUCO_STAGE_INTERP, st));
-    TRY(pm_interp<T>(phi, spos, e->skeys, e->perm, e->bs, mode, acc, st));
+    TRY(pm_interp<T>(phi, spos, e->skeys, e->perm, e->off, e->bs, mode, acc, st));
     TRY(stage_stop(e, CUCO_STAGE_INTERP, st));
     e->pm_prec = precision;
     return CUCO_OK;

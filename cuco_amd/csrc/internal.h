@@ -18,6 +18,7 @@ enum BoxMap : int {
     kMapLinesF,     // FFT y/x-line blocks, FP32
     kMapRowsR,      // register-FFT z-row blocks (FP32, power-of-two meshes)
     kMapLinesR,     // register-FFT y/x-line blocks
+    kMapBrick,      // 8^3-cell Morton bricks (FP32 interpolation)
     kMapCount
 };
 
@@ -162,10 +163,11 @@ int pm_green_x(Cx<T> *c, const BoxSet &bs, const PmTables &tb, hipStream_t s);
 template <typename T>
 int pm_fft_inverse(Cx<T> *c, T *phi, const BoxSet &bs, const PmTables &tb,
                    hipStream_t s);
+// off: the cell table of the sort (FP32: Morton keys, read per brick).
 template <typename T>
 int pm_interp(const T *phi, const void *spos, const uint32_t *skeys,
-              const uint32_t *perm, const BoxSet &bs, int mode, void *acc,
-              hipStream_t s);
+              const uint32_t *perm, const uint32_t *off, const BoxSet &bs, int mode,
+              void *acc, hipStream_t s);
 template <typename T>
 int pm_fft_x(Cx<T> *c, const BoxSet &bs, bool inverse, const PmTables &tb,
              hipStream_t s);

@@ -679,36 +679,68 @@ __global__ __launch_bounds__(256) void k_interp64(const double *__restrict__ phi
     acc[perm[i]] = a;
 }
 
+// FP32: one block per Morton brick of 8^3 cells (512 consecutive keys, so
+// the brick's particles are one sorted range).  The block stages the
+// potential at the 13^3 nodes its particles' stencils can reach (2 below,
+// 3 above the brick, periodic) in LDS once; every particle then reads its
+// 72 values from there.  Same arithmetic as the FP64 kernel, in FP32.
+constexpr int kBrick = 8;
+constexpr int kBrickNodes = kBrick + 5;
+
 __global__ __launch_bounds__(256) void k_interp32(const float *__restrict__ phi_all,
                                                   const float4 *__restrict__ spos,
                                                   const uint32_t *__restrict__ skeys,
                                                   const uint32_t *__restrict__ perm,
-                                                  BoxView v,
+                                                  const uint32_t *__restrict__ off, BoxView v,
                                                   float4 *__restrict__ acc)
 {
+    constexpr int E = kBrickNodes;
+    __shared__ float s_phi[E * E * E];
     int bi;
     uint32_t lb;
     if (!box_block(v, bi, lb)) return;
     const BoxDev &bx = v.box[bi];
-    const uint32_t il = lb * 256 + threadIdx.x;
-    if (il >= bx.n) return;
-    const uint32_t i = bx.p0 + il;
     const GridInfo &g = bx.g;
+    const uint32_t k0 = lb * (kBrick * kBrick * kBrick);
+    const uint32_t p_begin = off[bx.key_base + k0];
+    const uint32_t p_end = off[bx.key_base + k0 + kBrick * kBrick * kBrick];
+    if (p_begin == p_end) return;  // empty brick (or one outside the mesh)
     const int M = g.M;
-    const float facf = (float)g.fac, fac = (float)bx.pm_fac;
+    int x0, y0, z0;
+    key_cell(g, k0, x0, y0, z0);
     const float *phi = phi_all + bx.grid0;
-    const float4 q = spos[i];
-    int sx, sy, sz;
-    key_cell(g, skeys[i] - bx.key_base, sx, sy, sz);
-    const float dx = q.x * facf, dy = q.y * facf, dz = q.z * facf;
-    Lines<float> L;
-    load_lines(phi, M, sx, sy, sz, L);
-    float4 a;
-    a.x = interp_dim<float, 0>(L, dx, dy, dz, fac);
-    a.y = interp_dim<float, 1>(L, dx, dy, dz, fac);
-    a.z = interp_dim<float, 2>(L, dx, dy, dz, fac);
-    a.w = 0.f;
-    acc[perm[i]] = a;
+    for (int e = threadIdx.x; e < E * E * E; e += 256) {
+        const int ix = e / (E * E), r = e - ix * (E * E), iy = r / E, iz = r - iy * E;
+        const int gx = wrap_index(x0 - 2 + ix, M), gy = wrap_index(y0 - 2 + iy, M),
+                  gz = wrap_index(z0 - 2 + iz, M);
+        s_phi[e] = phi[((size_t)gx * M + gy) * M + gz];
+    }
+    __syncthreads();
+    const float facf = (float)g.fac, fac = (float)bx.pm_fac;
+    for (uint32_t i = p_begin + threadIdx.x; i < p_end; i += 256) {
+        const float4 q = spos[i];
+        int sx, sy, sz;
+        key_cell(g, skeys[i] - bx.key_base, sx, sy, sz);
+        const int lx = sx - x0, ly = sy - y0, lz = sz - z0;  // 0..7
+        Lines<float> L;
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int k = 0; k < 6; ++k) {
+                    L.x[a][b][k] = s_phi[((lx + k) * E + ly + 2 + a) * E + lz + 2 + b];
+                    L.y[a][b][k] = s_phi[((lx + 2 + a) * E + ly + k) * E + lz + 2 + b];
+                    L.z[a][b][k] = s_phi[((lx + 2 + a) * E + ly + 2 + b) * E + lz + k];
+                }
+        const float dx = q.x * facf, dy = q.y * facf, dz = q.z * facf;
+        float4 a;
+        a.x = interp_dim<float, 0>(L, dx, dy, dz, fac);
+        a.y = interp_dim<float, 1>(L, dx, dy, dz, fac);
+        a.z = interp_dim<float, 2>(L, dx, dy, dz, fac);
+        a.w = 0.f;
+        acc[perm[i]] = a;
+    }
 }
 
 template <typename T>
@@ -786,6 +818,11 @@ uint32_t pm_blocks(int M, int map)
         const uint32_t B = 1u << lines_log2<float>(M);
         return ((uint32_t)M * Zc + B - 1) / B;
     }
+    case kMapBrick: {  // 8^3-cell Morton bricks of the padded key cube
+        int bits = 0;
+        while ((1 << bits) < M) ++bits;
+        return bits >= 3 ? 1u << (3 * (bits - 3)) : 1u;
+    }
     case kMapRowsR: return fft32_blocks(M, true);
     case kMapLinesR: return fft32_blocks(M, false);
     default: return 0;
@@ -841,19 +878,18 @@ int pm_fft_inverse(Cx<T> *c, T *phi, const BoxSet &bs, const PmTables &tb,
 
 template <typename T>
 int pm_interp(const T *phi, const void *spos, const uint32_t *skeys,
-              const uint32_t *perm, const BoxSet &bs, int mode, void *acc,
-              hipStream_t s)
+              const uint32_t *perm, const uint32_t *off, const BoxSet &bs, int mode,
+              void *acc, hipStream_t s)
 {
     if (bs.total[kMapPart] == 0) return CUCO_OK;
-    const BoxView vp = bs.view(mode, kMapPart);
     if (sizeof(T) == 8) {
         k_interp64<<<xcd_grid(bs.total[kMapPart]), 256, 0, s>>>(
             reinterpret_cast<const double *>(phi), static_cast<const double4 *>(spos),
-            perm, vp, static_cast<double4 *>(acc));
+            perm, bs.view(mode, kMapPart), static_cast<double4 *>(acc));
     } else {
-        k_interp32<<<xcd_grid(bs.total[kMapPart]), 256, 0, s>>>(
+        k_interp32<<<xcd_grid(bs.total[kMapBrick]), 256, 0, s>>>(
             reinterpret_cast<const float *>(phi), static_cast<const float4 *>(spos),
-            skeys, perm, vp, static_cast<float4 *>(acc));
+            skeys, perm, off, bs.view(mode, kMapBrick), static_cast<float4 *>(acc));
     }
     CUCO_LAUNCH_CHECK();
     return CUCO_OK;
@@ -887,8 +923,8 @@ int pm_fft_x(Cx<T> *c, const BoxSet &bs, bool inverse, const PmTables &tb,
     template int pm_fft_inverse<T>(Cx<T> *, T *, const BoxSet &, const PmTables &,   \
                                    hipStream_t);                                     \
     template int pm_interp<T>(const T *, const void *, const uint32_t *,             \
-                              const uint32_t *, const BoxSet &, int, void *,         \
-                              hipStream_t);
+                              const uint32_t *, const uint32_t *, const BoxSet &,    \
+                              int, void *, hipStream_t);
 CUCO_PM_INST(double)
 CUCO_PM_INST(float)
 #undef CUCO_PM_INST

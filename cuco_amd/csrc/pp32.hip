@@ -127,8 +127,9 @@ __device__ inline float spline_fac(float r, float fac, float mj, const P32 &pp)
 // Candidates go four at a time: the four chains (LDS reads, rsq, table
 // lookup) are branch-free and interleave; the softened law is applied in
 // one wave-level branch per group, taken only when some lane of the wave
-// has a pair inside h.
-template <bool GENERIC, bool UNIFORM>
+// has a pair inside h.  NEAR=false: no staged j is within h of the tile's
+// bounding box (chunk_near), so no pair can be and the test is left out.
+template <bool GENERIC, bool UNIFORM, bool NEAR>
 __device__ inline void interact(const float4 *__restrict__ sj, int cnt,
                                 const P32 &pp, const float *__restrict__ tab,
                                 float xi, float yi, float zi, float &sx,
@@ -155,10 +156,12 @@ __device__ inline void interact(const float4 *__restrict__ sj, int cnt,
             if (GENERIC) idx = min(idx, kTab - 1);
             t[q] = tab[idx];
         }
-        const bool close = (r[0] < pp.h) | (r[1] < pp.h) | (r[2] < pp.h) | (r[3] < pp.h);
-        if (close) {
+        if (NEAR) {
+            const bool close = (r[0] < pp.h) | (r[1] < pp.h) | (r[2] < pp.h) | (r[3] < pp.h);
+            if (close) {
 #pragma unroll
-            for (int q = 0; q < 4; ++q) fac[q] = spline_fac<UNIFORM>(r[q], fac[q], mj[q], pp);
+                for (int q = 0; q < 4; ++q) fac[q] = spline_fac<UNIFORM>(r[q], fac[q], mj[q], pp);
+            }
         }
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
@@ -178,7 +181,7 @@ __device__ inline void interact(const float4 *__restrict__ sj, int cnt,
         const float r = r2 * rinv;
         const float rinv3 = rinv * rinv * rinv;
         float fac = UNIFORM ? rinv3 : p.w * rinv3;
-        fac = spline_fac<UNIFORM>(r, fac, p.w, pp);
+        if (NEAR) fac = spline_fac<UNIFORM>(r, fac, p.w, pp);
         int idx = (int)r;
         if (GENERIC) idx = min(idx, kTab - 1);
         float tv = tab[idx];
@@ -221,7 +224,7 @@ __device__ inline float rotate_down(float v)
 // unused slots carry index 0 and a position inside the tile).
 // rot / rotid hold the 64 staged j twice (slots 64..127 repeat 0..63), so
 // lane L reads slot L + s with no wrap.
-template <bool MASKED>
+template <bool MASKED, bool NEAR>
 __device__ inline void interact_sym(const float4 *__restrict__ rot,
                                     const uint32_t *__restrict__ rotid,
                                     const P32 &pp, const float *__restrict__ tab,
@@ -250,10 +253,12 @@ __device__ inline void interact_sym(const float4 *__restrict__ rot,
             // 1024-entry copy read its last (zero) entry.
             t[q] = tab[min((int)r[q], kTabSym - 1)];
         }
-        const bool close = (r[0] < pp.h) | (r[1] < pp.h) | (r[2] < pp.h) | (r[3] < pp.h);
-        if (close) {
+        if (NEAR) {
+            const bool close = (r[0] < pp.h) | (r[1] < pp.h) | (r[2] < pp.h) | (r[3] < pp.h);
+            if (close) {
 #pragma unroll
-            for (int q = 0; q < 4; ++q) fac[q] = spline_fac<true>(r[q], fac[q], 1.0f, pp);
+                for (int q = 0; q < 4; ++q) fac[q] = spline_fac<true>(r[q], fac[q], 1.0f, pp);
+            }
         }
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
@@ -267,6 +272,19 @@ __device__ inline void interact_sym(const float4 *__restrict__ rot,
             jz = rotate_down(fmaf(-dz[q], f, jz));
         }
     }
+}
+
+// True when some staged j (lanes with in set) lies within h of the tile's
+// bounding box; otherwise every pair of the chunk has r >= h and takes the
+// plain law.  Wave-uniform.
+__device__ inline bool chunk_near(float4 q, bool in, float bminx, float bminy, float bminz,
+                                  float bmaxx, float bmaxy, float bmaxz, float h)
+{
+    const float gx = fmaxf(0.f, fmaxf(bminx - q.x, q.x - bmaxx));
+    const float gy = fmaxf(0.f, fmaxf(bminy - q.y, q.y - bmaxy));
+    const float gz = fmaxf(0.f, fmaxf(bminz - q.z, q.z - bmaxz));
+    const bool near = in && gx * gx + gy * gy + gz * gz < h * h;
+    return __builtin_amdgcn_readfirstlane((uint32_t)(__ballot(near) != 0ull)) != 0u;
 }
 
 __device__ inline uint32_t wave_min_u32(uint32_t v)
@@ -394,16 +412,31 @@ __global__ __launch_bounds__(kThreads, SYM ? 5 : 1) void k_pp32(
             const uint32_t mn = uni(wave_min_u32((uint32_t)lane < (uint32_t)nj ? mid : 0xffffffffu));
             // Pairs inside the tile, lanes without a target, unused slots:
             // mask by index.
-            if (nj < 64 || cnt < 64u || mn < i0 + cnt)
-                interact_sym<true>(rot, rotid, pp, s_tab, xi, yi, zi, iid, sx_, sy_, sz_, jx, jy, jz);
-            else
-                interact_sym<false>(rot, rotid, pp, s_tab, xi, yi, zi, iid, sx_, sy_, sz_, jx, jy, jz);
+            const bool near = chunk_near(mine, lane < nj, bminx, bminy, bminz, bmaxx, bmaxy,
+                                         bmaxz, pp.h);
+            if (nj < 64 || cnt < 64u || mn < i0 + cnt) {
+                if (near)
+                    interact_sym<true, true>(rot, rotid, pp, s_tab, xi, yi, zi, iid, sx_, sy_, sz_, jx, jy, jz);
+                else
+                    interact_sym<true, false>(rot, rotid, pp, s_tab, xi, yi, zi, iid, sx_, sy_, sz_, jx, jy, jz);
+            } else if (near) {
+                interact_sym<false, true>(rot, rotid, pp, s_tab, xi, yi, zi, iid, sx_, sy_, sz_, jx, jy, jz);
+            } else {
+                interact_sym<false, false>(rot, rotid, pp, s_tab, xi, yi, zi, iid, sx_, sy_, sz_, jx, jy, jz);
+            }
             if ((uint32_t)lane < (uint32_t)nj) {
                 const uint32_t j = mid;
                 atomicAdd(&jacc[j], (unsigned long long)__double2ll_rn((double)jx * pp.jscale));
                 atomicAdd(&jacc[(size_t)n_sorted + j], (unsigned long long)__double2ll_rn((double)jy * pp.jscale));
                 atomicAdd(&jacc[2 * (size_t)n_sorted + j], (unsigned long long)__double2ll_rn((double)jz * pp.jscale));
             }
+        };
+        auto eval_default = [&](int nj, float &sx_, float &sy_, float &sz_) {
+            const float4 mine = lane < nj ? sj[lane] : make_float4(0.f, 0.f, 0.f, 0.f);
+            if (chunk_near(mine, lane < nj, bminx, bminy, bminz, bmaxx, bmaxy, bmaxz, pp.h))
+                interact<GENERIC, UNIFORM, true>(sj, nj, pp, s_tab, xi, yi, zi, sx_, sy_, sz_);
+            else
+                interact<GENERIC, UNIFORM, false>(sj, nj, pp, s_tab, xi, yi, zi, sx_, sy_, sz_);
         };
         uint32_t jc = 0;
         uint32_t cursor = 0;
@@ -505,7 +538,7 @@ __global__ __launch_bounds__(kThreads, SYM ? 5 : 1) void k_pp32(
                     if (SYM) {
                         eval_sym(64, sx, sy, sz);
                     } else {
-                        interact<GENERIC, UNIFORM>(sj, 64, pp, s_tab, xi, yi, zi, sx, sy, sz);
+                        eval_default(64, sx, sy, sz);
                     }
                     kahan_add(ax, kx, sx);
                     kahan_add(ay, ky, sy);
@@ -533,7 +566,7 @@ __global__ __launch_bounds__(kThreads, SYM ? 5 : 1) void k_pp32(
             if (SYM) {
                 eval_sym((int)jc, sx, sy, sz);
             } else {
-                interact<GENERIC, UNIFORM>(sj, (int)jc, pp, s_tab, xi, yi, zi, sx, sy, sz);
+                eval_default((int)jc, sx, sy, sz);
             }
             kahan_add(ax, kx, sx);
             kahan_add(ay, ky, sy);
