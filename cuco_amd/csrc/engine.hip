@@ -290,7 +290,7 @@ int update_pair_constants(cuco_engine *e)
     for (const HostBox &b : e->box) {
         const double edge = b.pp.asmthfac * b.pp.rcut;
         same_edge = same_edge && fabs(edge - edge0) <= 1e-9 * edge0;
-        cut = cut && pp32_table_cut(b.pp, b.g[kKeyMorton], e->tiles.block_shift);
+        cut = cut && pp32_table_cut(b.pp);
         h.push_back(make_p32(b.pp, b.g[kKeyMorton], e->uniform_mass, b.cap));
     }
     const bool aligned = fabs(edge0 - nearbyint(edge0)) < 1e-9 * edge0;
@@ -1785,7 +1785,6 @@ int cuco_set_tuning(cuco_engine *e, const cuco_tuning *t)
         e->tiles.block_shift = t->pp_block_shift;
         if (e->sorted_mode == kKeyMorton) e->sorted_mode = -1;  // rebuild tiles
         destroy_graph(e);
-        TRY(update_pair_constants(e));  // the table cut depends on the block size
     }
     if (t->pp_workgroups_per_cu != e->pp32_wg_per_cu) {
         e->pp32_wg_per_cu = t->pp_workgroups_per_cu;

@@ -95,7 +95,7 @@ int launch_pp64(const double4 *spos, const uint32_t *perm, const uint32_t *off,
 struct TileInfo {
     uint32_t *tile_start = nullptr;   // per tile: first sorted index
     uint32_t *tile_count = nullptr;   // per tile: particles (<= 64)
-    uint32_t *block_tiles = nullptr;  // per Morton block: tiles, then offsets
+    uint32_t *block_tiles = nullptr;  // per Morton super-block: tiles, then offsets
     uint32_t *counters = nullptr;     // [0] total tiles, [1] work counter
     uint32_t *scan_scratch = nullptr;
     uint32_t max_tiles = 0;
@@ -121,7 +121,7 @@ struct P32 {
 P32 make_p32(const PairParams &pp, const GridInfo &g, double uniform_mass,
              uint32_t n_box);
 // True when the zero tail of the LDS table can make the rcut cut alone.
-bool pp32_table_cut(const PairParams &pp, const GridInfo &g, int block_shift);
+bool pp32_table_cut(const PairParams &pp);
 
 // Builds the tile list of every box and bs.tile_map (Morton keys).
 int launch_build_tiles(const uint32_t *off, const BoxSet &bs, uint32_t nkeys,

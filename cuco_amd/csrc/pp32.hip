@@ -6,8 +6,10 @@
 //
 //  * Particles are sorted by a Morton cell key, so consecutive particles are
 //    spatially compact at every density.  A tile is <= 64 consecutive
-//    particles inside one Morton block of 2^s cells per dimension; one
-//    wavefront owns one tile, one lane one target particle.
+//    particles: a run of 64 inside one Morton block of 2^s cells per
+//    dimension where blocks are dense, several neighbouring sparse blocks of
+//    one 2^(s+1) super-block where they are not; one wavefront owns one
+//    tile, one lane one target particle.
 //  * Positions are FP32 relative to their cell corner (the subtraction is
 //    done in FP64 by the reorder kernel).  A tile works in a frame anchored
 //    at the corner of its first particle's cell: every coordinate is the
@@ -36,7 +38,7 @@ constexpr int kThreads = 64 * kWaves;
 constexpr int kRanges = 128;   // cell ranges per wave before the j stream
 constexpr int kJBuf = 128;     // staged j particles per wave
 constexpr int kTab = 4096;     // LDS erfc table; zero from the cut bin on
-constexpr int kTabSym = 1024;  // its copy in the each-pair-once kernel (index clamped)
+constexpr int kTabSym = 2048;  // its copy in the each-pair-once kernel
 
 __device__ inline void wave_sync()
 {
@@ -119,17 +121,18 @@ __device__ inline float spline_fac(float r, float fac, float mj, const P32 &pp)
 // Pair sum of one lane against cnt staged j particles.
 //
 // GENERIC=false is the reference configuration: rcut sits on a table bin
-// edge (RCUT/ASMTH = 4.5 puts it on bin 750) so the table's zeros do the cut,
-// and the host has checked that no candidate can reach bin kTab.  GENERIC
-// keeps the explicit r^2 <= rcut^2 mask and clamps the bin.  UNIFORM folds
-// the common particle mass into the LDS table.
+// edge (RCUT/ASMTH = 4.5 puts it on bin 750) so the table's zeros do the cut.
+// GENERIC keeps the explicit r^2 <= rcut^2 mask and clamps the bin.  CLAMP:
+// the tile is wide enough for a candidate to reach past the table (the
+// caller's per-tile test), so the bin is clamped.  UNIFORM folds the common
+// particle mass into the LDS table.
 //
 // Candidates go four at a time: the four chains (LDS reads, rsq, table
 // lookup) are branch-free and interleave; the softened law is applied in
 // one wave-level branch per group, taken only when some lane of the wave
 // has a pair inside h.  NEAR=false: no staged j is within h of the tile's
 // bounding box (chunk_near), so no pair can be and the test is left out.
-template <bool GENERIC, bool UNIFORM, bool NEAR>
+template <bool GENERIC, bool UNIFORM, bool NEAR, bool CLAMP>
 __device__ inline void interact(const float4 *__restrict__ sj, int cnt,
                                 const P32 &pp, const float *__restrict__ tab,
                                 float xi, float yi, float zi, float &sx,
@@ -153,7 +156,7 @@ __device__ inline void interact(const float4 *__restrict__ sj, int cnt,
             const float rinv3 = rinv * rinv * rinv;
             fac[q] = UNIFORM ? rinv3 : mj[q] * rinv3;
             int idx = (int)r[q];
-            if (GENERIC) idx = min(idx, kTab - 1);
+            if (GENERIC || CLAMP) idx = min(idx, kTab - 1);
             t[q] = tab[idx];
         }
         if (NEAR) {
@@ -183,7 +186,7 @@ __device__ inline void interact(const float4 *__restrict__ sj, int cnt,
         float fac = UNIFORM ? rinv3 : p.w * rinv3;
         if (NEAR) fac = spline_fac<UNIFORM>(r, fac, p.w, pp);
         int idx = (int)r;
-        if (GENERIC) idx = min(idx, kTab - 1);
+        if (GENERIC || CLAMP) idx = min(idx, kTab - 1);
         float tv = tab[idx];
         if (GENERIC) tv = r2 <= pp.rcut2 ? tv : 0.0f;
         const float f = fac * tv;
@@ -224,7 +227,7 @@ __device__ inline float rotate_down(float v)
 // unused slots carry index 0 and a position inside the tile).
 // rot / rotid hold the 64 staged j twice (slots 64..127 repeat 0..63), so
 // lane L reads slot L + s with no wrap.
-template <bool MASKED, bool NEAR>
+template <bool MASKED, bool NEAR, bool CLAMP>
 __device__ inline void interact_sym(const float4 *__restrict__ rot,
                                     const uint32_t *__restrict__ rotid,
                                     const P32 &pp, const float *__restrict__ tab,
@@ -249,9 +252,10 @@ __device__ inline void interact_sym(const float4 *__restrict__ rot,
             const float rinv = __builtin_amdgcn_rsqf(r2);
             r[q] = r2 * rinv;
             fac[q] = rinv * rinv * rinv;
-            // The table's zero tail from the cut bin on: bins past the
-            // 1024-entry copy read its last (zero) entry.
-            t[q] = tab[min((int)r[q], kTabSym - 1)];
+            // The table's zero tail from the cut bin on.  CLAMP: bins past
+            // the 1024-entry copy read its last (zero) entry; without it the
+            // tile's extent keeps every bin inside (tile_clamp).
+            t[q] = tab[CLAMP ? min((int)r[q], kTabSym - 1) : (int)r[q]];
         }
         if (NEAR) {
             const bool close = (r[0] < pp.h) | (r[1] < pp.h) | (r[2] < pp.h) | (r[3] < pp.h);
@@ -384,13 +388,21 @@ __global__ __launch_bounds__(kThreads, SYM ? 5 : 1) void k_pp32(
         const int nqx = (int)floorf((bmaxx + pp.rcut) * pp.inv_cellf + 0.01f) - lqx + 1;
         const int nqy = (int)floorf((bmaxy + pp.rcut) * pp.inv_cellf + 0.01f) - lqy + 1;
         const int nqz = (int)floorf((bmaxz + pp.rcut) * pp.inv_cellf + 0.01f) - lqz + 1;
-        // A tile spans at most one Morton block plus the cut on each side;
-        // anything larger means non-finite input and the tile is skipped.
+        // A tile spans at most one Morton super-block plus the cut on each
+        // side; anything larger means non-finite input and the tile is
+        // skipped.
         const bool sane = nqx > 0 && nqy > 0 && nqz > 0 && nqx <= 2 * M &&
                           nqy <= 2 * M && nqz <= 2 * M;
         const uint32_t total_q = uni(sane ? (uint32_t)nqx * nqy * nqz : 0u);
 
         float ax = 0.f, ay = 0.f, az = 0.f, kx = 0.f, ky = 0.f, kz = 0.f;
+        // Every candidate lies within sqrt(filter2) of the tile's box, so a
+        // pair's distance is at most that plus the box diagonal; below the
+        // last bin of the LDS table (with margin for rounding) the bin needs
+        // no clamp.
+        const float ex = bmaxx - bminx, ey = bmaxy - bminy, ez = bmaxz - bminz;
+        const bool clamp =
+            !(sqrtf(ex * ex + ey * ey + ez * ez) + sqrtf(pp.filter2) < (float)(kT - 2));
         // SYM: the j side of one staged chunk, added to jacc (fixed point).
         const uint32_t iid = valid ? i : 0xffffffffu;
         auto eval_sym = [&](int nj, float &sx_, float &sy_, float &sz_) {
@@ -414,16 +426,29 @@ __global__ __launch_bounds__(kThreads, SYM ? 5 : 1) void k_pp32(
             // mask by index.
             const bool near = chunk_near(mine, lane < nj, bminx, bminy, bminz, bmaxx, bmaxy,
                                          bmaxz, pp.h);
-            if (nj < 64 || cnt < 64u || mn < i0 + cnt) {
-                if (near)
-                    interact_sym<true, true>(rot, rotid, pp, s_tab, xi, yi, zi, iid, sx_, sy_, sz_, jx, jy, jz);
-                else
-                    interact_sym<true, false>(rot, rotid, pp, s_tab, xi, yi, zi, iid, sx_, sy_, sz_, jx, jy, jz);
-            } else if (near) {
-                interact_sym<false, true>(rot, rotid, pp, s_tab, xi, yi, zi, iid, sx_, sy_, sz_, jx, jy, jz);
+            const bool masked = nj < 64 || cnt < 64u || mn < i0 + cnt;
+#define CUCO_SYM(MK, NR, CL) \
+    interact_sym<MK, NR, CL>(rot, rotid, pp, s_tab, xi, yi, zi, iid, sx_, sy_, sz_, jx, jy, jz)
+            if (clamp) {
+                if (masked) {
+                    if (near) CUCO_SYM(true, true, true);
+                    else CUCO_SYM(true, false, true);
+                } else if (near) {
+                    CUCO_SYM(false, true, true);
+                } else {
+                    CUCO_SYM(false, false, true);
+                }
             } else {
-                interact_sym<false, false>(rot, rotid, pp, s_tab, xi, yi, zi, iid, sx_, sy_, sz_, jx, jy, jz);
+                if (masked) {
+                    if (near) CUCO_SYM(true, true, false);
+                    else CUCO_SYM(true, false, false);
+                } else if (near) {
+                    CUCO_SYM(false, true, false);
+                } else {
+                    CUCO_SYM(false, false, false);
+                }
             }
+#undef CUCO_SYM
             if ((uint32_t)lane < (uint32_t)nj) {
                 const uint32_t j = mid;
                 atomicAdd(&jacc[j], (unsigned long long)__double2ll_rn((double)jx * pp.jscale));
@@ -433,10 +458,19 @@ __global__ __launch_bounds__(kThreads, SYM ? 5 : 1) void k_pp32(
         };
         auto eval_default = [&](int nj, float &sx_, float &sy_, float &sz_) {
             const float4 mine = lane < nj ? sj[lane] : make_float4(0.f, 0.f, 0.f, 0.f);
-            if (chunk_near(mine, lane < nj, bminx, bminy, bminz, bmaxx, bmaxy, bmaxz, pp.h))
-                interact<GENERIC, UNIFORM, true>(sj, nj, pp, s_tab, xi, yi, zi, sx_, sy_, sz_);
-            else
-                interact<GENERIC, UNIFORM, false>(sj, nj, pp, s_tab, xi, yi, zi, sx_, sy_, sz_);
+            const bool near =
+                chunk_near(mine, lane < nj, bminx, bminy, bminz, bmaxx, bmaxy, bmaxz, pp.h);
+#define CUCO_DEF(NR, CL) \
+    interact<GENERIC, UNIFORM, NR, CL>(sj, nj, pp, s_tab, xi, yi, zi, sx_, sy_, sz_)
+            if (clamp) {
+                if (near) CUCO_DEF(true, true);
+                else CUCO_DEF(false, true);
+            } else if (near) {
+                CUCO_DEF(true, false);
+            } else {
+                CUCO_DEF(false, false);
+            }
+#undef CUCO_DEF
         };
         uint32_t jc = 0;
         uint32_t cursor = 0;
@@ -585,42 +619,73 @@ __global__ __launch_bounds__(kThreads, SYM ? 5 : 1) void k_pp32(
     }
 }
 
-__global__ void k_block_tiles(const uint32_t *__restrict__ off, uint32_t nblocks,
-                              int shift3, uint32_t *__restrict__ block_tiles)
+// Tiles of one super-block: the 8 Morton blocks (2^shift cells a side) of
+// one block a level up.  Consecutive blocks whose particles add up to at
+// most 64 share a tile (in sparse regions a block alone would leave most of
+// the wave idle); a block with more than 64 particles is cut into runs of
+// 64.  A tile is a contiguous range of sorted particles inside one
+// super-block.  Returns the number of tiles; writes them from t0 on when
+// tstart is given.
+__device__ inline uint32_t super_tiles(const uint32_t *__restrict__ off, uint32_t sb, int shift3,
+                                       uint32_t *__restrict__ tstart,
+                                       uint32_t *__restrict__ tcount, uint32_t t0)
 {
-    uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= nblocks) return;
-    uint32_t s = off[(size_t)b << shift3];
-    uint32_t e = off[(size_t)(b + 1) << shift3];
-    block_tiles[b] = (e - s + 63u) / 64u;
+    const size_t base = (size_t)sb << (shift3 + 3);
+    uint32_t t = t0, gs = off[base], gn = 0;
+    auto emit = [&](uint32_t start, uint32_t n) {
+        if (tstart) {
+            tstart[t] = start;
+            tcount[t] = n;
+        }
+        ++t;
+    };
+    for (int k = 0; k < 8; ++k) {
+        const uint32_t bs = off[base + ((size_t)k << shift3)];
+        const uint32_t be = off[base + ((size_t)(k + 1) << shift3)];
+        const uint32_t c = be - bs;
+        if (c > 64u) {
+            if (gn) emit(gs, gn);
+            gn = 0;
+            for (uint32_t p = bs; p < be; p += 64u) emit(p, min(64u, be - p));
+        } else if (gn + c > 64u) {
+            emit(gs, gn);
+            gs = bs;
+            gn = c;
+        } else {
+            if (gn == 0) gs = bs;
+            gn += c;
+        }
+    }
+    if (gn) emit(gs, gn);
+    return t - t0;
+}
+
+__global__ void k_block_tiles(const uint32_t *__restrict__ off, uint32_t nsuper, int shift3,
+                              uint32_t *__restrict__ super_count)
+{
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < nsuper) super_count[b] = super_tiles(off, b, shift3, nullptr, nullptr, 0);
 }
 
 __global__ void k_fill_tiles(const uint32_t *__restrict__ off,
-                             const uint32_t *__restrict__ tile_off,
-                             uint32_t nblocks, int shift3,
-                             uint32_t *__restrict__ tstart,
+                             const uint32_t *__restrict__ tile_off, uint32_t nsuper,
+                             int shift3, uint32_t *__restrict__ tstart,
                              uint32_t *__restrict__ tcount)
 {
-    uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= nblocks) return;
-    uint32_t s = off[(size_t)b << shift3];
-    uint32_t e = off[(size_t)(b + 1) << shift3];
-    uint32_t t = tile_off[b];
-    for (uint32_t p = s; p < e; p += 64u, ++t) {
-        tstart[t] = p;
-        tcount[t] = min(64u, e - p);
-    }
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < nsuper) (void)super_tiles(off, b, shift3, tstart, tcount, tile_off[b]);
 }
 
-// First tile of each box: boxes start on Morton block boundaries, so it is
-// the scanned tile offset of the box's first block.
+// First tile of each box: boxes start on super-block boundaries (a box's
+// key range is 8^bits with bits > shift), so it is the scanned tile offset
+// of the box's first super-block.
 __global__ void k_box_tiles(const uint32_t *__restrict__ tile_off,
                             const BoxDev *__restrict__ boxes, int nboxes,
                             int shift3, const uint32_t *__restrict__ counters,
                             uint32_t *__restrict__ tile_map)
 {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b < nboxes) tile_map[b] = tile_off[boxes[b].key_base >> shift3];
+    if (b < nboxes) tile_map[b] = tile_off[boxes[b].key_base >> (shift3 + 3)];
     if (b == nboxes) tile_map[b] = counters[0];
 }
 
@@ -654,23 +719,19 @@ int launch_build_tiles(const uint32_t *off, const BoxSet &bs, uint32_t nkeys,
                        TileInfo &t, hipStream_t s)
 {
     const int shift3 = 3 * t.block_shift;
-    const uint32_t nblocks = nkeys >> shift3;
+    const uint32_t nsuper = nkeys >> (shift3 + 3);
     // t.nblocks is the capacity (blocks of the smallest tunable shift).
-    if (nblocks == 0 || nblocks > t.nblocks) {
-        set_error("tile build: %u Morton blocks exceed capacity %u", nblocks,
-                  t.nblocks);
+    if (nsuper == 0 || nsuper > t.nblocks) {
+        set_error("tile build: %u Morton super-blocks exceed capacity %u", nsuper, t.nblocks);
         return CUCO_ERR_INVALID_ARG;
     }
-    k_block_tiles<<<grid_for(nblocks, 256), 256, 0, s>>>(off, nblocks, shift3,
-                                                         t.block_tiles);
+    k_block_tiles<<<grid_for(nsuper, 256), 256, 0, s>>>(off, nsuper, shift3, t.block_tiles);
     CUCO_LAUNCH_CHECK();
-    int rc = exclusive_scan_u32(t.block_tiles, t.block_tiles, nblocks,
-                                t.scan_scratch, t.counters, s);
+    int rc = exclusive_scan_u32(t.block_tiles, t.block_tiles, nsuper, t.scan_scratch,
+                                t.counters, s);
     if (rc) return rc;
-    k_fill_tiles<<<grid_for(nblocks, 256), 256, 0, s>>>(off, t.block_tiles,
-                                                        nblocks, shift3,
-                                                        t.tile_start,
-                                                        t.tile_count);
+    k_fill_tiles<<<grid_for(nsuper, 256), 256, 0, s>>>(off, t.block_tiles, nsuper, shift3,
+                                                       t.tile_start, t.tile_count);
     CUCO_LAUNCH_CHECK();
     k_box_tiles<<<grid_for(bs.nboxes + 1, 64), 64, 0, s>>>(
         t.block_tiles, bs.dev[kKeyMorton], bs.nboxes, shift3, t.counters, bs.tile_map);
@@ -708,17 +769,13 @@ P32 make_p32(const PairParams &pp, const GridInfo &g, double uniform_mass,
     return p;
 }
 
-bool pp32_table_cut(const PairParams &pp, const GridInfo &g, int block_shift)
+bool pp32_table_cut(const PairParams &pp)
 {
-    // The table alone can make the cut when rcut sits on a bin edge and no
-    // candidate can reach past the LDS table: a candidate lies within rcut
-    // (x 1.0001) of a tile's box, and the box spans at most one Morton block
-    // (plus the clamp of the last cell), so r <= rcut + sqrt(3)(2^shift+1) cell.
-    const double sc = pp.asmthfac;
-    const double edge = sc * pp.rcut;
-    const bool aligned = fabs(edge - nearbyint(edge)) < 1e-9 * edge;
-    const double rmax = sc * (pp.rcut * 1.001 + sqrt(3.0) * ((1 << block_shift) + 1) * g.cell);
-    return aligned && rmax < kTab - 2;
+    // The table alone can make the cut when rcut sits on a bin edge; bins a
+    // wide tile's candidates could reach past the LDS table are clamped per
+    // tile in the kernel.
+    const double edge = pp.asmthfac * pp.rcut;
+    return fabs(edge - nearbyint(edge)) < 1e-9 * edge;
 }
 
 int launch_pp32(const float4 *spos, const uint32_t *skeys,
