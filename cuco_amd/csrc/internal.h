@@ -18,7 +18,7 @@ enum BoxMap : int {
     kMapLinesF,     // FFT y/x-line blocks, FP32
     kMapRowsR,      // register-FFT z-row blocks (FP32, power-of-two meshes)
     kMapLinesR,     // register-FFT y/x-line blocks
-    kMapBrick,      // 8^3-cell Morton bricks (FP32 interpolation)
+    kMapBrick,      // 16^3-cell Morton bricks (FP32 interpolation)
     kMapCount
 };
 

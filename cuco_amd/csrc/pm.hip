@@ -528,15 +528,19 @@ __global__ __launch_bounds__(256) void k_cic_gather(
     const int M = g.M;
     const int t = threadIdx.x;
     int x, y, z;
+    uint32_t k0 = 0;  // Morton key of the node's own cell (Morton keys)
     if (g.key_mode == kKeyMorton) {
         // Each wave a 4x4x4 Morton sub-cube, so the cell-table entries a wave
-        // reads are a few contiguous Morton runs.
+        // reads are a few contiguous Morton runs; the cube's keys are its
+        // first key plus the lane.
         const int nb8 = (M + 7) / 8;
         const int bz = (int)(lb % nb8), by = (int)((lb / nb8) % nb8), bxx = (int)(lb / (nb8 * nb8));
-        const int w = t >> 6, l = t & 63;
-        x = bxx * 4 + (int)compact3((uint32_t)l >> 2);
-        y = by * 8 + (w >> 1) * 4 + (int)compact3((uint32_t)l >> 1);
-        z = bz * 8 + (w & 1) * 4 + (int)compact3((uint32_t)l);
+        const int w = __builtin_amdgcn_readfirstlane(t >> 6), l = t & 63;
+        const int x0 = bxx * 4, y0 = by * 8 + (w >> 1) * 4, z0 = bz * 8 + (w & 1) * 4;
+        x = x0 + (int)compact3((uint32_t)l >> 2);
+        y = y0 + (int)compact3((uint32_t)l >> 1);
+        z = z0 + (int)compact3((uint32_t)l);
+        k0 = cell_key(g, x0, y0, z0) + (uint32_t)l;
     } else {
         const int nbz = (M + kBz - 1) / kBz, nby = (M + kBy - 1) / kBy;
         const int bz = (int)(lb % nbz), by = (int)((lb / nbz) % nby), bxx = (int)(lb / (nbz * nby));
@@ -549,14 +553,34 @@ __global__ __launch_bounds__(256) void k_cic_gather(
     // Corner c = (a, bb, cc): the node is the cell's upper neighbour along
     // the axes whose bit is set; cell = node - (a, bb, cc).
     uint32_t cs[8], ce[8];
+    if (g.key_mode == kKeyMorton && (M & (M - 1)) == 0) {
+        // Power-of-two mesh: a step of -1 along one axis is a decrement of
+        // that axis's interleaved bits, which wraps 0 to M-1 by itself.
+        const uint32_t all = g.nkeys - 1;
+        const uint32_t zm = 0x49249249u & all, ym = (zm << 1) & all, xm = (zm << 2) & all;
+        auto dec = [](uint32_t k, uint32_t m) { return (((k & m) - 1u) & m) | (k & ~m); };
+        uint32_t key[8];
+        key[0] = k0;
+        key[1] = dec(k0, zm);
+        key[2] = dec(k0, ym);
+        key[3] = dec(key[2], zm);
 #pragma unroll
-    for (int c = 0; c < 8; ++c) {
-        const int cx = wrap_index(x - (c >> 2), M);
-        const int cy = wrap_index(y - ((c >> 1) & 1), M);
-        const int cz = wrap_index(z - (c & 1), M);
-        const uint32_t key = bx.key_base + cell_key(g, cx, cy, cz);
-        cs[c] = off[key];
-        ce[c] = off[key + 1];
+        for (int c = 0; c < 4; ++c) key[4 + c] = dec(key[c], xm);
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            cs[c] = off[bx.key_base + key[c]];
+            ce[c] = off[bx.key_base + key[c] + 1];
+        }
+    } else {
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            const int cx = wrap_index(x - (c >> 2), M);
+            const int cy = wrap_index(y - ((c >> 1) & 1), M);
+            const int cz = wrap_index(z - (c & 1), M);
+            const uint32_t key = bx.key_base + cell_key(g, cx, cy, cz);
+            cs[c] = off[key];
+            ce[c] = off[key + 1];
+        }
     }
     // Cells inside one wave: one load each, issued together (the index of an
     // empty or straddling cell is redirected to a valid slot and masked).
@@ -679,13 +703,43 @@ __global__ __launch_bounds__(256) void k_interp64(const double *__restrict__ phi
     acc[perm[i]] = a;
 }
 
-// FP32: one block per Morton brick of 8^3 cells (512 consecutive keys, so
-// the brick's particles are one sorted range).  The block stages the
-// potential at the 13^3 nodes its particles' stencils can reach (2 below,
+// FP32: one block per Morton brick of 16^3 cells (4096 consecutive keys,
+// so the brick's particles are one sorted range).  The block stages the
+// potential at the 21^3 nodes its particles' stencils can reach (2 below,
 // 3 above the brick, periodic) in LDS once; every particle then reads its
-// 72 values from there.  Same arithmetic as the FP64 kernel, in FP32.
-constexpr int kBrick = 8;
+// 72 values from there.  The same stencil and CIC weights as the FP64
+// kernel, evaluated as nested linear interpolations with fused
+// multiply-adds (FP32 needs no reference rounding order).
+constexpr int kBrick = 16;
 constexpr int kBrickNodes = kBrick + 5;
+
+// Unscaled stencil at node a of a line v[0..5] (v[k] = phi(s-2+k)).
+__device__ inline float stencil32(const float *v, int a)
+{
+    return fmaf(4.0f / 3.0f, v[a + 1] - v[a + 3], -(1.0f / 6.0f) * (v[a] - v[a + 4]));
+}
+
+__device__ inline float lerp32(float g0, float g1, float d) { return fmaf(d, g1 - g0, g0); }
+
+// Component DIM at offsets (dx, dy, dz) inside the cell: the stencil at the
+// 8 CIC nodes, weighted (1-d) / d along each axis.
+template <int DIM>
+__device__ inline float interp32(const Lines<float> &L, float dx, float dy, float dz)
+{
+    float g[2][2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int c = 0; c < 2; ++c)
+                g[a][b][c] = DIM == 0 ? stencil32(L.x[b][c], a)
+                           : DIM == 1 ? stencil32(L.y[a][c], b)
+                                      : stencil32(L.z[a][b], c);
+    const float s00 = lerp32(g[0][0][0], g[0][0][1], dz), s01 = lerp32(g[0][1][0], g[0][1][1], dz);
+    const float s10 = lerp32(g[1][0][0], g[1][0][1], dz), s11 = lerp32(g[1][1][0], g[1][1][1], dz);
+    return lerp32(lerp32(s00, s01, dy), lerp32(s10, s11, dy), dx);
+}
 
 __global__ __launch_bounds__(256) void k_interp32(const float *__restrict__ phi_all,
                                                   const float4 *__restrict__ spos,
@@ -695,25 +749,29 @@ __global__ __launch_bounds__(256) void k_interp32(const float *__restrict__ phi_
                                                   float4 *__restrict__ acc)
 {
     constexpr int E = kBrickNodes;
+    constexpr uint32_t kCells = kBrick * kBrick * kBrick;
     __shared__ float s_phi[E * E * E];
     int bi;
     uint32_t lb;
     if (!box_block(v, bi, lb)) return;
     const BoxDev &bx = v.box[bi];
     const GridInfo &g = bx.g;
-    const uint32_t k0 = lb * (kBrick * kBrick * kBrick);
+    const uint32_t k0 = lb * kCells;
     const uint32_t p_begin = off[bx.key_base + k0];
-    const uint32_t p_end = off[bx.key_base + k0 + kBrick * kBrick * kBrick];
+    const uint32_t p_end = off[bx.key_base + k0 + kCells];
     if (p_begin == p_end) return;  // empty brick (or one outside the mesh)
     const int M = g.M;
     int x0, y0, z0;
     key_cell(g, k0, x0, y0, z0);
     const float *phi = phi_all + bx.grid0;
-    for (int e = threadIdx.x; e < E * E * E; e += 256) {
-        const int ix = e / (E * E), r = e - ix * (E * E), iy = r / E, iz = r - iy * E;
-        const int gx = wrap_index(x0 - 2 + ix, M), gy = wrap_index(y0 - 2 + iy, M),
-                  gz = wrap_index(z0 - 2 + iz, M);
-        s_phi[e] = phi[((size_t)gx * M + gy) * M + gz];
+    // A thread owns node columns (iy, iz) and walks each along x.
+    for (int col = threadIdx.x; col < E * E; col += 256) {
+        const int iy = col / E, iz = col - iy * E;
+        const size_t yz = (size_t)wrap_index(y0 - 2 + iy, M) * M + wrap_index(z0 - 2 + iz, M);
+        const size_t MM = (size_t)M * M;
+#pragma unroll
+        for (int ix = 0; ix < E; ++ix)
+            s_phi[(ix * E + iy) * E + iz] = phi[(size_t)wrap_index(x0 - 2 + ix, M) * MM + yz];
     }
     __syncthreads();
     const float facf = (float)g.fac, fac = (float)bx.pm_fac;
@@ -721,7 +779,7 @@ __global__ __launch_bounds__(256) void k_interp32(const float *__restrict__ phi_
         const float4 q = spos[i];
         int sx, sy, sz;
         key_cell(g, skeys[i] - bx.key_base, sx, sy, sz);
-        const int lx = sx - x0, ly = sy - y0, lz = sz - z0;  // 0..7
+        const int lx = sx - x0, ly = sy - y0, lz = sz - z0;  // 0..kBrick-1
         Lines<float> L;
 #pragma unroll
         for (int a = 0; a < 2; ++a)
@@ -734,12 +792,8 @@ __global__ __launch_bounds__(256) void k_interp32(const float *__restrict__ phi_
                     L.z[a][b][k] = s_phi[((lx + 2 + a) * E + ly + 2 + b) * E + lz + k];
                 }
         const float dx = q.x * facf, dy = q.y * facf, dz = q.z * facf;
-        float4 a;
-        a.x = interp_dim<float, 0>(L, dx, dy, dz, fac);
-        a.y = interp_dim<float, 1>(L, dx, dy, dz, fac);
-        a.z = interp_dim<float, 2>(L, dx, dy, dz, fac);
-        a.w = 0.f;
-        acc[perm[i]] = a;
+        acc[perm[i]] = make_float4(fac * interp32<0>(L, dx, dy, dz), fac * interp32<1>(L, dx, dy, dz),
+                                   fac * interp32<2>(L, dx, dy, dz), 0.f);
     }
 }
 
@@ -818,10 +872,10 @@ uint32_t pm_blocks(int M, int map)
         const uint32_t B = 1u << lines_log2<float>(M);
         return ((uint32_t)M * Zc + B - 1) / B;
     }
-    case kMapBrick: {  // 8^3-cell Morton bricks of the padded key cube
+    case kMapBrick: {  // 16^3-cell Morton bricks of the padded key cube
         int bits = 0;
         while ((1 << bits) < M) ++bits;
-        return bits >= 3 ? 1u << (3 * (bits - 3)) : 1u;
+        return bits >= 4 ? 1u << (3 * (bits - 4)) : 1u;
     }
     case kMapRowsR: return fft32_blocks(M, true);
     case kMapLinesR: return fft32_blocks(M, false);
