@@ -97,6 +97,17 @@ __host__ __device__ inline int cell_coord(double fac, double x, int M)
     return (int)fmin(fmax(fac * x, 0.0), (double)(M - 1));
 }
 
+// off[k] and off[k+1] of a cell table in one 8-byte load (the entries are
+// only 4-byte aligned, which global loads accept).
+typedef uint32_t u32x2_a4 __attribute__((ext_vector_type(2), aligned(4)));
+__device__ inline void load_range(const uint32_t *__restrict__ off, uint32_t k, uint32_t &s,
+                                  uint32_t &e)
+{
+    const u32x2_a4 v = *reinterpret_cast<const u32x2_a4 *>(off + k);
+    s = v.x;
+    e = v.y;
+}
+
 __host__ __device__ inline int wrap_index(int a, int M)
 {
     a = a >= M ? a - M : a;

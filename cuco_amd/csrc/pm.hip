@@ -568,8 +568,7 @@ __global__ __launch_bounds__(256) void k_cic_gather(
         for (int c = 0; c < 4; ++c) key[4 + c] = dec(key[c], xm);
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
-            cs[c] = off[bx.key_base + key[c]];
-            ce[c] = off[bx.key_base + key[c] + 1];
+            load_range(off, bx.key_base + key[c], cs[c], ce[c]);
         }
     } else {
 #pragma unroll
@@ -578,8 +577,7 @@ __global__ __launch_bounds__(256) void k_cic_gather(
             const int cy = wrap_index(y - ((c >> 1) & 1), M);
             const int cz = wrap_index(z - (c & 1), M);
             const uint32_t key = bx.key_base + cell_key(g, cx, cy, cz);
-            cs[c] = off[key];
-            ce[c] = off[key + 1];
+            load_range(off, key, cs[c], ce[c]);
         }
     }
     // Cells inside one wave: one load each, issued together (the index of an

@@ -498,8 +498,9 @@ __global__ __launch_bounds__(kThreads, SYM ? 5 : 1) void k_pp32(
                         uy += uy < 0 ? M : 0;
                         uz += uz < 0 ? M : 0;
                         const uint32_t key = key_base + cell_key(g, ux, uy, uz);
-                        s = off[key];
-                        c = off[key + 1] - s;
+                        uint32_t e1;
+                        load_range(off, key, s, e1);
+                        c = e1 - s;
                         if (SYM) {
                             // only j after the tile's first particle
                             const uint32_t e = s + c, lo = i0 + 1;
