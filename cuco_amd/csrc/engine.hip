@@ -79,6 +79,7 @@ struct HostBox {
     PairParams pp{};
     GridInfo g[2]{};   // per KeyMode
     double pm_fac = 0;
+    double mass_total = 0;  // sum of the box's particle masses
 };
 
 struct cuco_engine {
@@ -328,6 +329,7 @@ int update_boxes(cuco_engine *e)
             d.cgrid0 = cgrid0;
             d.tab0 = (uint32_t)tab0;
             d.pm_fac = h.pm_fac;
+            d.dep_scale = deposit_scale(h.mass_total);
             d.half = make_plan(M / 2);
             d.full = make_plan(M);
             kb[m] += h.g[m].nkeys;
@@ -439,8 +441,7 @@ int ensure_pm(cuco_engine *e)
     TRY(dev_alloc(e->rho[k], sizeof(T) * e->grid_elems));
     TRY(dev_alloc(e->phi[k], sizeof(T) * e->grid_elems));
     TRY(dev_alloc(e->cgrid[k], sizeof(Cx<T>) * e->cgrid_elems));
-    TRY(dev_alloc(e->dep_scratch[k],
-                  pm_deposit_scratch_bytes(e->bs.capacity, e->bs.wave_slots, k == 0)));
+    TRY(dev_alloc(e->dep_scratch[k], pm_deposit_scratch_bytes(e->bs, k == 0)));
     return make_pm_tables<T>(e, e->tb[k]);
 }
 
@@ -1050,6 +1051,7 @@ int cuco_upload_particles(cuco_engine *e, int64_t n, const double *pos,
     for (size_t b = 0; b < e->box.size(); ++b) {
         const double L = e->box[b].prm.box_size;
         const int64_t nb = e->ensemble ? (int64_t)e->box[b].cap : n;
+        double mtot = 0;
         for (int64_t k = 0; k < nb; ++k, ++i) {
             double x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
             if (!std::isfinite(x) || !std::isfinite(y) || !std::isfinite(z)) {
@@ -1061,9 +1063,11 @@ int cuco_upload_particles(cuco_engine *e, int64_t n, const double *pos,
             TRY(wrap_host(&z, L));
             hp[i] = make_double4(x, y, z, mass ? mass[i] : uniform_mass);
             if (hp[i].w != common) common = 0;
+            mtot += hp[i].w;
             hv[i] = vel ? make_double4(vel[3 * i], vel[3 * i + 1], vel[3 * i + 2], 0.0)
                         : make_double4(0, 0, 0, 0);
         }
+        e->box[b].mass_total = mtot;
     }
     CUCO_HIP(hipMemcpyAsync(e->pos, hp.data(), sizeof(double4) * n, hipMemcpyHostToDevice, e->stream));
     CUCO_HIP(hipMemcpyAsync(e->vel, hv.data(), sizeof(double4) * n, hipMemcpyHostToDevice, e->stream));

@@ -10,15 +10,14 @@ namespace cuco {
 // Work units counted by the block maps (one prefix over boxes each).
 enum BoxMap : int {
     kMapPart = 0,   // 256 particles per block
-    kMapGatherX,    // deposit gather bricks, x-major keys
-    kMapGatherM,    // deposit gather bricks, Morton keys
+    kMapGatherX,    // FP64 deposit gather bricks (x-major keys)
     kMapRowsD,      // FFT z-row blocks, FP64
     kMapRowsF,      // FFT z-row blocks, FP32
     kMapLinesD,     // FFT y/x-line blocks, FP64
     kMapLinesF,     // FFT y/x-line blocks, FP32
     kMapRowsR,      // register-FFT z-row blocks (FP32, power-of-two meshes)
     kMapLinesR,     // register-FFT y/x-line blocks
-    kMapBrick,      // 16^3-cell Morton bricks (FP32 interpolation)
+    kMapBrick,      // 16^3-cell Morton bricks (FP32 deposit, interpolation)
     kMapCount
 };
 
@@ -149,12 +148,14 @@ struct PmTables {
     void *gsinc = nullptr;   // M: 1/sinc(pi k/M)^4
     void *gprod = nullptr;   // M: gexp * gsinc (one load per element, FP32 kernels)
 };
-// scratch: pm_deposit_scratch_bytes(capacity, wave_slots, fp64) bytes
-// (per-cell corner sums and per-wave partials).
+// scratch: pm_deposit_scratch_bytes(bs, fp64) bytes (FP64: per-cell corner
+// sums and per-wave partials; FP32: one 17^3 partial tile per brick).
 template <typename T>
 int pm_deposit(const void *spos, const uint32_t *skeys, const uint32_t *off,
                const BoxSet &bs, int mode, T *rho, void *scratch, hipStream_t s);
-size_t pm_deposit_scratch_bytes(uint32_t capacity, uint32_t wave_slots, int fp64);
+size_t pm_deposit_scratch_bytes(const BoxSet &bs, int fp64);
+// Fixed-point scale of the FP32 deposit tiles for a box of total mass m_total.
+double deposit_scale(double m_total);
 template <typename T>
 int pm_fft_forward(const T *rho, Cx<T> *c, const BoxSet &bs, const PmTables &tb,
                    hipStream_t s);

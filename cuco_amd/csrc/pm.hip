@@ -22,6 +22,8 @@
 //
 // This file is compiled with -ffp-contract=off so the FP64 instantiations
 // keep the reference's rounding sequence for the weights and stencil.
+#include <cmath>
+
 #include "internal.h"
 
 namespace cuco {
@@ -384,7 +386,8 @@ __global__ __launch_bounds__(kFftThreads) void k_lines_green(
 }
 
 // ------------------------------------------------------------- deposit
-// Two passes, both deterministic and free of atomics:
+// FP64: two passes, both deterministic and free of atomics, every node
+// summed in one fixed order:
 //
 //  k_cic_segments: one lane per sorted particle.  Each lane forms its eight
 //    CIC corner contributions m*wx*wy*wz (pm_periodic.c:123-131), and the
@@ -397,6 +400,8 @@ __global__ __launch_bounds__(kFftThreads) void k_lines_green(
 //    the cell-table reads of neighbouring nodes share cache lines.  A node
 //    adds, in a fixed order, the corner sum of each of its eight neighbour
 //    cells (a straddling cell adds its wave partials in wave order).
+//
+// FP32 (Morton keys): k_cic_brick and k_cic_sum, below.
 
 struct Src64 {
     const double4 *p;
@@ -426,16 +431,12 @@ struct Src32 {
     }
 };
 
-template <typename T>
-__device__ inline T shfl_up_t(T v, int o) { return __shfl_up(v, o, 64); }
-
 // Waves are counted from each box's first particle, so a box's sums do not
 // depend on where the box sits in the particle arrays.
-template <typename T, typename V>
 __global__ __launch_bounds__(256) void k_cic_segments(
-    V src, const uint32_t *__restrict__ skeys, BoxView v,
-    T *__restrict__ cell_sums, T *__restrict__ first_part,
-    T *__restrict__ last_part)
+    Src64 src, const uint32_t *__restrict__ skeys, BoxView v,
+    double *__restrict__ cell_sums, double *__restrict__ first_part,
+    double *__restrict__ last_part)
 {
     int bi;
     uint32_t lb;
@@ -449,12 +450,11 @@ __global__ __launch_bounds__(256) void k_cic_segments(
     const bool valid = il < nb;
     const uint32_t i = bx.p0 + il;
     const uint32_t key = valid ? skeys[i] : 0xffffffffu;
-    T val[8];
+    double val[8];
     if (valid) {
-        T dx, dy, dz, m;
+        double dx, dy, dz, m;
         src.weights(i, bx.g, dx, dy, dz, m);
-        const T wx[2] = {(T)1.0 - dx, dx}, wy[2] = {(T)1.0 - dy, dy},
-                wz[2] = {(T)1.0 - dz, dz};
+        const double wx[2] = {1.0 - dx, dx}, wy[2] = {1.0 - dy, dy}, wz[2] = {1.0 - dz, dz};
 #pragma unroll
         for (int c = 0; c < 8; ++c) val[c] = m * wx[c >> 2] * wy[(c >> 1) & 1] * wz[c & 1];
     } else {
@@ -469,8 +469,8 @@ __global__ __launch_bounds__(256) void k_cic_segments(
         const bool take = lane >= o && ko == key;
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
-            const T t = shfl_up_t(val[c], o);
-            val[c] += take ? t : (T)0;
+            const double t = __shfl_up(val[c], o, 64);
+            val[c] += take ? t : 0.0;
         }
     }
     // Both shuffles run on every lane: a shuffle inside a short-circuit
@@ -505,20 +505,15 @@ __global__ __launch_bounds__(256) void k_cic_segments(
 
 constexpr int kBx = 4, kBy = 4, kBz = 16;  // x-major node brick, 256 threads
 
-// Blocks per box of the gather: x-major keys walk 4x4x16 bricks, Morton keys
-// 4x8x8 bricks whose waves are 4x4x4 Morton sub-cubes.
-inline uint32_t gather_blocks(int M, int key_mode)
+inline uint32_t gather_blocks(int M)
 {
-    if (key_mode == kKeyMorton)
-        return (uint32_t)(((M + 3) / 4) * ((M + 7) / 8) * ((M + 7) / 8));
     return (uint32_t)(((M + kBx - 1) / kBx) * ((M + kBy - 1) / kBy) * ((M + kBz - 1) / kBz));
 }
 
-template <typename T>
 __global__ __launch_bounds__(256) void k_cic_gather(
-    const uint32_t *__restrict__ off, BoxView v, const T *__restrict__ cell_sums,
-    const T *__restrict__ first_part, const T *__restrict__ last_part,
-    T *__restrict__ rho_all)
+    const uint32_t *__restrict__ off, BoxView v, const double *__restrict__ cell_sums,
+    const double *__restrict__ first_part, const double *__restrict__ last_part,
+    double *__restrict__ rho_all)
 {
     int bi;
     uint32_t lb;
@@ -527,85 +522,217 @@ __global__ __launch_bounds__(256) void k_cic_gather(
     const GridInfo &g = bx.g;
     const int M = g.M;
     const int t = threadIdx.x;
-    int x, y, z;
-    uint32_t k0 = 0;  // Morton key of the node's own cell (Morton keys)
-    if (g.key_mode == kKeyMorton) {
-        // Each wave a 4x4x4 Morton sub-cube, so the cell-table entries a wave
-        // reads are a few contiguous Morton runs; the cube's keys are its
-        // first key plus the lane.
-        const int nb8 = (M + 7) / 8;
-        const int bz = (int)(lb % nb8), by = (int)((lb / nb8) % nb8), bxx = (int)(lb / (nb8 * nb8));
-        const int w = __builtin_amdgcn_readfirstlane(t >> 6), l = t & 63;
-        const int x0 = bxx * 4, y0 = by * 8 + (w >> 1) * 4, z0 = bz * 8 + (w & 1) * 4;
-        x = x0 + (int)compact3((uint32_t)l >> 2);
-        y = y0 + (int)compact3((uint32_t)l >> 1);
-        z = z0 + (int)compact3((uint32_t)l);
-        k0 = cell_key(g, x0, y0, z0) + (uint32_t)l;
-    } else {
-        const int nbz = (M + kBz - 1) / kBz, nby = (M + kBy - 1) / kBy;
-        const int bz = (int)(lb % nbz), by = (int)((lb / nbz) % nby), bxx = (int)(lb / (nbz * nby));
-        z = bz * kBz + (t % kBz);
-        y = by * kBy + ((t / kBz) % kBy);
-        x = bxx * kBx + t / (kBz * kBy);
-    }
+    const int nbz = (M + kBz - 1) / kBz, nby = (M + kBy - 1) / kBy;
+    const int bz = (int)(lb % nbz), by = (int)((lb / nbz) % nby), bxx = (int)(lb / (nbz * nby));
+    const int z = bz * kBz + (t % kBz);
+    const int y = by * kBy + ((t / kBz) % kBy);
+    const int x = bxx * kBx + t / (kBz * kBy);
     if (x >= M || y >= M || z >= M) return;
     const uint32_t p0 = bx.p0;
     // Corner c = (a, bb, cc): the node is the cell's upper neighbour along
     // the axes whose bit is set; cell = node - (a, bb, cc).
     uint32_t cs[8], ce[8];
-    if (g.key_mode == kKeyMorton && (M & (M - 1)) == 0) {
-        // Power-of-two mesh: a step of -1 along one axis is a decrement of
-        // that axis's interleaved bits, which wraps 0 to M-1 by itself.
-        const uint32_t all = g.nkeys - 1;
-        const uint32_t zm = 0x49249249u & all, ym = (zm << 1) & all, xm = (zm << 2) & all;
-        auto dec = [](uint32_t k, uint32_t m) { return (((k & m) - 1u) & m) | (k & ~m); };
-        uint32_t key[8];
-        key[0] = k0;
-        key[1] = dec(k0, zm);
-        key[2] = dec(k0, ym);
-        key[3] = dec(key[2], zm);
 #pragma unroll
-        for (int c = 0; c < 4; ++c) key[4 + c] = dec(key[c], xm);
-#pragma unroll
-        for (int c = 0; c < 8; ++c) {
-            load_range(off, bx.key_base + key[c], cs[c], ce[c]);
-        }
-    } else {
-#pragma unroll
-        for (int c = 0; c < 8; ++c) {
-            const int cx = wrap_index(x - (c >> 2), M);
-            const int cy = wrap_index(y - ((c >> 1) & 1), M);
-            const int cz = wrap_index(z - (c & 1), M);
-            const uint32_t key = bx.key_base + cell_key(g, cx, cy, cz);
-            load_range(off, key, cs[c], ce[c]);
-        }
+    for (int c = 0; c < 8; ++c) {
+        const int cx = wrap_index(x - (c >> 2), M);
+        const int cy = wrap_index(y - ((c >> 1) & 1), M);
+        const int cz = wrap_index(z - (c & 1), M);
+        load_range(off, bx.key_base + cell_key(g, cx, cy, cz), cs[c], ce[c]);
     }
     // Cells inside one wave: one load each, issued together (the index of an
     // empty or straddling cell is redirected to a valid slot and masked).
-    T val[8];
+    double val[8];
     bool straddle = false;
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
         const bool nonempty = ce[c] > cs[c];
         const bool single = nonempty && ((cs[c] - p0) >> 6) == ((ce[c] - 1 - p0) >> 6);
         straddle |= nonempty && !single;
-        const T tv = cell_sums[(single ? (size_t)cs[c] * 8 : 0) + c];
-        val[c] = single ? tv : (T)0;
+        const double tv = cell_sums[(single ? (size_t)cs[c] * 8 : 0) + c];
+        val[c] = single ? tv : 0.0;
     }
     double sum = 0;
 #pragma unroll
-    for (int c = 0; c < 8; ++c) sum += (double)val[c];
+    for (int c = 0; c < 8; ++c) sum += val[c];
     if (straddle) {
         for (int c = 0; c < 8; ++c) {
             if (ce[c] <= cs[c]) continue;
             const uint32_t w0 = (cs[c] - p0) >> 6, w1 = (ce[c] - 1 - p0) >> 6;
             if (w0 == w1) continue;
-            sum += (double)last_part[((size_t)bx.wave0 + w0) * 8 + c];
+            sum += last_part[((size_t)bx.wave0 + w0) * 8 + c];
             for (uint32_t w = w0 + 1; w <= w1; ++w)
-                sum += (double)first_part[((size_t)bx.wave0 + w) * 8 + c];
+                sum += first_part[((size_t)bx.wave0 + w) * 8 + c];
         }
     }
-    rho_all[bx.grid0 + ((size_t)x * M + y) * M + z] = (T)sum;
+    rho_all[bx.grid0 + ((size_t)x * M + y) * M + z] = sum;
+}
+
+// FP32 deposit over Morton bricks of 16^3 cells (4096 consecutive keys, so
+// a brick's particles are one sorted range).  A brick's cells reach the
+// 17^3 nodes at local coordinates 0..16 of its corner; nodes 1..15 along
+// every axis get nothing from other bricks.
+//
+//  k_cic_brick: one block per brick.  The block sums its particles' CIC
+//    corners into the 17^3-node tile (a wave-level segmented scan by cell
+//    first where a wave holds repeated cells, then the corner sums added in
+//    64-bit fixed point with LDS atomics: integer adds commute, so the tile
+//    does not depend on the order of waves or particles).  Interior nodes
+//    go to the mesh; the shell (a coordinate 0 or 16) to the brick's
+//    partial tile.
+//  k_cic_shell: one block per brick for its nodes with a local coordinate 0
+//    (the lower faces): the brick's own shell value plus, along every axis
+//    where the coordinate is 0, the brick below at 16 (periodic; at the
+//    mesh edge the last brick, at M minus its corner), summed in a fixed
+//    order.
+constexpr int kDepBrick = 16;
+constexpr int kDepEdge = kDepBrick + 1;
+constexpr int kDepNodes = kDepEdge * kDepEdge * kDepEdge;
+constexpr uint32_t kDepCells = kDepBrick * kDepBrick * kDepBrick;
+
+__global__ __launch_bounds__(256) void k_cic_brick(Src32 src, const uint32_t *__restrict__ skeys,
+                                                   const uint32_t *__restrict__ off, BoxView v,
+                                                   float *__restrict__ partial,
+                                                   float *__restrict__ rho_all)
+{
+    __shared__ unsigned long long s_node[kDepNodes];
+    int bi;
+    uint32_t lb;
+    if (!box_block(v, bi, lb)) return;
+    const BoxDev &bx = v.box[bi];
+    const GridInfo &g = bx.g;
+    const int M = g.M;
+    const uint32_t k0 = lb * kDepCells;
+    int x0, y0, z0;
+    key_cell(g, k0, x0, y0, z0);
+    if (x0 >= M || y0 >= M || z0 >= M) return;  // outside the mesh: no cells
+    for (int e = threadIdx.x; e < kDepNodes; e += 256) s_node[e] = 0ull;
+    const uint32_t p_begin = off[bx.key_base + k0];
+    const uint32_t p_end = off[bx.key_base + k0 + kDepCells];
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const double scale = bx.dep_scale;
+    for (uint32_t base = p_begin; base < p_end; base += 256) {
+        const uint32_t i = base + threadIdx.x;
+        const bool valid = i < p_end;
+        const uint32_t key = valid ? skeys[i] : 0xffffffffu;
+        float val[8];
+        if (valid) {
+            float dx, dy, dz, m;
+            src.weights(i, g, dx, dy, dz, m);
+            const float wx[2] = {1.0f - dx, dx}, wy[2] = {1.0f - dy, dy}, wz[2] = {1.0f - dz, dz};
+#pragma unroll
+            for (int c = 0; c < 8; ++c) val[c] = m * wx[c >> 2] * wy[(c >> 1) & 1] * wz[c & 1];
+        } else {
+#pragma unroll
+            for (int c = 0; c < 8; ++c) val[c] = 0.f;
+        }
+        const uint32_t kprev = __shfl_up(key, 1, 64);
+        const uint32_t knext = __shfl_down(key, 1, 64);
+        // Segmented inclusive scan by cell (keys non-decreasing along the
+        // wave), only where some cell repeats: then one lane per cell adds.
+        if (__ballot(valid && lane > 0 && kprev == key) != 0ull) {
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t ko = __shfl_up(key, o, 64);
+                const bool take = lane >= o && ko == key;
+#pragma unroll
+                for (int c = 0; c < 8; ++c) {
+                    const float t = __shfl_up(val[c], o, 64);
+                    val[c] += take ? t : 0.f;
+                }
+            }
+        }
+        if (valid && (lane == 63 || knext != key)) {
+            int cx, cy, cz;
+            key_cell(g, key - bx.key_base, cx, cy, cz);
+            const int n0 = ((cx - x0) * kDepEdge + (cy - y0)) * kDepEdge + (cz - z0);
+#pragma unroll
+            for (int c = 0; c < 8; ++c)
+                atomicAdd(&s_node[n0 + ((c >> 2) * kDepEdge + ((c >> 1) & 1)) * kDepEdge + (c & 1)],
+                          (unsigned long long)__double2ll_rn((double)val[c] * scale));
+        }
+    }
+    __syncthreads();
+    const double inv = 1.0 / scale;
+    float *part = partial + (size_t)(v.map[bi] + lb) * kDepNodes;
+    float *rho = rho_all + bx.grid0;
+    // A thread owns node columns (ly, lz) and walks them along x.  Interior
+    // nodes inside the mesh are complete; the rest -- the shell and, where M
+    // is not a multiple of 16, the node at M (node 0 of the next period) --
+    // wait for k_cic_shell.
+    for (int col = threadIdx.x; col < kDepEdge * kDepEdge; col += 256) {
+        const int ly = col / kDepEdge, lz = col - ly * kDepEdge;
+        const bool yz_in = ly > 0 && lz > 0 && ly < kDepBrick && lz < kDepBrick && y0 + ly < M &&
+                           z0 + lz < M;
+        const size_t yz = (size_t)(y0 + ly) * M + (z0 + lz);
+        for (int lx = 0; lx < kDepEdge; ++lx) {
+            const int e = lx * kDepEdge * kDepEdge + col;
+            const float val = (float)((double)(long long)s_node[e] * inv);
+            if (yz_in && lx > 0 && lx < kDepBrick && x0 + lx < M)
+                rho[(size_t)(x0 + lx) * M * M + yz] = val;
+            else
+                part[e] = val;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_cic_shell(const float *__restrict__ partial, BoxView v,
+                                                   float *__restrict__ rho_all)
+{
+    int bi;
+    uint32_t lb;
+    if (!box_block(v, bi, lb)) return;
+    const BoxDev &bx = v.box[bi];
+    const GridInfo &g = bx.g;
+    const int M = g.M;
+    int o[3];
+    key_cell(g, lb * kDepCells, o[0], o[1], o[2]);
+    if (o[0] >= M || o[1] >= M || o[2] >= M) return;
+    // Per axis: the brick below (the last one of the mesh at the edge) and
+    // the local coordinate the node has in its tile.
+    const int last = ((M - 1) / kDepBrick) * kDepBrick;
+    int below[3], lbel[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        below[d] = o[d] == 0 ? last : o[d] - kDepBrick;
+        lbel[d] = o[d] == 0 ? M - last : kDepBrick;
+    }
+    // Tile of the brick below along the set bits of c (bit 2: x, 1: y, 0: z).
+    const float *tile[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        const uint32_t brick =
+            cell_key(g, (c & 4) ? below[0] : o[0], (c & 2) ? below[1] : o[1],
+                     (c & 1) ? below[2] : o[2]) / kDepCells;
+        tile[c] = partial + (size_t)(v.map[bi] + brick) * kDepNodes;
+    }
+    float *rho = rho_all + bx.grid0;
+    // Lower-face nodes: x = 0 (16x16), then y = 0 with x > 0 (15x16), then
+    // z = 0 with x, y > 0 (15x15).
+    constexpr int kFx = kDepBrick * kDepBrick, kFy = (kDepBrick - 1) * kDepBrick;
+    constexpr int kFaces = kFx + kFy + (kDepBrick - 1) * (kDepBrick - 1);
+    for (int e = threadIdx.x; e < kFaces; e += 256) {
+        int l[3];
+        if (e < kFx) {
+            l[0] = 0; l[1] = e / kDepBrick; l[2] = e % kDepBrick;
+        } else if (e < kFx + kFy) {
+            const int f = e - kFx;
+            l[0] = 1 + f / kDepBrick; l[1] = 0; l[2] = f % kDepBrick;
+        } else {
+            const int f = e - kFx - kFy;
+            l[0] = 1 + f / (kDepBrick - 1); l[1] = 1 + f % (kDepBrick - 1); l[2] = 0;
+        }
+        if (o[0] + l[0] >= M || o[1] + l[1] >= M || o[2] + l[2] >= M) continue;
+        float sum = 0.f;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            if (((c & 4) && l[0]) || ((c & 2) && l[1]) || ((c & 1) && l[2])) continue;
+            const int ax = (c & 4) ? lbel[0] : l[0], ay = (c & 2) ? lbel[1] : l[1],
+                      az = (c & 1) ? lbel[2] : l[2];
+            sum += tile[c][(ax * kDepEdge + ay) * kDepEdge + az];
+        }
+        rho[((size_t)(o[0] + l[0]) * M + (o[1] + l[1])) * M + (o[2] + l[2])] = sum;
+    }
 }
 
 // ------------------------------------------------------------- interpolation
@@ -819,45 +946,53 @@ template <typename T>
 int pm_deposit(const void *spos, const uint32_t *skeys, const uint32_t *off,
                const BoxSet &bs, int mode, T *rho, void *scratch, hipStream_t s)
 {
-    T *cell_sums = static_cast<T *>(scratch);
-    T *first_part = cell_sums + (size_t)bs.capacity * 8;
-    T *last_part = first_part + (size_t)bs.wave_slots * 8;
-    const BoxView vp = bs.view(mode, kMapPart);
-    if (bs.total[kMapPart] > 0) {
-        if (sizeof(T) == 8) {
-            Src64 src{static_cast<const double4 *>(spos)};
-            k_cic_segments<double, Src64><<<xcd_grid(bs.total[kMapPart]), 256, 0, s>>>(
-                src, skeys, vp, reinterpret_cast<double *>(cell_sums),
-                reinterpret_cast<double *>(first_part),
-                reinterpret_cast<double *>(last_part));
-        } else {
-            Src32 src{static_cast<const float4 *>(spos)};
-            k_cic_segments<float, Src32><<<xcd_grid(bs.total[kMapPart]), 256, 0, s>>>(
-                src, skeys, vp, reinterpret_cast<float *>(cell_sums),
-                reinterpret_cast<float *>(first_part),
-                reinterpret_cast<float *>(last_part));
+    if constexpr (sizeof(T) == 4) {
+        float *partial = static_cast<float *>(scratch);
+        k_cic_brick<<<xcd_grid(bs.total[kMapBrick]), 256, 0, s>>>(
+            Src32{static_cast<const float4 *>(spos)}, skeys, off, bs.view(mode, kMapBrick),
+            partial, reinterpret_cast<float *>(rho));
+        CUCO_LAUNCH_CHECK();
+        k_cic_shell<<<xcd_grid(bs.total[kMapBrick]), 256, 0, s>>>(
+            partial, bs.view(mode, kMapBrick), reinterpret_cast<float *>(rho));
+        CUCO_LAUNCH_CHECK();
+    } else {
+        double *cell_sums = static_cast<double *>(scratch);
+        double *first_part = cell_sums + (size_t)bs.capacity * 8;
+        double *last_part = first_part + (size_t)bs.wave_slots * 8;
+        if (bs.total[kMapPart] > 0) {
+            k_cic_segments<<<xcd_grid(bs.total[kMapPart]), 256, 0, s>>>(
+                Src64{static_cast<const double4 *>(spos)}, skeys, bs.view(mode, kMapPart),
+                cell_sums, first_part, last_part);
+            CUCO_LAUNCH_CHECK();
         }
+        k_cic_gather<<<xcd_grid(bs.total[kMapGatherX]), 256, 0, s>>>(
+            off, bs.view(mode, kMapGatherX), cell_sums, first_part, last_part, rho);
         CUCO_LAUNCH_CHECK();
     }
-    const int map = mode == kKeyMorton ? kMapGatherM : kMapGatherX;
-    k_cic_gather<T><<<xcd_grid(bs.total[map]), 256, 0, s>>>(
-        off, bs.view(mode, map), cell_sums, first_part, last_part, rho);
-    CUCO_LAUNCH_CHECK();
     return CUCO_OK;
 }
 
-size_t pm_deposit_scratch_bytes(uint32_t capacity, uint32_t wave_slots, int fp64)
+size_t pm_deposit_scratch_bytes(const BoxSet &bs, int fp64)
 {
-    const size_t elem = fp64 ? 8 : 4;
-    return elem * 8 * ((size_t)capacity + (size_t)wave_slots * 2 + 64);
+    if (!fp64) return sizeof(float) * (size_t)kDepNodes * bs.total[kMapBrick];
+    return sizeof(double) * 8 * ((size_t)bs.capacity + (size_t)bs.wave_slots * 2 + 64);
+}
+
+double deposit_scale(double m_total)
+{
+    // m_total * scale < 2^61: no node of a tile can reach the int64 range,
+    // with room for the rounding of every contribution.
+    if (!(m_total > 0) || !std::isfinite(m_total)) return 1.0;
+    int ex = 0;
+    (void)std::frexp(m_total, &ex);  // m_total < 2^ex
+    return std::ldexp(1.0, 61 - ex);
 }
 
 uint32_t pm_blocks(int M, int map)
 {
     const uint32_t Zc = (uint32_t)(M / 2 + 1);
     switch (map) {
-    case kMapGatherX: return gather_blocks(M, kKeyXMajor);
-    case kMapGatherM: return gather_blocks(M, kKeyMorton);
+    case kMapGatherX: return gather_blocks(M);
     case kMapRowsD: { const uint32_t B = fft_elems<double>() / (M / 2); return ((uint32_t)M * M + B - 1) / B; }
     case kMapRowsF: {
         if (fft32_supported(M)) return 0;
