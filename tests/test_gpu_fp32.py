@@ -64,6 +64,18 @@ def test_halo128_error_percentiles():
     assert errs["pm"]["p99"] <= P99_BAR, errs
 
 
+@pytest.mark.parametrize("M", [40, 48, 96])
+def test_error_percentiles_other_meshes(orc, M):
+    """Meshes that are not powers of two (LDS Stockham FFT) or not multiples
+    of the 16-cell deposit and interpolation bricks (the last brick wraps)."""
+    box = ics.small_box(f"mesh{M}", 24, M, 60.0, 30 + M, "halos", n_halos=8)
+    pos, vel, mass = ics.generate(box)
+    eng, errs = fp32_errors(box, pos, mass, orc)
+    print(M, errs)
+    for k in ("pp", "pm", "total"):
+        assert errs[k]["p99"] <= P99_BAR, (k, errs)
+
+
 def test_fp32_is_deterministic_and_counts_cover_pairs():
     box = halo_box(seed=4)
     pos, vel, mass = ics.generate(box)

@@ -148,14 +148,18 @@ def test_errors_are_reported_not_fatal(lattice):
 
 
 @pytest.mark.parametrize("precision", [E.FP64, E.FP32])
-def test_deposit_cells_spanning_many_waves(orc, precision):
+@pytest.mark.parametrize("M", [32, 40, 24])
+def test_deposit_cells_spanning_many_waves(orc, precision, M):
     """Dense cells hold hundreds of particles (their sorted run crosses many
     64-particle waves) and pairs sit on wave boundaries; the mesh density must
-    still match the oracle's direct CIC deposit."""
-    box = ics.small_box("dense", 16, 32, 40.0, 8)
+    still match the oracle's direct CIC deposit.  Meshes that are and are not
+    multiples of the FP32 deposit's 16-cell bricks; a dense blob on the upper
+    mesh edge deposits across the periodic boundary."""
+    box = ics.small_box("dense", 16, M, 40.0, 8)
     rng = np.random.default_rng(8)
     cell = box.L / box.M
-    blobs = [np.array([3.5, 7.5, 11.5]) * cell, np.array([20.2, 0.1, 31.9]) * cell]
+    blobs = [np.array([3.5, 7.5, 11.5]) * cell,
+             np.array([0.63 * M, 0.1, M - 0.1]) * cell]
     pos = np.concatenate([
         rng.uniform(0, box.L, (3000, 3)),
         blobs[0] + rng.uniform(-0.45, 0.45, (700, 3)) * cell,
