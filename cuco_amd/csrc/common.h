@@ -97,6 +97,14 @@ __host__ __device__ inline int cell_coord(double fac, double x, int M)
     return (int)fmin(fmax(fac * x, 0.0), (double)(M - 1));
 }
 
+// Orders one wave's LDS accesses before and after the call.
+__device__ inline void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // off[k] and off[k+1] of a cell table in one 8-byte load (the entries are
 // only 4-byte aligned, which global loads accept).
 typedef uint32_t u32x2_a4 __attribute__((ext_vector_type(2), aligned(4)));

@@ -120,8 +120,7 @@ __global__ __launch_bounds__(kThreads) void k_rs_downsweep(
 {
     __shared__ uint32_t s_keys[kSortTile];
     __shared__ uint32_t s_vals[kSortTile];
-    __shared__ uint32_t s_wcnt[4][kBins];
-    __shared__ uint32_t s_run[kBins];
+    __shared__ uint32_t s_wcnt[4][kBins];  // per-wave digit counts, then offsets
     __shared__ uint32_t s_tileoff[kBins];
     __shared__ uint32_t s_gbase[kBins];
     __shared__ uint32_t s_scan[4];
@@ -132,54 +131,61 @@ __global__ __launch_bounds__(kThreads) void k_rs_downsweep(
     uint32_t tot;
     uint32_t dbase = block_excl_scan256(totals[tid], s_scan, tot);
     s_gbase[tid] = dbase + counts[(size_t)tid * nblocks + blockIdx.x];
-    s_run[tid] = 0;
 #pragma unroll
     for (int w = 0; w < 4; ++w) s_wcnt[w][tid] = 0;
     __syncthreads();
 
+    // Each wave ranks its own contiguous quarter of the tile, 64 keys at a
+    // time: a key's rank among equal digits of the quarter is the wave's
+    // running count of that digit plus its rank inside the 64 (eight-ballot
+    // match).  No block barrier until the quarter is done.
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    uint32_t *wc = s_wcnt[wave];
+    const uint32_t wbase = base + (uint32_t)wave * (kSortTile / 4);
     uint32_t my_key[kItems], my_val[kItems], my_rank[kItems];
-
 #pragma unroll
     for (int r = 0; r < kItems; ++r) {
-        const uint32_t e = base + r * kThreads + tid;
+        const uint32_t e = wbase + (uint32_t)r * 64 + lane;
         const bool valid = e < n;
-        uint32_t k = valid ? keys_in[e] : 0u;
-        uint32_t v = valid ? (vals_in ? vals_in[e] : e) : 0u;
-        uint32_t d = (k >> shift) & 255u;
+        const uint32_t k = valid ? keys_in[e] : 0u;
+        const uint32_t v = valid ? (vals_in ? vals_in[e] : e) : 0u;
+        const uint32_t d = (k >> shift) & 255u;
         uint64_t m = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < 8; ++b) {
             const bool bit = (d >> b) & 1u;
-            uint64_t bb = __ballot(bit);
+            const uint64_t bb = __ballot(bit);
             m &= bit ? bb : ~bb;
         }
         const uint32_t rank_w = (uint32_t)__popcll(m & lt);
-        if (valid && rank_w == 0) s_wcnt[wave][d] = (uint32_t)__popcll(m);
-        __syncthreads();
-        uint32_t pre = s_run[d];
-        for (int w = 0; w < wave; ++w) pre += s_wcnt[w][d];
-        my_rank[r] = pre + rank_w;
+        const uint32_t before = valid ? wc[d] : 0u;
+        wave_sync();  // every lane has read the count before its leader updates it
+        if (valid && rank_w == 0) wc[d] = before + (uint32_t)__popcll(m);
+        wave_sync();
+        my_rank[r] = before + rank_w;
         my_key[r] = k;
         my_val[r] = v;
-        __syncthreads();
-        s_run[tid] += s_wcnt[0][tid] + s_wcnt[1][tid] + s_wcnt[2][tid] +
-                      s_wcnt[3][tid];
-#pragma unroll
-        for (int w = 0; w < 4; ++w) s_wcnt[w][tid] = 0;
-        __syncthreads();
     }
+    __syncthreads();
 
+    // Per digit: tile offset, then each wave's start inside the digit run.
+    const uint32_t c0 = s_wcnt[0][tid], c1 = s_wcnt[1][tid], c2 = s_wcnt[2][tid],
+                   c3 = s_wcnt[3][tid];
     uint32_t tt;
-    s_tileoff[tid] = block_excl_scan256(s_run[tid], s_scan, tt);
+    const uint32_t toff = block_excl_scan256(c0 + c1 + c2 + c3, s_scan, tt);
+    s_tileoff[tid] = toff;
+    s_wcnt[0][tid] = toff;
+    s_wcnt[1][tid] = toff + c0;
+    s_wcnt[2][tid] = toff + c0 + c1;
+    s_wcnt[3][tid] = toff + c0 + c1 + c2;
     __syncthreads();
 
 #pragma unroll
     for (int r = 0; r < kItems; ++r) {
-        const uint32_t e = base + r * kThreads + tid;
+        const uint32_t e = wbase + (uint32_t)r * 64 + lane;
         if (e < n) {
-            uint32_t d = (my_key[r] >> shift) & 255u;
-            uint32_t lp = s_tileoff[d] + my_rank[r];
+            const uint32_t d = (my_key[r] >> shift) & 255u;
+            const uint32_t lp = wc[d] + my_rank[r];
             s_keys[lp] = my_key[r];
             s_vals[lp] = my_val[r];
         }
