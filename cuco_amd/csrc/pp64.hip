@@ -62,13 +62,43 @@ __device__ inline void pair64(const PairParams &pp, const double *__restrict__ t
     }
 }
 
-// Squared distance in cell units from u to the cell interval [c, c+1),
-// shrunk by a tolerance far above FP64 rounding of fac*x.
-__device__ inline double gap_cells(double u, int c)
+// Cell pruning only has to keep every cell that can hold a pair inside
+// rcut: cells it keeps beyond rcut contribute nothing (their pairs fail the
+// r^2 test) and do not change the order of the others.  So it runs in FP32
+// with margins far above FP32 rounding of coordinates below 2^12 cells.
+//
+// Squared distance in cell units from u to the cell interval [c, c+1), made
+// smaller by the margin.
+__device__ inline float gap_cells(float u, int c)
 {
-    double g = fmax(0.0, fmax((double)c - u, u - (double)(c + 1)));
-    g = fmax(0.0, g - 1e-6);
+    float g = fmaxf(0.0f, fmaxf((float)c - u, u - (float)(c + 1)));
+    g = fmaxf(0.0f, g - 1e-2f);
     return g * g;
+}
+
+__device__ inline int wave_min_i(int v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+    return __builtin_amdgcn_readfirstlane(v);
+}
+
+__device__ inline int wave_max_i(int v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    return __builtin_amdgcn_readfirstlane(v);
+}
+
+// z cells of column (ix, iy) that can hold a pair inside rcut for the target
+// at cell coordinate uz: [za, zb] inside the reference range [zl, zr], or
+// za > zb.  gxy: squared gap to the column in cells (both widened by the
+// margin).
+__device__ inline void z_window(float uz, float gxy, float rc2, int zl, int zr, int &za, int &zb)
+{
+    const float zext = sqrtf(fmaxf(rc2 - gxy, 0.0f)) + 1e-2f;
+    za = max(zl, (int)floorf(uz - zext));
+    zb = min(zr, (int)floorf(uz + zext));
 }
 
 template <bool COUNT>
@@ -83,18 +113,20 @@ __global__ __launch_bounds__(kThreads) void k_pp64(
     for (int k = threadIdx.x; k < kNTab; k += kThreads) tab[k] = (double)table[k];
     __syncthreads();
     const uint32_t il = blockIdx.x * kThreads + threadIdx.x;
-    if (il >= n) return;
-    const uint32_t i = i0 + il;
-    if (active_end && active_end[perm[i]] != ti_current) return;
+    const uint32_t i = i0 + min(il, n - 1);
+    // Lanes past the end or outside the active set compute nothing but stay
+    // for the wave-wide decisions below.
+    bool live = il < n;
+    if (live && active_end && active_end[perm[i]] != ti_current) live = false;
 
     const int M = pp.M;
     const double fac = pp.fac;
     const double4 p = spos[i];
-    if (!(p.x >= 0 && p.x < pp.L && p.y >= 0 && p.y < pp.L && p.z >= 0 && p.z < pp.L)) {
+    if (live && !(p.x >= 0 && p.x < pp.L && p.y >= 0 && p.y < pp.L && p.z >= 0 && p.z < pp.L)) {
         // Positions are wrapped on upload and by every drift; anything else
         // is left without a short-range force rather than indexing wildly.
         acc[perm[i]] = make_double4(0.0, 0.0, 0.0, 0.0);
-        return;
+        live = false;
     }
     int xl = (int)(fac * (p.x - pp.rcut) + M);
     xl -= M;
@@ -106,47 +138,88 @@ __global__ __launch_bounds__(kThreads) void k_pp64(
     zl -= M;
     int zr = (int)(fac * (p.z + pp.rcut));
 
-    const double ux = fac * p.x, uy = fac * p.y, uz = fac * p.z;
-    const double rc = pp.rcut * fac;
-    const double rc2 = rc * rc;
+    const float ux = (float)(fac * p.x), uy = (float)(fac * p.y), uz = (float)(fac * p.z);
+    const float rc2 = (float)(pp.rcut * fac * pp.rcut * fac) * (1.0f + 1e-4f);
 
     double ax = 0, ay = 0, az = 0;
     uint32_t in_count = 0;
-    for (int ix = xl; ix <= xr; ++ix) {
-        const double gx = gap_cells(ux, ix);
-        if (gx > rc2) continue;
-        const int iix = wrap_index(ix, M);
-        for (int iy = yl; iy <= yr; ++iy) {
-            const double gxy = gx + gap_cells(uy, iy);
-            if (gxy > rc2) continue;
-            const int iiy = wrap_index(iy, M);
-            const double zext = sqrt(rc2 - gxy);
-            const int za = max(zl, (int)floor(uz - zext - 1e-6));
-            const int zb = min(zr, (int)floor(uz + zext + 1e-6));
-            if (za > zb) continue;
-            const uint32_t col = ((uint32_t)iix * M + iiy) * (uint32_t)M;
-            // Up to two contiguous runs in reference order (wrapped part first
-            // when the range starts below 0).
-            int ra[2], rb[2], nr = 0;
-            if (za < 0 && zb >= 0) {
-                ra[0] = za + M; rb[0] = M - 1;
-                ra[1] = 0; rb[1] = zb; nr = 2;
-            } else if (za < M && zb >= M) {
-                ra[0] = za; rb[0] = M - 1;
-                ra[1] = 0; rb[1] = zb - M; nr = 2;
-            } else {
-                ra[0] = wrap_index(za, M); rb[0] = wrap_index(zb, M); nr = 1;
+
+    // Wave-uniform walk when the wave's targets are close together (dense
+    // regions: a handful of z cells of one column): the wave visits the union
+    // of its lanes' cell ranges in the reference order (ix, iy, z ascending,
+    // unwrapped) with uniform loop bounds and uniform j loads, and every lane
+    // takes only the cells of its own range -- the pairs and their order for
+    // each target are exactly those of the per-lane walk below.
+    constexpr int kBig = 1 << 30;
+    const int XL = wave_min_i(live ? xl : kBig), XR = wave_max_i(live ? xr : -kBig);
+    const int YL = wave_min_i(live ? yl : kBig), YR = wave_max_i(live ? yr : -kBig);
+    const int ZL = wave_min_i(live ? zl : kBig), ZR = wave_max_i(live ? zr : -kBig);
+    if (XL > XR) return;  // no live lane
+    const bool compact = XR - XL <= (xr - xl) + 1 && YR - YL <= (yr - yl) + 1 &&
+                         ZR - ZL <= (zr - zl) + 8 && XR - XL < M && YR - YL < M && ZR - ZL < M;
+    if (__builtin_amdgcn_readfirstlane((uint32_t)(__ballot(live && !compact) == 0ull))) {
+        for (int ix = XL; ix <= XR; ++ix) {
+            const float gx = gap_cells(ux, ix);
+            const bool in_x = live && ix >= xl && ix <= xr && gx <= rc2;
+            const int iix = wrap_index(ix, M);
+            for (int iy = YL; iy <= YR; ++iy) {
+                const float gxy = gx + gap_cells(uy, iy);
+                int za = 1, zb = 0;
+                if (in_x && iy >= yl && iy <= yr && gxy <= rc2) z_window(uz, gxy, rc2, zl, zr, za, zb);
+                const int ZA = wave_min_i(za <= zb ? za : kBig);
+                const int ZB = wave_max_i(za <= zb ? zb : -kBig);
+                if (ZA > ZB) continue;
+                const uint32_t col = ((uint32_t)iix * M + wrap_index(iy, M)) * (uint32_t)M;
+                for (int z = ZA; z <= ZB; ++z) {
+                    const bool in_z = z >= za && z <= zb;
+                    uint32_t js, je;
+                    load_range(off, col + wrap_index(z, M), js, je);
+                    js = __builtin_amdgcn_readfirstlane(js);
+                    je = __builtin_amdgcn_readfirstlane(je);
+                    for (uint32_t j = js; j < je; ++j) {
+                        const double4 q = spos[j];
+                        if (in_z && j != i) pair64(pp, tab, p.x, p.y, p.z, q, ax, ay, az, in_count);
+                    }
+                }
             }
-            for (int q = 0; q < nr; ++q) {
-                const uint32_t js = off[col + ra[q]];
-                const uint32_t je = off[col + rb[q] + 1];
-                for (uint32_t j = js; j < je; ++j) {
-                    if (j == i) continue;
-                    pair64(pp, tab, p.x, p.y, p.z, spos[j], ax, ay, az, in_count);
+        }
+    } else if (live) {
+        for (int ix = xl; ix <= xr; ++ix) {
+            const float gx = gap_cells(ux, ix);
+            if (gx > rc2) continue;
+            const int iix = wrap_index(ix, M);
+            for (int iy = yl; iy <= yr; ++iy) {
+                const float gxy = gx + gap_cells(uy, iy);
+                if (gxy > rc2) continue;
+                const int iiy = wrap_index(iy, M);
+                int za, zb;
+                z_window(uz, gxy, rc2, zl, zr, za, zb);
+                if (za > zb) continue;
+                const uint32_t col = ((uint32_t)iix * M + iiy) * (uint32_t)M;
+                // Up to two contiguous runs in reference order (wrapped part
+                // first when the range starts below 0).
+                int ra[2], rb[2], nr = 0;
+                if (za < 0 && zb >= 0) {
+                    ra[0] = za + M; rb[0] = M - 1;
+                    ra[1] = 0; rb[1] = zb; nr = 2;
+                } else if (za < M && zb >= M) {
+                    ra[0] = za; rb[0] = M - 1;
+                    ra[1] = 0; rb[1] = zb - M; nr = 2;
+                } else {
+                    ra[0] = wrap_index(za, M); rb[0] = wrap_index(zb, M); nr = 1;
+                }
+                for (int q = 0; q < nr; ++q) {
+                    const uint32_t js = off[col + ra[q]];
+                    const uint32_t je = off[col + rb[q] + 1];
+                    for (uint32_t j = js; j < je; ++j) {
+                        if (j == i) continue;
+                        pair64(pp, tab, p.x, p.y, p.z, spos[j], ax, ay, az, in_count);
+                    }
                 }
             }
         }
     }
+    if (!live) return;
     acc[perm[i]] = make_double4(ax * pp.G, ay * pp.G, az * pp.G, 0.0);
     if (COUNT) atomicAdd(counter, (unsigned long long)in_count);
 }
