@@ -4,11 +4,15 @@
 // Layout in HBM: real grids are M^3 x-major (index (x*M+y)*M+z), unpadded;
 // the complex grid is M x M x (M/2+1), z innermost.
 //
-//  deposit  one thread per mesh node gathers the particles of its eight
-//           neighbouring cells (sorted by cell, read through the CSR table)
-//           with the reference's CIC weights (pm_periodic.c:97-132).  Every
-//           node is written once, in a fixed order: no atomics.
-//  FFT      hand-written Stockham autosort FFT in LDS, radix 4/2/3/5:
+//  deposit  FP64: one thread per mesh node gathers the particles of its
+//           eight neighbouring cells (sorted by cell, read through the CSR
+//           table) with the reference's CIC weights (pm_periodic.c:97-132);
+//           every node is written once, in a fixed order: no atomics.
+//           FP32: per 16^3-cell Morton brick a node tile summed in 64-bit
+//           fixed point in LDS, brick faces added in a fixed order.
+//  FFT      hand-written Stockham autosort FFT in LDS, radix 4/2/3/5 (FP64,
+//           and FP32 meshes that are not powers of two; the FP32
+//           power-of-two meshes use the register FFT of fft32.hip):
 //           z rows: real length-M r2c through a length-M/2 complex FFT and the
 //                   split post-process; c2r the mirror image;
 //           y lines: strided batched complex FFT, lines contiguous in z so
@@ -16,12 +20,14 @@
 //           x lines: forward FFT, multiply by the Green's function
 //                   (pm_periodic.c:146-197; DC zeroed, :197), inverse FFT,
 //                   all in one LDS residency.
-//  interp   one thread per particle: the gradient stencil of pm_periodic.c:
-//           260-267 evaluated at the 8 CIC nodes and summed with the CIC
-//           weights in the reference order (pm_periodic.c:297-306).
+//  interp   the gradient stencil of pm_periodic.c:260-267 at the 8 CIC
+//           nodes, summed with the CIC weights (pm_periodic.c:297-306).
+//           FP64: one thread per particle, the reference order.  FP32: per
+//           16^3-cell brick, the potential staged in LDS.
 //
-// This file is compiled with -ffp-contract=off so the FP64 instantiations
-// keep the reference's rounding sequence for the weights and stencil.
+// This file is compiled with -ffp-contract=off so the FP64 kernels keep the
+// reference's rounding sequence for the weights and stencil (the FP32
+// interpolation fuses explicitly with fmaf).
 #include <cmath>
 
 #include "internal.h"
