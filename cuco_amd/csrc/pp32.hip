@@ -469,28 +469,53 @@ __global__ __launch_bounds__(kThreads, SYM ? 5 : 1) void k_pp32(
         uint32_t cursor = 0;
         unsigned long long ncand = 0;
 
+        // Cell number cursor + lane as digits (dx, dy, dz) of the mixed radix
+        // (nqx, nqy, nqz), advanced by the digits of 64 each round (no
+        // division per cell); wx0.. = the wrapped cell of digit 0.
+        uint32_t dz = 0, dy = 0, dx = 0, sz = 0, sy = 0, sx = 0;
+        uint32_t wx0 = 0, wy0 = 0, wz0 = 0;
+        if (total_q) {
+            const uint32_t nz = (uint32_t)nqz, ny = (uint32_t)nqy;
+            dz = (uint32_t)lane % nz;
+            dy = ((uint32_t)lane / nz) % ny;
+            dx = ((uint32_t)lane / nz) / ny;
+            sz = 64u % nz;
+            sy = (64u / nz) % ny;
+            sx = (64u / nz) / ny;
+            wx0 = (uint32_t)(((c0x + lqx) % M + M) % M);
+            wy0 = (uint32_t)(((c0y + lqy) % M + M) % M);
+            wz0 = (uint32_t)(((c0z + lqz) % M + M) % M);
+        }
+        // A digit is below nq <= 2M, so two conditional subtractions wrap.
+        const uint32_t Mu = (uint32_t)M;
+        auto wrap2 = [Mu](uint32_t u) {
+            u = u >= Mu ? u - Mu : u;
+            return u >= Mu ? u - Mu : u;
+        };
+        // SYM: a cell whose key precedes the tile's first particle's holds
+        // only particles before it (keys ascend in sorted order).
+        const uint32_t key_first = skeys[i0];
+
         while (cursor < total_q) {
             // Enumerate cells into the range list.
             uint32_t nr = 0;
             while (cursor < total_q && nr + 64 <= (uint32_t)kR) {
-                const uint32_t q = cursor + lane;
-                bool ok = q < total_q;
+                bool ok = cursor + lane < total_q;
                 uint32_t s = 0, c = 0, packed = 0;
                 if (ok) {
-                    const int qz = (int)(q % (uint32_t)nqz) + lqz;
-                    const uint32_t tq = q / (uint32_t)nqz;
-                    const int qy = (int)(tq % (uint32_t)nqy) + lqy;
-                    const int qx = (int)(tq / (uint32_t)nqy) + lqx;
+                    const int qz = (int)dz + lqz;
+                    const int qy = (int)dy + lqy;
+                    const int qx = (int)dx + lqx;
                     const float gx = fmaxf(0.f, fmaxf(qx * pp.cellf - bmaxx, bminx - (qx + 1) * pp.cellf));
                     const float gy = fmaxf(0.f, fmaxf(qy * pp.cellf - bmaxy, bminy - (qy + 1) * pp.cellf));
                     const float gz = fmaxf(0.f, fmaxf(qz * pp.cellf - bmaxz, bminz - (qz + 1) * pp.cellf));
                     ok = gx * gx + gy * gy + gz * gz <= pp.filter2;
+                    // FP32 keys are Morton keys.
+                    const uint32_t key = key_base + ((spread3(wrap2(wx0 + dx)) << 2) |
+                                                     (spread3(wrap2(wy0 + dy)) << 1) |
+                                                     spread3(wrap2(wz0 + dz)));
+                    if (SYM) ok = ok && key >= key_first;
                     if (ok) {
-                        int ux = (c0x + qx) % M, uy = (c0y + qy) % M, uz = (c0z + qz) % M;
-                        ux += ux < 0 ? M : 0;
-                        uy += uy < 0 ? M : 0;
-                        uz += uz < 0 ? M : 0;
-                        const uint32_t key = key_base + cell_key(g, ux, uy, uz);
                         uint32_t e1;
                         load_range(off, key, s, e1);
                         c = e1 - s;
@@ -504,6 +529,13 @@ __global__ __launch_bounds__(kThreads, SYM ? 5 : 1) void k_pp32(
                         packed = pack_q(qx, qy, qz);
                     }
                 }
+                dz += sz;
+                const uint32_t carry_z = dz >= (uint32_t)nqz ? 1u : 0u;
+                dz -= carry_z * (uint32_t)nqz;
+                dy += sy + carry_z;
+                const uint32_t carry_y = dy >= (uint32_t)nqy ? 1u : 0u;
+                dy -= carry_y * (uint32_t)nqy;
+                dx += sx + carry_y;
                 const uint64_t bal = __ballot(ok);
                 if (ok) {
                     const uint32_t pos = nr + (uint32_t)__popcll(bal & lt);
