@@ -560,18 +560,32 @@ __global__ __launch_bounds__(kThreads, SYM ? 5 : 1) void k_pp32(
             wave_sync();
             const uint32_t total_j = carry;
 
-            // Stream, filter and evaluate.
+            // Stream, filter and evaluate.  Candidate slots are taken 64 at a
+            // time; rc is the range holding the window's first slot f0.
+            // Ranges are non-empty, so the ranges rc+1.. start at strictly
+            // increasing slots after f0: lane L looks up where range rc+1+L
+            // starts, marks that slot of the window (a permute: each lane
+            // sends a 1 to the lane of its start slot; lanes without a
+            // start in the window send 0 to slot 0, which no range after rc
+            // can start at), and slot s then lies in range rc + (number of
+            // marks at or below s).
+            uint32_t rc = 0;
             for (uint32_t f0 = 0; f0 < total_j; f0 += 64) {
                 const uint32_t f = f0 + lane;
                 bool ok = f < total_j;
+                const uint32_t rn = rc + 1 + (uint32_t)lane;
+                const uint32_t st = rn < nr ? rp[rn] - f0 : 0xffffffffu;
+                const bool mark = st < 64u;
+                const int got = __builtin_amdgcn_ds_permute(mark ? (int)(st * 4u) : 0, mark ? 1 : 0);
+                const uint64_t starts = __ballot(got != 0);
+                const uint32_t lo =
+                    rc + __builtin_amdgcn_mbcnt_hi((uint32_t)(starts >> 32),
+                                                   __builtin_amdgcn_mbcnt_lo((uint32_t)starts, 0u)) +
+                    (got != 0 ? 1u : 0u);
+                rc = uni(rc + (uint32_t)__popcll(__ballot(st <= 64u)));
                 float4 qj = make_float4(0.f, 0.f, 0.f, 0.f);
                 uint32_t jsorted = 0;
                 if (ok) {
-                    uint32_t lo = 0, hi = nr;
-                    while (hi - lo > 1) {
-                        const uint32_t mid = (lo + hi) >> 1;
-                        if (rp[mid] <= f) lo = mid; else hi = mid;
-                    }
                     const uint32_t j = rs[lo] + (f - rp[lo]);
                     jsorted = j;
                     int qx, qy, qz;
