@@ -284,13 +284,6 @@ __device__ inline bool chunk_near(float4 q, bool in, float bminx, float bminy, f
     return __builtin_amdgcn_readfirstlane((uint32_t)(__ballot(near) != 0ull)) != 0u;
 }
 
-__device__ inline uint32_t wave_min_u32(uint32_t v)
-{
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, o, 64));
-    return v;
-}
-
 // Tiles of all boxes form one work list; tv.map holds each box's first
 // tile, p32s[b] the scaled constants of box b.
 // SYM: each pair once (section above); the tile's own sums go to iacc
@@ -414,12 +407,12 @@ __global__ __launch_bounds__(kThreads, SYM ? 5 : 1) void k_pp32(
             rotid[lane] = mid;
             rotid[lane + 64] = mid;
             wave_sync();
-            const uint32_t mn = uni(wave_min_u32((uint32_t)lane < (uint32_t)nj ? mid : 0xffffffffu));
             // Pairs inside the tile, lanes without a target, unused slots:
             // mask by index.
+            const bool inside = __ballot(lane < nj && mid < i0 + cnt) != 0ull;
             const bool near = chunk_near(mine, lane < nj, bminx, bminy, bminz, bmaxx, bmaxy,
                                          bmaxz, pp.h);
-            const bool masked = nj < 64 || cnt < 64u || mn < i0 + cnt;
+            const bool masked = nj < 64 || cnt < 64u || inside;
 #define CUCO_SYM(MK, NR, CL) \
     interact_sym<MK, NR, CL>(rot, rotid, pp, s_tab, xi, yi, zi, iid, sx_, sy_, sz_, jx, jy, jz)
             if (clamp) {
