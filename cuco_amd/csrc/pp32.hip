@@ -39,6 +39,7 @@ constexpr int kRanges = 128;   // cell ranges per wave before the j stream
 constexpr int kJBuf = 128;     // staged j particles per wave
 constexpr int kTab = 4096;     // LDS erfc table; zero from the cut bin on
 constexpr int kTabSym = 2048;  // its copy in the each-pair-once kernel
+constexpr float kCoarseExtent = 4.0f;  // tile extent (cells) for block enumeration
 
 __device__ inline float wave_min(float v)
 {
@@ -379,7 +380,24 @@ __global__ __launch_bounds__(kThreads, SYM ? 5 : 1) void k_pp32(
         // skipped.
         const bool sane = nqx > 0 && nqy > 0 && nqz > 0 && nqx <= 2 * M &&
                           nqy <= 2 * M && nqz <= 2 * M;
-        const uint32_t total_q = uni(sane ? (uint32_t)nqx * nqy * nqz : 0u);
+        // Enumeration unit: one cell, or for a wide tile (a merge of sparse
+        // blocks) an aligned 2x2x2-cell Morton block, whose cells form one
+        // contiguous key range: eight times fewer units to test, and the
+        // staging takes each candidate's cell from the low bits of its key.
+        // Blocks need an even mesh (they wrap onto blocks) and a reach of at
+        // most M cells per axis.  Only the each-pair-once variant uses them
+        // (it runs the halo workloads; the default variant keeps cells).
+        const float ext = fmaxf(bmaxx - bminx, fmaxf(bmaxy - bminy, bmaxz - bminz));
+        const bool coarse =
+            SYM && uni((uint32_t)(sane && (M & 1) == 0 && nqx <= M && nqy <= M && nqz <= M &&
+                                  ext >= kCoarseExtent * pp.cellf)) != 0u;
+        const int lv = coarse ? 1 : 0;
+        // Units (absolute, unwrapped) covering cells c0 + lq .. c0 + lq + nq - 1.
+        const int ux0 = (c0x + lqx) >> lv, uy0 = (c0y + lqy) >> lv, uz0 = (c0z + lqz) >> lv;
+        const int nux = ((c0x + lqx + nqx - 1) >> lv) - ux0 + 1;
+        const int nuy = ((c0y + lqy + nqy - 1) >> lv) - uy0 + 1;
+        const int nuz = ((c0z + lqz + nqz - 1) >> lv) - uz0 + 1;
+        const uint32_t total_q = uni(sane ? (uint32_t)nux * nuy * nuz : 0u);
 
         float ax = 0.f, ay = 0.f, az = 0.f, kx = 0.f, ky = 0.f, kz = 0.f;
         // Every candidate lies within sqrt(filter2) of the tile's box, so a
@@ -462,55 +480,73 @@ __global__ __launch_bounds__(kThreads, SYM ? 5 : 1) void k_pp32(
         uint32_t cursor = 0;
         unsigned long long ncand = 0;
 
-        // Cell number cursor + lane as digits (dx, dy, dz) of the mixed radix
-        // (nqx, nqy, nqz), advanced by the digits of 64 each round (no
-        // division per cell); wx0.. = the wrapped cell of digit 0.
+        // Unit number cursor + lane as digits (dx, dy, dz) of the mixed radix
+        // (nux, nuy, nuz), advanced by the digits of 64 each round (no
+        // division per unit); wx0.. = the wrapped unit of digit 0.
         uint32_t dz = 0, dy = 0, dx = 0, sz = 0, sy = 0, sx = 0;
         uint32_t wx0 = 0, wy0 = 0, wz0 = 0;
+        const int Mb = M >> lv;
         if (total_q) {
-            const uint32_t nz = (uint32_t)nqz, ny = (uint32_t)nqy;
+            const uint32_t nz = (uint32_t)nuz, ny = (uint32_t)nuy;
             dz = (uint32_t)lane % nz;
             dy = ((uint32_t)lane / nz) % ny;
             dx = ((uint32_t)lane / nz) / ny;
             sz = 64u % nz;
             sy = (64u / nz) % ny;
             sx = (64u / nz) / ny;
-            wx0 = (uint32_t)(((c0x + lqx) % M + M) % M);
-            wy0 = (uint32_t)(((c0y + lqy) % M + M) % M);
-            wz0 = (uint32_t)(((c0z + lqz) % M + M) % M);
+            wx0 = (uint32_t)((ux0 % Mb + Mb) % Mb);
+            wy0 = (uint32_t)((uy0 % Mb + Mb) % Mb);
+            wz0 = (uint32_t)((uz0 % Mb + Mb) % Mb);
         }
-        // A digit is below nq <= 2M, so two conditional subtractions wrap.
-        const uint32_t Mu = (uint32_t)M;
+        // A digit is below nu <= 2 Mb + 1 (cells: nq <= 2M; blocks: nq <= M),
+        // so two conditional subtractions wrap.
+        const uint32_t Mu = (uint32_t)Mb;
         auto wrap2 = [Mu](uint32_t u) {
             u = u >= Mu ? u - Mu : u;
             return u >= Mu ? u - Mu : u;
         };
-        // SYM: a cell whose key precedes the tile's first particle's holds
-        // only particles before it (keys ascend in sorted order).
+        // First cell of digit 0 relative to the anchor cell, and the unit's
+        // width: cells, and scaled length.
+        const int qbx = (ux0 << lv) - c0x, qby = (uy0 << lv) - c0y, qbz = (uz0 << lv) - c0z;
+        const int ub = 1 << lv;
+        const float uw = (float)ub * pp.cellf;
+        const uint32_t span = 1u << (3 * lv);  // keys per unit
+        // SYM: a unit whose keys all precede the tile's first particle's
+        // holds only particles before it (keys ascend in sorted order).
         const uint32_t key_first = skeys[i0];
 
         while (cursor < total_q) {
             // Enumerate cells into the range list.
             uint32_t nr = 0;
-            while (cursor < total_q && nr + 64 <= (uint32_t)kR) {
+            // Wide tiles keep filling the range list while a whole round
+            // fits (a round that does not fit is redone for the next batch):
+            // their rounds find few non-empty units each.
+            while (cursor < total_q && (coarse || nr + 64 <= (uint32_t)kR)) {
                 bool ok = cursor + lane < total_q;
                 uint32_t s = 0, c = 0, packed = 0;
                 if (ok) {
-                    const int qz = (int)dz + lqz;
-                    const int qy = (int)dy + lqy;
-                    const int qx = (int)dx + lqx;
-                    const float gx = fmaxf(0.f, fmaxf(qx * pp.cellf - bmaxx, bminx - (qx + 1) * pp.cellf));
-                    const float gy = fmaxf(0.f, fmaxf(qy * pp.cellf - bmaxy, bminy - (qy + 1) * pp.cellf));
-                    const float gz = fmaxf(0.f, fmaxf(qz * pp.cellf - bmaxz, bminz - (qz + 1) * pp.cellf));
+                    const int qz = (int)dz * ub + qbz;
+                    const int qy = (int)dy * ub + qby;
+                    const int qx = (int)dx * ub + qbx;
+                    const float lx = qx * pp.cellf, ly = qy * pp.cellf, lz = qz * pp.cellf;
+                    const float gx = fmaxf(0.f, fmaxf(lx - bmaxx, bminx - (lx + uw)));
+                    const float gy = fmaxf(0.f, fmaxf(ly - bmaxy, bminy - (ly + uw)));
+                    const float gz = fmaxf(0.f, fmaxf(lz - bmaxz, bminz - (lz + uw)));
                     ok = gx * gx + gy * gy + gz * gz <= pp.filter2;
-                    // FP32 keys are Morton keys.
-                    const uint32_t key = key_base + ((spread3(wrap2(wx0 + dx)) << 2) |
-                                                     (spread3(wrap2(wy0 + dy)) << 1) |
-                                                     spread3(wrap2(wz0 + dz)));
-                    if (SYM) ok = ok && key >= key_first;
+                    // FP32 keys are Morton keys; a block's key is its cells'
+                    // keys without their 3 low bits.
+                    const uint32_t key = key_base + (((spread3(wrap2(wx0 + dx)) << 2) |
+                                                      (spread3(wrap2(wy0 + dy)) << 1) |
+                                                      spread3(wrap2(wz0 + dz))) << (3 * lv));
+                    if (SYM) ok = ok && key + span > key_first;
                     if (ok) {
                         uint32_t e1;
-                        load_range(off, key, s, e1);
+                        if (coarse) {
+                            s = off[key];
+                            e1 = off[key + 8];
+                        } else {
+                            load_range(off, key, s, e1);
+                        }
                         c = e1 - s;
                         if (SYM) {
                             // only j after the tile's first particle
@@ -522,22 +558,24 @@ __global__ __launch_bounds__(kThreads, SYM ? 5 : 1) void k_pp32(
                         packed = pack_q(qx, qy, qz);
                     }
                 }
-                dz += sz;
-                const uint32_t carry_z = dz >= (uint32_t)nqz ? 1u : 0u;
-                dz -= carry_z * (uint32_t)nqz;
-                dy += sy + carry_z;
-                const uint32_t carry_y = dy >= (uint32_t)nqy ? 1u : 0u;
-                dy -= carry_y * (uint32_t)nqy;
-                dx += sx + carry_y;
                 const uint64_t bal = __ballot(ok);
+                const uint32_t nnew = uni((uint32_t)__popcll(bal));
+                if (nr + nnew > (uint32_t)kR) break;
                 if (ok) {
                     const uint32_t pos = nr + (uint32_t)__popcll(bal & lt);
                     rs[pos] = s;
                     rp[pos] = c;
                     rq[pos] = packed;
                 }
-                nr = uni(nr + (uint32_t)__popcll(bal));
+                nr += nnew;
                 cursor += 64;
+                dz += sz;
+                const uint32_t carry_z = dz >= (uint32_t)nuz ? 1u : 0u;
+                dz -= carry_z * (uint32_t)nuz;
+                dy += sy + carry_z;
+                const uint32_t carry_y = dy >= (uint32_t)nuy ? 1u : 0u;
+                dy -= carry_y * (uint32_t)nuy;
+                dx += sx + carry_y;
             }
             wave_sync();
             // Exclusive prefix of the range lengths.
@@ -583,6 +621,13 @@ __global__ __launch_bounds__(kThreads, SYM ? 5 : 1) void k_pp32(
                     jsorted = j;
                     int qx, qy, qz;
                     unpack_q(rq[lo], qx, qy, qz);
+                    if (coarse) {
+                        // the cell inside the 2x2x2 block: the key's low bits
+                        const uint32_t kj = skeys[j];
+                        qx += (int)((kj >> 2) & 1u);
+                        qy += (int)((kj >> 1) & 1u);
+                        qz += (int)(kj & 1u);
+                    }
                     const float4 pj = spos[j];
                     qj = make_float4(pj.x * pp.s + qx * pp.cellf,
                                      pj.y * pp.s + qy * pp.cellf,
