@@ -23,20 +23,35 @@ namespace {
 
 constexpr int kThreads = 256;
 
+// Minimum image.  WRAP: the reference's per-pair tests.  Otherwise the
+// caller passes the image shift of j's cell (sx, sy, sz: -L for a cell
+// index below 0, +L above M-1, else 0): when rcut plus one cell stays below
+// half the box (cell_images), a pair in an image cell is exactly one the
+// tests would wrap and no other pair is, and dx + (-L) is the same IEEE
+// operation as dx -= L (dx + 0.0 leaves dx unchanged), so the result is
+// bit-identical.
+template <bool WRAP>
 __device__ inline void pair64(const PairParams &pp, const double *__restrict__ tab,
                               double px, double py, double pz, double4 q,
+                              double sx, double sy, double sz,
                               double &ax, double &ay, double &az,
                               uint32_t &in_count)
 {
     double dx = q.x - px;
     double dy = q.y - py;
     double dz = q.z - pz;
-    if (dx > pp.boxhalf) dx -= pp.L;
-    if (dx < -pp.boxhalf) dx += pp.L;
-    if (dy > pp.boxhalf) dy -= pp.L;
-    if (dy < -pp.boxhalf) dy += pp.L;
-    if (dz > pp.boxhalf) dz -= pp.L;
-    if (dz < -pp.boxhalf) dz += pp.L;
+    if (WRAP) {
+        if (dx > pp.boxhalf) dx -= pp.L;
+        if (dx < -pp.boxhalf) dx += pp.L;
+        if (dy > pp.boxhalf) dy -= pp.L;
+        if (dy < -pp.boxhalf) dy += pp.L;
+        if (dz > pp.boxhalf) dz -= pp.L;
+        if (dz < -pp.boxhalf) dz += pp.L;
+    } else {
+        dx += sx;
+        dy += sy;
+        dz += sz;
+    }
     double r2 = dx * dx + dy * dy + dz * dz;
     if (r2 > pp.rcut2) return;
     ++in_count;
@@ -101,7 +116,13 @@ __device__ inline void z_window(float uz, float gxy, float rc2, int zl, int zr, 
     zb = min(zr, (int)floorf(uz + zext));
 }
 
-template <bool COUNT>
+// Image shift of a neighbour cell index c (unwrapped) on a mesh of M.
+__device__ inline double image_shift(int c, int M, double L)
+{
+    return c < 0 ? -L : (c >= M ? L : 0.0);
+}
+
+template <bool COUNT, bool WRAP>
 __global__ __launch_bounds__(kThreads) void k_pp64(
     const double4 *__restrict__ spos, const uint32_t *__restrict__ perm,
     const uint32_t *__restrict__ off, uint32_t i0, uint32_t n, PairParams pp,
@@ -162,7 +183,9 @@ __global__ __launch_bounds__(kThreads) void k_pp64(
             const float gx = gap_cells(ux, ix);
             const bool in_x = live && ix >= xl && ix <= xr && gx <= rc2;
             const int iix = wrap_index(ix, M);
+            const double sx = image_shift(ix, M, pp.L);
             for (int iy = YL; iy <= YR; ++iy) {
+                const double sy = image_shift(iy, M, pp.L);
                 const float gxy = gx + gap_cells(uy, iy);
                 int za = 1, zb = 0;
                 if (in_x && iy >= yl && iy <= yr && gxy <= rc2) z_window(uz, gxy, rc2, zl, zr, za, zb);
@@ -172,13 +195,16 @@ __global__ __launch_bounds__(kThreads) void k_pp64(
                 const uint32_t col = ((uint32_t)iix * M + wrap_index(iy, M)) * (uint32_t)M;
                 for (int z = ZA; z <= ZB; ++z) {
                     const bool in_z = z >= za && z <= zb;
+                    const double sz = image_shift(z, M, pp.L);
                     uint32_t js, je;
                     load_range(off, col + wrap_index(z, M), js, je);
                     js = __builtin_amdgcn_readfirstlane(js);
                     je = __builtin_amdgcn_readfirstlane(je);
                     for (uint32_t j = js; j < je; ++j) {
                         const double4 q = spos[j];
-                        if (in_z && j != i) pair64(pp, tab, p.x, p.y, p.z, q, ax, ay, az, in_count);
+                        if (in_z && j != i)
+                            pair64<WRAP>(pp, tab, p.x, p.y, p.z, q, sx, sy, sz, ax, ay, az,
+                                         in_count);
                     }
                 }
             }
@@ -188,10 +214,12 @@ __global__ __launch_bounds__(kThreads) void k_pp64(
             const float gx = gap_cells(ux, ix);
             if (gx > rc2) continue;
             const int iix = wrap_index(ix, M);
+            const double sx = image_shift(ix, M, pp.L);
             for (int iy = yl; iy <= yr; ++iy) {
                 const float gxy = gx + gap_cells(uy, iy);
                 if (gxy > rc2) continue;
                 const int iiy = wrap_index(iy, M);
+                const double sy = image_shift(iy, M, pp.L);
                 int za, zb;
                 z_window(uz, gxy, rc2, zl, zr, za, zb);
                 if (za > zb) continue;
@@ -199,21 +227,24 @@ __global__ __launch_bounds__(kThreads) void k_pp64(
                 // Up to two contiguous runs in reference order (wrapped part
                 // first when the range starts below 0).
                 int ra[2], rb[2], nr = 0;
+                double rs[2];  // image shift of each run
                 if (za < 0 && zb >= 0) {
-                    ra[0] = za + M; rb[0] = M - 1;
-                    ra[1] = 0; rb[1] = zb; nr = 2;
+                    ra[0] = za + M; rb[0] = M - 1; rs[0] = -pp.L;
+                    ra[1] = 0; rb[1] = zb; rs[1] = 0.0; nr = 2;
                 } else if (za < M && zb >= M) {
-                    ra[0] = za; rb[0] = M - 1;
-                    ra[1] = 0; rb[1] = zb - M; nr = 2;
+                    ra[0] = za; rb[0] = M - 1; rs[0] = 0.0;
+                    ra[1] = 0; rb[1] = zb - M; rs[1] = pp.L; nr = 2;
                 } else {
-                    ra[0] = wrap_index(za, M); rb[0] = wrap_index(zb, M); nr = 1;
+                    ra[0] = wrap_index(za, M); rb[0] = wrap_index(zb, M);
+                    rs[0] = image_shift(za, M, pp.L); nr = 1;
                 }
                 for (int q = 0; q < nr; ++q) {
                     const uint32_t js = off[col + ra[q]];
                     const uint32_t je = off[col + rb[q] + 1];
                     for (uint32_t j = js; j < je; ++j) {
                         if (j == i) continue;
-                        pair64(pp, tab, p.x, p.y, p.z, spos[j], ax, ay, az, in_count);
+                        pair64<WRAP>(pp, tab, p.x, p.y, p.z, spos[j], sx, sy, rs[q], ax, ay,
+                                     az, in_count);
                     }
                 }
             }
@@ -234,14 +265,21 @@ int launch_pp64(const double4 *spos, const uint32_t *perm, const uint32_t *off,
                 const int32_t *active_end, int32_t ti_current, hipStream_t s)
 {
     if (n == 0) return CUCO_OK;
-    if (pair_counter)
-        k_pp64<true><<<grid_for(n, kThreads), kThreads, 0, s>>>(
-            spos, perm, off, i0, n, pp, table, acc, pair_counter, active_end,
-            ti_current);
-    else
-        k_pp64<false><<<grid_for(n, kThreads), kThreads, 0, s>>>(
-            spos, perm, off, i0, n, pp, table, acc, nullptr, active_end,
-            ti_current);
+    // Cell-level image shifts are exact when a neighbour cell's pairs all
+    // lie within rcut + one cell of the target, below half the box by a
+    // margin far above rounding; otherwise the per-pair tests run.
+    const bool cell_images = pp.rcut + 1.001 * (pp.L / pp.M) < pp.boxhalf;
+#define CUCO_PP64(COUNT, WRAP, CTR)                                                      \
+    k_pp64<COUNT, WRAP><<<grid_for(n, kThreads), kThreads, 0, s>>>(                      \
+        spos, perm, off, i0, n, pp, table, acc, CTR, active_end, ti_current)
+    if (pair_counter) {
+        if (cell_images) CUCO_PP64(true, false, pair_counter);
+        else CUCO_PP64(true, true, pair_counter);
+    } else {
+        if (cell_images) CUCO_PP64(false, false, nullptr);
+        else CUCO_PP64(false, true, nullptr);
+    }
+#undef CUCO_PP64
     CUCO_LAUNCH_CHECK();
     return CUCO_OK;
 }

@@ -10,6 +10,7 @@ import pytest
 from cuco_amd import engine as E
 from cuco_amd import ics
 
+import oracle
 from helpers import halo_box, make_engine, oracle_params, rel_err
 
 pytestmark = pytest.mark.gpu
@@ -87,6 +88,27 @@ def test_unequal_masses_and_coincident_particles(orc):
     pos[2] = [0.0, 0.0, 0.0]  # on the box corner
     pos[3] = [np.nextafter(box.L, 0.0), 5.0, box.L * 0.5]
     check_fp64(box, pos, mass, orc)
+
+
+@pytest.mark.parametrize("rcut_cells", [7.0, 5.625])
+def test_rcut_up_to_half_box(orc, rcut_cells):
+    """rcut at the largest value the parameter check admits ((M/2 - 1)
+    cells): the kernel keeps the reference's per-pair image tests, since a
+    cell-level image shift needs rcut + one cell below half the box with a
+    margin; at the usual 4.5 asmth it takes the cell-level shifts.  Both
+    bit-identical to the oracle."""
+    box = ics.small_box("wide", 16, 16, 20.0, 21)
+    pos, vel, mass = ics.generate(box)
+    rcut = rcut_cells * box.L / box.M
+    eng = E.Engine(max_particles=box.N, pm_grid=box.M, box_size=box.L, G=ics.G_INTERNAL,
+                   asmth=box.asmth, rcut=rcut, soft_h=box.soft_h)
+    eng.upload(pos, None, mass)
+    eng.gravity_linklist(E.FP64)
+    app = eng.accelerations(pm=False)
+    p = oracle.params(box.M, box.L, ics.G_INTERNAL, box.asmth, rcut, box.soft_h)
+    ref, npairs = orc.pp_force(p, pos, mass)
+    assert npairs > 0
+    assert np.array_equal(app, ref)
 
 
 def test_ragged_particle_counts(orc):
