@@ -97,6 +97,27 @@ def test_error_percentiles_other_meshes(orc, M):
         assert errs[k]["p99"] <= P99_BAR, (k, errs)
 
 
+@pytest.mark.parametrize("M", [40, 48, 96])
+def test_pair_variants_other_meshes(M):
+    """Pair force alone for both variants on meshes whose Morton grid is
+    larger than the mesh (the 2x2x2-cell blocks of the each-pair-once
+    variant wrap on M/2 blocks).  Sparse boxes, so most tiles are wide and
+    take blocks."""
+    box = ics.small_box(f"mesh{M}", 24, M, 60.0, 50 + M, "halos", n_halos=8)
+    pos, vel, mass = ics.generate(box)
+    eng = make_engine(box)
+    eng.upload(pos, None, mass)
+    eng.compute_accelerations(E.FP64, True)
+    ref_pp, ref_pm = eng.accelerations()
+    for sym in (0, 1):
+        eng.set_tuning(3, 8, 0, sym)
+        eng.compute_accelerations(E.FP32, True)
+        err = rel_err(eng.accelerations()[0], ref_pp, ref_pp + ref_pm)
+        print(M, sym, np.median(err), np.percentile(err, 99), err.max())
+        assert np.percentile(err, 99) <= 2e-5, (M, sym)
+        assert np.median(err) <= 1e-6, (M, sym)
+
+
 def test_fp32_is_deterministic_and_counts_cover_pairs():
     box = halo_box(seed=4)
     pos, vel, mass = ics.generate(box)
