@@ -492,6 +492,19 @@ int mode_of(int precision) { return precision == CUCO_FP64 ? kKeyXMajor : kKeyMo
 int do_pp(cuco_engine *e, int precision, unsigned long long *counter,
           const int32_t *active_end = nullptr, int32_t ti_current = 0)
 {
+    if (precision != CUCO_FP64) {
+        // The FP32 pair kernel packs a candidate cell's offset from its
+        // tile's anchor cell into 10 bits per axis (|q| <= 511): rcut in
+        // cells plus a Morton super-block (<= 32 cells) plus margin.
+        for (const HostBox &h : e->box) {
+            const double reach = h.pp.rcut * h.pp.fac + 36.0;
+            if (reach > 511.0) {
+                set_error("FP32 pair force: rcut of %.1f cells exceeds the FP32 kernel's "
+                          "reach of 475 cells (use CUCO_FP64)", h.pp.rcut * h.pp.fac);
+                return CUCO_ERR_UNSUPPORTED;
+            }
+        }
+    }
     TRY(sort_particles(e, mode_of(precision)));
     TRY(stage_start(e, CUCO_STAGE_PP));
     if (precision == CUCO_FP64) {

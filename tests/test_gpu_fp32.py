@@ -118,6 +118,23 @@ def test_pair_variants_other_meshes(M):
         assert np.median(err) <= 1e-6, (M, sym)
 
 
+def test_fp32_refuses_rcut_beyond_its_reach():
+    """The FP32 pair kernel addresses cells up to 475 cells from a tile; the
+    parameter check admits rcut up to M/2 - 1 cells, so a larger rcut on a
+    fine mesh is refused with a status (nothing is computed)."""
+    M, L = 960, 1000.0
+    cell = L / M
+    rng = np.random.default_rng(3)
+    pos = rng.uniform(0.0, L, (2000, 3))
+    eng = E.Engine(max_particles=2000, pm_grid=M, box_size=L, G=ics.G_INTERNAL,
+                   asmth=1.25 * cell, rcut=477.0 * cell, soft_h=0.1 * cell)
+    eng.upload(pos, None, np.ones(2000))
+    with pytest.raises(E.CucoError) as ei:
+        eng.gravity_linklist(E.FP32)
+    assert ei.value.status == 5
+    assert "reach" in str(ei.value)
+
+
 def test_fp32_is_deterministic_and_counts_cover_pairs():
     box = halo_box(seed=4)
     pos, vel, mass = ics.generate(box)
