@@ -33,7 +33,9 @@ typedef enum cuco_status {
 
 typedef enum cuco_precision {
     CUCO_FP64 = 0, /* plain path: FP64 everywhere, reference cell order */
-    CUCO_FP32 = 1  /* chosen path: FP32 cell-relative pairs, FP32 mesh */
+    CUCO_FP32 = 1  /* chosen path: FP32 cell-relative pairs, FP32 mesh;
+                      its pair force takes rcut up to 475 mesh cells
+                      (CUCO_ERR_UNSUPPORTED beyond) */
 } cuco_precision;
 
 /* Split and mesh parameters (allvars.h:9,24-25; pm_periodic.c:27-28;
