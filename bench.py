@@ -41,10 +41,13 @@ def percentiles(err):
             "max": float(err.max())}
 
 
-def cpu_baseline(box, pos, mass, n_targets, seed=5006):
+def cpu_baseline(box, pos, mass, n_targets, seed=5006, gpu_fp64=None):
     """FP64 oracle on the host: PP on a random target sample (the cell list
     over all particles is built as in a full evaluation) plus the full PM;
-    returns the extrapolated whole-box force evals/s."""
+    returns the extrapolated whole-box force evals/s.  With gpu_fp64 = the
+    GPU FP64 path's (a_pp, a_pm), the oracle's results also check it: PP on
+    the sampled targets bit for bit, PM over all particles relative to the
+    GPU FP64 |a_total| (whose PP part is the one checked bit for bit)."""
     import oracle
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
     threads = min(threads, os.cpu_count() or threads)
@@ -53,12 +56,25 @@ def cpu_baseline(box, pos, mass, n_targets, seed=5006):
                       box.soft_h, nthreads=threads)
     rng = np.random.default_rng(seed)
     targets = np.sort(rng.choice(box.N, n_targets, replace=False))
-    oracle.pp_force(p, pos, mass, targets)
+    orc_pp, _ = oracle.pp_force(p, pos, mass, targets)
     t_setup, t_sample = oracle.pp_last_timing()
     t_pp = t_setup + t_sample * box.N / n_targets
     t0 = time.perf_counter()
-    oracle.pm_force(p, pos, mass)
+    orc_pm = oracle.pm_force(p, pos, mass)
     t_pm = time.perf_counter() - t0
+    parity = None
+    if gpu_fp64 is not None:
+        g_pp, g_pm = gpu_fp64
+        den = np.linalg.norm(g_pp + g_pm, axis=1)
+        den = np.where(den > 0, den, 1.0)
+        parity = {
+            "targets": int(n_targets),
+            "pp_bit_identical": bool(np.array_equal(g_pp[targets], orc_pp)),
+            "pp_max_rel": float((np.linalg.norm(g_pp[targets] - orc_pp, axis=1) /
+                                 den[targets]).max()),
+            "pm_max_rel": float((np.linalg.norm(g_pm - orc_pm, axis=1) / den).max()),
+        }
+        del den
     value = box.N / (t_pp + t_pm)
     # The faithful single-threaded structure: the same PP loop on one core.
     p1 = oracle.params(box.M, box.L, ics.G_INTERNAL, box.asmth, box.rcut,
@@ -66,7 +82,7 @@ def cpu_baseline(box, pos, mass, n_targets, seed=5006):
     n1 = max(1000, n_targets // 20)
     oracle.pp_force(p1, pos, mass, targets[:n1])
     _, t_sample1 = oracle.pp_last_timing()
-    return {
+    return parity, {
         "value": value,
         "unit": "particle force evals/s",
         "cores": threads,
@@ -370,8 +386,10 @@ def main() -> int:
                                          args.ensemble_steps)
     if not args.no_cpu_baseline:
         t0 = time.perf_counter()
-        out["cpu_baseline"] = cpu_baseline(box, pos, mass, args.cpu_targets)
-        log(f"[bench] cpu baseline in {time.perf_counter() - t0:.1f}s")
+        parity, out["cpu_baseline"] = cpu_baseline(box, pos, mass, args.cpu_targets,
+                                                   gpu_fp64=(ref_pp, ref_pm))
+        out["fp64_vs_oracle"] = parity
+        log(f"[bench] cpu baseline in {time.perf_counter() - t0:.1f}s; FP64 vs oracle {parity}")
     print(json.dumps(out), flush=True)
     return 0
 
