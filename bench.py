@@ -166,6 +166,50 @@ def ensemble_bench(E, ics, nboxes, steps):
     }
 
 
+def timeline_bench(E, ics, box, pos, vel, mass, steps):
+    """Individual power-of-two timesteps (run.c / timestep.c) on the
+    workload, FP32: `steps` passes of run()'s body from a_start, each a
+    drift to the next sync point, the active set's forces (PM on PM steps)
+    and its kicks and new timesteps.  Reports the per-step wall time and
+    the active particles per second."""
+    eng = E.Engine(max_particles=box.N, **box.engine_kwargs())
+    eng.upload(pos, vel, mass)
+    tl = dict(time_begin=box.a_start, time_max=box.a_end, omega0=ics.OMEGA_M,
+              omega_lambda=ics.OMEGA_L, hubble=ics.HUBBLE_INTERNAL,
+              err_tol_int_accuracy=0.025,
+              max_size_timestep=math.log(box.a_end / box.a_start) / 4,
+              min_size_timestep=0.0, softening_halo=box.softening,
+              softening_halo_max_phys=box.softening)
+    eng.timeline_init(**tl)
+    eng.timeline_run(E.FP32, max_steps=1)   # the first step: every particle
+    eng.synchronize()
+    active, done = 0, 0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        st = eng.timeline_run(E.FP32, max_steps=1)
+        active += st["active"]
+        done += 1
+        if st["finished"]:
+            break
+    eng.synchronize()
+    t = time.perf_counter() - t0
+    b, e = eng.timesteps()
+    levels = np.unique(e - b)
+    eng.close()
+    log(f"[bench] timeline: {done} steps, {active / max(done, 1) / box.N:.3f} of the "
+        f"particles active per step, {t / max(done, 1) * 1e3:.2f} ms/step")
+    return {
+        "workload": (f"{box.name}, individual power-of-two timesteps from a = {box.a_start} "
+                     f"(ErrTolIntAccuracy 0.025, MaxSizeTimestep ln(a_end/a_start)/4), "
+                     f"FP32; {done} steps after the first (all-active) one"),
+        "steps": done,
+        "ms_per_step": t / max(done, 1) * 1e3,
+        "active_per_step": active / max(done, 1),
+        "active_evals_per_s": active / t,
+        "timestep_levels": [int(x) for x in levels],
+    }
+
+
 def pmc_traffic(kernel: str, workload: str, key: str = "traffic_bytes"):
     """HBM bytes per dispatch of `kernel` on `workload` from the committed
     rocprofv3 PMC summary (profiles/<kernel>_<workload>_pmc.md, the line
@@ -194,6 +238,8 @@ def main() -> int:
     ap.add_argument("--ensemble-boxes", type=int, default=64,
                     help="boxes of the batched-vs-looped ensemble run (0: skip)")
     ap.add_argument("--ensemble-steps", type=int, default=20)
+    ap.add_argument("--timeline-steps", type=int, default=12,
+                    help="individual-timestep steps timed after the first (0: skip)")
     ap.add_argument("--pm-overlap", choices=["auto", "0", "1"], default="auto",
                     help="PM chain on a second stream beside the pair force: "
                          "the autotuner's choice, or forced off / on")
@@ -381,6 +427,8 @@ def main() -> int:
         "cpu_baseline": None,
     }
     eng.close()
+    if args.timeline_steps > 0:
+        out["timeline"] = timeline_bench(E, ics, box, pos, vel, mass, args.timeline_steps)
     if args.ensemble_boxes > 0:
         out["ensemble"] = ensemble_bench(E, ics, args.ensemble_boxes,
                                          args.ensemble_steps)

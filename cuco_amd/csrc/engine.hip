@@ -125,6 +125,7 @@ struct cuco_engine {
     float *table32 = nullptr;  // 1024 entries for the FP32 kernel
     float htable[kNTab];
     TileInfo tiles;
+    ActiveTiles atiles;  // FP32 active-set tiles of the integer timeline
     PairSym psym;  // each-pair-once variant of the FP32 pair force
     int pp32_workgroups = 0;
     int pp32_wg_per_cu = 8;
@@ -516,10 +517,12 @@ int do_pp(cuco_engine *e, int precision, unsigned long long *counter,
         }
     } else {
         e->psym.n_sorted = e->n;
+        e->atiles.t.block_shift = e->tiles.block_shift;
         TRY(launch_pp32(e->spos32, e->skeys, e->perm, e->off, e->bs, e->p32,
                         e->table_cut, e->uniform_mass, e->table32, e->tiles,
                         e->acc_pp32, counter, e->pp32_workgroups, active_end,
-                        ti_current, e->psym, e->stream));
+                        ti_current, e->psym, e->atiles.scan ? &e->atiles : nullptr,
+                        e->n, e->stream));
     }
     TRY(stage_stop(e, CUCO_STAGE_PP));
     e->pp_prec = precision;
@@ -972,6 +975,10 @@ int cuco_destroy(cuco_engine *e)
     free_ptr(e->tiles.tile_start); free_ptr(e->tiles.tile_count);
     free_ptr(e->tiles.block_tiles); free_ptr(e->tiles.counters);
     free_ptr(e->tiles.scan_scratch);
+    free_ptr(e->atiles.scan); free_ptr(e->atiles.nscratch); free_ptr(e->atiles.list);
+    free_ptr(e->atiles.tile_map); free_ptr(e->atiles.t.tile_start);
+    free_ptr(e->atiles.t.tile_count); free_ptr(e->atiles.t.block_tiles);
+    free_ptr(e->atiles.t.counters); free_ptr(e->atiles.t.scan_scratch);
     for (int k = 0; k < 2; ++k) {
         free_ptr(e->rho[k]); free_ptr(e->phi[k]); free_ptr(e->cgrid[k]);
         free_ptr(e->dep_scratch[k]);
@@ -1457,6 +1464,23 @@ int cuco_timeline_init(cuco_engine *e, const cuco_timeline *t)
         TRY(dev_alloc(e->ti_scalars, sizeof(int32_t) * 4));
         TRY(dev_alloc(e->kick_table, sizeof(double) * kDriftTable));
     }
+    if (!e->atiles.scan) {
+        // Tiles of the active targets (FP32 active-set pair force).
+        ActiveTiles &a = e->atiles;
+        a.capacity = e->capacity;
+        a.nkeys = e->nkeys[kKeyMorton];
+        a.t.nblocks = e->tiles.nblocks;
+        a.t.max_tiles = e->tiles.max_tiles;
+        TRY(dev_alloc(a.scan, sizeof(uint32_t) * ((size_t)e->capacity + 1)));
+        TRY(dev_alloc(a.nscratch, sizeof(uint32_t) * scan_scratch_words(e->capacity)));
+        TRY(dev_alloc(a.list, sizeof(uint32_t) * (size_t)e->capacity));
+        TRY(dev_alloc(a.tile_map, sizeof(uint32_t) * (e->box.size() + 1)));
+        TRY(dev_alloc(a.t.tile_start, sizeof(uint32_t) * a.t.max_tiles));
+        TRY(dev_alloc(a.t.tile_count, sizeof(uint32_t) * a.t.max_tiles));
+        TRY(dev_alloc(a.t.block_tiles, sizeof(uint32_t) * a.t.nblocks));
+        TRY(dev_alloc(a.t.counters, sizeof(uint32_t) * 4));
+        TRY(dev_alloc(a.t.scan_scratch, sizeof(uint32_t) * scan_scratch_words(a.t.nblocks)));
+    }
     e->tl = *t;
     e->drift_tab.assign(kDriftTable, 0.0);
     e->kick_tab.assign(kDriftTable, 0.0);
@@ -1848,7 +1872,7 @@ int cuco_autotune(cuco_engine *e, cuco_tuning *chosen, double *best_ms)
                 if (!rc) rc = launch_pp32(e->spos32, e->skeys, e->perm, e->off, e->bs, e->p32,
                                           e->table_cut, e->uniform_mass, e->table32, e->tiles,
                                           e->acc_pp32, nullptr, e->pp32_workgroups,
-                                          nullptr, 0, e->psym, e->stream);
+                                          nullptr, 0, e->psym, nullptr, e->n, e->stream);
                 if (!rc && (hipEventRecord(b, e->stream) != hipSuccess ||
                             hipEventSynchronize(b) != hipSuccess)) rc = CUCO_ERR_HIP;
                 float t_ms = 0;

@@ -133,13 +133,29 @@ struct PairSym {
     unsigned long long *jacc = nullptr;
     uint32_t n_sorted = 0;
 };
+// Tiles of the active targets of an integer-timeline step (FP32 default
+// variant): the active sorted indices in sorted order (list), cut into runs
+// of at most 64 inside each Morton super-block; t.tile_start indexes list.
+struct ActiveTiles {
+    TileInfo t;
+    uint32_t *scan = nullptr;      // n+1: active particles before each sorted index
+    uint32_t *nscratch = nullptr;  // scan scratch for n entries
+    uint32_t *list = nullptr;      // n: active sorted indices
+    uint32_t *tile_map = nullptr;  // nboxes+1: first active tile per box
+    uint32_t capacity = 0;         // particles the buffers hold
+    uint32_t nkeys = 0;            // Morton keys of all boxes
+};
+// With active_end and at, only the active targets are evaluated, packed into
+// their own tiles (at); without at, tiles holding an active target are
+// evaluated whole and only active lanes store.
 int launch_pp32(const float4 *spos, const uint32_t *skeys,
                 const uint32_t *perm, const uint32_t *off, const BoxSet &bs,
                 const P32 *p32s, bool table_cut, double uniform_mass,
                 const float *table32, const TileInfo &t, float4 *acc,
                 unsigned long long *cand_counter, int num_workgroups,
                 const int32_t *active_end, int32_t ti_current,
-                const PairSym &sym, hipStream_t s);
+                const PairSym &sym, const ActiveTiles *at, uint32_t n_sorted,
+                hipStream_t s);
 
 // ------------------------------------------------------------ PM
 struct PmTables {

@@ -301,7 +301,7 @@ __global__ __launch_bounds__(kThreads, SYM ? 5 : 1) void k_pp32(
     unsigned long long *__restrict__ cand,
     const int32_t *__restrict__ active_end, int32_t ti_current,
     float4 *__restrict__ iacc, unsigned long long *__restrict__ jacc,
-    uint32_t n_sorted)
+    uint32_t n_sorted, const uint32_t *__restrict__ tlist)
 {
     // SYM trades half of the range list for the doubled rotation buffer.
     constexpr int kR = SYM ? kRanges / 2 : kRanges;
@@ -347,7 +347,11 @@ __global__ __launch_bounds__(kThreads, SYM ? 5 : 1) void k_pp32(
         const uint32_t i0 = tstart[t];
         const uint32_t cnt = tcount[t];
         const bool valid = (uint32_t)lane < cnt;
-        const uint32_t i = i0 + (valid ? (uint32_t)lane : 0u);
+        // tlist: the tile is a run of the active-target list (ia: its first
+        // target, the anchor); else a run of the sorted particles.
+        const uint32_t ia = tlist ? tlist[i0] : i0;
+        const uint32_t i = tlist ? tlist[i0 + (valid ? (uint32_t)lane : 0u)]
+                                 : i0 + (valid ? (uint32_t)lane : 0u);
         // Active set: a tile without an active target is skipped; in the
         // others every lane computes and only active lanes store.
         bool act = valid;
@@ -355,7 +359,7 @@ __global__ __launch_bounds__(kThreads, SYM ? 5 : 1) void k_pp32(
         if (uni((uint32_t)(__ballot(act) != 0ull)) == 0u) continue;
 
         int c0x, c0y, c0z, cx, cy, cz;
-        key_cell(g, skeys[i0] - key_base, c0x, c0y, c0z);
+        key_cell(g, skeys[ia] - key_base, c0x, c0y, c0z);
         key_cell(g, skeys[i] - key_base, cx, cy, cz);
         const float4 pi = spos[i];
         const float xi = pi.x * pp.s + (float)unwrap_delta(cx - c0x, M) * pp.cellf;
@@ -513,7 +517,7 @@ __global__ __launch_bounds__(kThreads, SYM ? 5 : 1) void k_pp32(
         const uint32_t span = 1u << (3 * lv);  // keys per unit
         // SYM: a unit whose keys all precede the tile's first particle's
         // holds only particles before it (keys ascend in sorted order).
-        const uint32_t key_first = skeys[i0];
+        const uint32_t key_first = skeys[ia];
 
         while (cursor < total_q) {
             // Enumerate cells into the range list.
@@ -767,6 +771,61 @@ __global__ void k_box_tiles(const uint32_t *__restrict__ tile_off,
     if (b == nboxes) tile_map[b] = counters[0];
 }
 
+// Active targets of an integer-timeline step, packed into their own tiles.
+// flag: 1 where the particle at sorted index s is active; scanned in place
+// (exclusive, scan[n] = total), scan[s+1] - scan[s] is that flag again.
+__global__ void k_act_flags(const uint32_t *__restrict__ perm,
+                            const int32_t *__restrict__ active_end, int32_t ti_current,
+                            uint32_t n, uint32_t *__restrict__ flag)
+{
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < n) flag[s] = active_end[perm[s]] == ti_current ? 1u : 0u;
+}
+
+__global__ void k_act_list(const uint32_t *__restrict__ scan, uint32_t n,
+                           uint32_t *__restrict__ list)
+{
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < n && scan[s + 1] > scan[s]) list[scan[s]] = s;
+}
+
+// Active targets of super-block sb: list positions [a0, a0 + count).
+__device__ inline uint32_t super_active(const uint32_t *__restrict__ off,
+                                        const uint32_t *__restrict__ scan, uint32_t sb,
+                                        int shift3, uint32_t &a0)
+{
+    const size_t base = (size_t)sb << (shift3 + 3);
+    a0 = scan[off[base]];
+    return scan[off[base + ((size_t)1 << (shift3 + 3))]] - a0;
+}
+
+__global__ void k_block_tiles_act(const uint32_t *__restrict__ off,
+                                  const uint32_t *__restrict__ scan, uint32_t nsuper,
+                                  int shift3, uint32_t *__restrict__ super_count)
+{
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nsuper) return;
+    uint32_t a0;
+    super_count[b] = (super_active(off, scan, b, shift3, a0) + 63u) / 64u;
+}
+
+__global__ void k_fill_tiles_act(const uint32_t *__restrict__ off,
+                                 const uint32_t *__restrict__ scan,
+                                 const uint32_t *__restrict__ tile_off, uint32_t nsuper,
+                                 int shift3, uint32_t *__restrict__ tstart,
+                                 uint32_t *__restrict__ tcount)
+{
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nsuper) return;
+    uint32_t a0;
+    const uint32_t c = super_active(off, scan, b, shift3, a0);
+    uint32_t t = tile_off[b];
+    for (uint32_t p = 0; p < c; p += 64u, ++t) {
+        tstart[t] = a0 + p;
+        tcount[t] = min(64u, c - p);
+    }
+}
+
 // Each-pair-once result: own sums plus the fixed-point j sums, scaled to
 // accelerations and scattered to original order.
 __global__ void k_pp32_combine(const float4 *__restrict__ iacc,
@@ -789,6 +848,42 @@ __global__ void k_pp32_combine(const float4 *__restrict__ iacc,
     const float y = a.y + (float)((double)(long long)jacc[(size_t)n_sorted + i] * inv);
     const float z = a.z + (float)((double)(long long)jacc[2 * (size_t)n_sorted + i] * inv);
     acc[perm[i]] = make_float4(x * sc, y * sc, z * sc, 0.f);
+}
+
+}  // namespace
+
+namespace {
+
+int build_active_tiles(const uint32_t *off, const uint32_t *perm, const int32_t *active_end,
+                       int32_t ti_current, uint32_t n, const BoxSet &bs,
+                       const ActiveTiles &at, hipStream_t s)
+{
+    const int shift3 = 3 * at.t.block_shift;
+    const uint32_t nsuper = at.nkeys >> (shift3 + 3);
+    if (nsuper == 0 || nsuper > at.t.nblocks || n > at.capacity) {
+        set_error("active tiles: %u super-blocks / %u particles exceed capacity", nsuper, n);
+        return CUCO_ERR_INVALID_ARG;
+    }
+    k_act_flags<<<grid_for(n, 256), 256, 0, s>>>(perm, active_end, ti_current, n, at.scan);
+    CUCO_LAUNCH_CHECK();
+    int rc = exclusive_scan_u32(at.scan, at.scan, n, at.nscratch, at.scan + n, s);
+    if (rc) return rc;
+    k_act_list<<<grid_for(n, 256), 256, 0, s>>>(at.scan, n, at.list);
+    CUCO_LAUNCH_CHECK();
+    k_block_tiles_act<<<grid_for(nsuper, 256), 256, 0, s>>>(off, at.scan, nsuper, shift3,
+                                                            at.t.block_tiles);
+    CUCO_LAUNCH_CHECK();
+    rc = exclusive_scan_u32(at.t.block_tiles, at.t.block_tiles, nsuper, at.t.scan_scratch,
+                            at.t.counters, s);
+    if (rc) return rc;
+    k_fill_tiles_act<<<grid_for(nsuper, 256), 256, 0, s>>>(off, at.scan, at.t.block_tiles,
+                                                           nsuper, shift3, at.t.tile_start,
+                                                           at.t.tile_count);
+    CUCO_LAUNCH_CHECK();
+    k_box_tiles<<<grid_for(bs.nboxes + 1, 64), 64, 0, s>>>(
+        at.t.block_tiles, bs.dev[kKeyMorton], bs.nboxes, shift3, at.t.counters, at.tile_map);
+    CUCO_LAUNCH_CHECK();
+    return CUCO_OK;
 }
 
 }  // namespace
@@ -862,7 +957,8 @@ int launch_pp32(const float4 *spos, const uint32_t *skeys,
                 const float *table32, const TileInfo &t, float4 *acc,
                 unsigned long long *cand_counter, int num_workgroups,
                 const int32_t *active_end, int32_t ti_current,
-                const PairSym &sym, hipStream_t s)
+                const PairSym &sym, const ActiveTiles *at, uint32_t n_sorted,
+                hipStream_t s)
 {
     if (bs.total[kMapPart] == 0) return CUCO_OK;
     const bool generic = !table_cut;
@@ -881,17 +977,29 @@ int launch_pp32(const float4 *spos, const uint32_t *skeys,
         k_pp32<false, true, true><<<nwg, kThreads, 0, s>>>(
             spos, skeys, perm, off, tv, p32s, mass, table32, t.tile_start,
             t.tile_count, t.counters, acc, cand_counter, nullptr, 0, sym.iacc,
-            sym.jacc, sym.n_sorted);
+            sym.jacc, sym.n_sorted, nullptr);
         CUCO_LAUNCH_CHECK();
         k_pp32_combine<<<xcd_grid(bs.total[kMapPart]), 256, 0, s>>>(
             sym.iacc, sym.jacc, sym.n_sorted, perm, bs.view(kKeyMorton, kMapPart), p32s, acc);
         CUCO_LAUNCH_CHECK();
         return CUCO_OK;
     }
-#define CUCO_PP32(GENERIC, UNIFORM)                                                   \
-    k_pp32<GENERIC, UNIFORM, false><<<nwg, kThreads, 0, s>>>(                         \
-        spos, skeys, perm, off, tv, p32s, mass, table32, t.tile_start, t.tile_count,  \
-        t.counters, acc, cand_counter, active_end, ti_current, nullptr, nullptr, 0u)
+    // Active set with its own tiles: every listed target is active.
+    const bool packed = active_end && at;
+    const TileInfo &tt = packed ? at->t : t;
+    const BoxView tva{bs.dev[kKeyMorton], packed ? at->tile_map : bs.tile_map, bs.nboxes};
+    const int32_t *ae = packed ? nullptr : active_end;
+    const uint32_t *tl = packed ? at->list : nullptr;
+    if (packed) {
+        rc = build_active_tiles(off, perm, active_end, ti_current, n_sorted, bs, *at, s);
+        if (rc) return rc;
+        rc = launch_fill_u32(at->t.counters + 1, 0u, 1, s);
+        if (rc) return rc;
+    }
+#define CUCO_PP32(GENERIC, UNIFORM)                                                     \
+    k_pp32<GENERIC, UNIFORM, false><<<nwg, kThreads, 0, s>>>(                           \
+        spos, skeys, perm, off, tva, p32s, mass, table32, tt.tile_start, tt.tile_count, \
+        tt.counters, acc, cand_counter, ae, ti_current, nullptr, nullptr, 0u, tl)
     if (generic) {
         if (uniform) CUCO_PP32(true, true); else CUCO_PP32(true, false);
     } else {

@@ -85,7 +85,13 @@ def test_active_set_force_equals_full_evaluation(orc):
         full.set_softening(eng_soft(tl, st["time"]))
         full.gravity_linklist(precision)
         fa = full.accelerations(pp=True, pm=False)
-        assert np.array_equal(app[active], fa[active]), precision
+        if precision == E.FP64:
+            assert np.array_equal(app[active], fa[active])
+        else:
+            # FP32: the active targets run in tiles of their own, so their
+            # candidates are chunked differently: equal to FP32 rounding.
+            scale = np.linalg.norm(fa, axis=1).max()
+            assert np.abs(app[active] - fa[active]).max() <= 1e-5 * scale
 
 
 def eng_soft(tl, a):
