@@ -80,6 +80,7 @@ struct HostBox {
     GridInfo g[2]{};   // per KeyMode
     double pm_fac = 0;
     double mass_total = 0;  // sum of the box's particle masses
+    double mass_max = 0;    // largest particle mass of the box
 };
 
 struct cuco_engine {
@@ -293,7 +294,7 @@ int update_pair_constants(cuco_engine *e)
         const double edge = b.pp.asmthfac * b.pp.rcut;
         same_edge = same_edge && fabs(edge - edge0) <= 1e-9 * edge0;
         cut = cut && pp32_table_cut(b.pp);
-        h.push_back(make_p32(b.pp, b.g[kKeyMorton], e->uniform_mass, b.cap));
+        h.push_back(make_p32(b.pp, b.g[kKeyMorton], e->uniform_mass, b.mass_max, b.cap));
     }
     const bool aligned = fabs(edge0 - nearbyint(edge0)) < 1e-9 * edge0;
     e->table_cut = cut && same_edge;
@@ -1067,11 +1068,12 @@ int cuco_upload_particles(cuco_engine *e, int64_t n, const double *pos,
     }
     std::vector<double4> hp((size_t)n), hv((size_t)n);
     double common = mass ? mass[0] : uniform_mass;
+    bool mass_max_changed = false;
     int64_t i = 0;
     for (size_t b = 0; b < e->box.size(); ++b) {
         const double L = e->box[b].prm.box_size;
         const int64_t nb = e->ensemble ? (int64_t)e->box[b].cap : n;
-        double mtot = 0;
+        double mtot = 0, mmax = 0;
         for (int64_t k = 0; k < nb; ++k, ++i) {
             double x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
             if (!std::isfinite(x) || !std::isfinite(y) || !std::isfinite(z)) {
@@ -1084,10 +1086,13 @@ int cuco_upload_particles(cuco_engine *e, int64_t n, const double *pos,
             hp[i] = make_double4(x, y, z, mass ? mass[i] : uniform_mass);
             if (hp[i].w != common) common = 0;
             mtot += hp[i].w;
+            mmax = std::max(mmax, hp[i].w);
             hv[i] = vel ? make_double4(vel[3 * i], vel[3 * i + 1], vel[3 * i + 2], 0.0)
                         : make_double4(0, 0, 0, 0);
         }
         e->box[b].mass_total = mtot;
+        if (mmax != e->box[b].mass_max) mass_max_changed = true;
+        e->box[b].mass_max = mmax;
     }
     CUCO_HIP(hipMemcpyAsync(e->pos, hp.data(), sizeof(double4) * n, hipMemcpyHostToDevice, e->stream));
     CUCO_HIP(hipMemcpyAsync(e->vel, hv.data(), sizeof(double4) * n, hipMemcpyHostToDevice, e->stream));
@@ -1096,7 +1101,7 @@ int cuco_upload_particles(cuco_engine *e, int64_t n, const double *pos,
     e->n = (uint32_t)n;
     if (!e->ensemble) e->box[0].n = (uint32_t)n;
     const double um = common > 0 ? common : 0;
-    const bool mass_changed = um != e->uniform_mass;
+    const bool mass_changed = um != e->uniform_mass || mass_max_changed;
     e->uniform_mass = um;
     TRY(update_boxes(e));
     if (mass_changed) {

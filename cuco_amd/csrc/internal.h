@@ -117,8 +117,9 @@ struct P32 {
     int M;
     float jscale, jinv;           // fixed-point scale of the each-pair-once j sums
 };
+// mass_max: the box's largest particle mass (bounds the fixed-point j sums).
 P32 make_p32(const PairParams &pp, const GridInfo &g, double uniform_mass,
-             uint32_t n_box);
+             double mass_max, uint32_t n_box);
 // True when the zero tail of the LDS table can make the rcut cut alone.
 bool pp32_table_cut(const PairParams &pp);
 

@@ -215,17 +215,18 @@ __device__ inline float rotate_down(float v)
     return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x134, 0xF, 0xF, false));
 }
 
-// Pair sums of one lane against the 64 staged j (uniform mass folded into
-// the table, cut by the table).  With MASKED a pair counts only when j's
+// Pair sums of one lane against the 64 staged j (cut by the table).  UNI:
+// the common particle mass is folded into the table; otherwise the target
+// side takes j's mass (the staged w) and the j side the target's (mi).  With MASKED a pair counts only when j's
 // sorted index exceeds i's (iid = 0xffffffff for lanes without a target;
 // unused slots carry index 0 and a position inside the tile).
 // rot / rotid hold the 64 staged j twice (slots 64..127 repeat 0..63), so
 // lane L reads slot L + s with no wrap.
-template <bool MASKED, bool NEAR, bool CLAMP>
+template <bool UNI, bool MASKED, bool NEAR, bool CLAMP>
 __device__ inline void interact_sym(const float4 *__restrict__ rot,
                                     const uint32_t *__restrict__ rotid,
                                     const P32 &pp, const float *__restrict__ tab,
-                                    float xi, float yi, float zi, uint32_t iid,
+                                    float xi, float yi, float zi, float mi, uint32_t iid,
                                     float &sx, float &sy, float &sz, float &jx,
                                     float &jy, float &jz)
 {
@@ -233,11 +234,12 @@ __device__ inline void interact_sym(const float4 *__restrict__ rot,
     const float4 *rl = rot + lane;
     const uint32_t *il = rotid + lane;
     for (int s = 0; s < 64; s += 4) {
-        float dx[4], dy[4], dz[4], r[4], fac[4], t[4];
+        float dx[4], dy[4], dz[4], r[4], fac[4], t[4], mj[4];
         bool keep[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const float4 p = rl[s + q];
+            mj[q] = p.w;
             if (MASKED) keep[q] = il[s + q] > iid;
             dx[q] = p.x - xi;
             dy[q] = p.y - yi;
@@ -262,12 +264,14 @@ __device__ inline void interact_sym(const float4 *__restrict__ rot,
         for (int q = 0; q < 4; ++q) {
             float f = fac[q] * t[q];
             if (MASKED) f = keep[q] ? f : 0.0f;
-            sx = fmaf(dx[q], f, sx);
-            sy = fmaf(dy[q], f, sy);
-            sz = fmaf(dz[q], f, sz);
-            jx = rotate_down(fmaf(-dx[q], f, jx));
-            jy = rotate_down(fmaf(-dy[q], f, jy));
-            jz = rotate_down(fmaf(-dz[q], f, jz));
+            const float fi = UNI ? f : f * mj[q];
+            const float fj = UNI ? f : f * mi;
+            sx = fmaf(dx[q], fi, sx);
+            sy = fmaf(dy[q], fi, sy);
+            sz = fmaf(dz[q], fi, sz);
+            jx = rotate_down(fmaf(-dx[q], fj, jx));
+            jy = rotate_down(fmaf(-dy[q], fj, jy));
+            jz = rotate_down(fmaf(-dz[q], fj, jz));
         }
     }
 }
@@ -436,7 +440,8 @@ __global__ __launch_bounds__(kThreads, SYM ? 5 : 1) void k_pp32(
                                          bmaxz, pp.h);
             const bool masked = nj < 64 || cnt < 64u || inside;
 #define CUCO_SYM(MK, NR, CL) \
-    interact_sym<MK, NR, CL>(rot, rotid, pp, s_tab, xi, yi, zi, iid, sx_, sy_, sz_, jx, jy, jz)
+    interact_sym<UNIFORM, MK, NR, CL>(rot, rotid, pp, s_tab, xi, yi, zi, pi.w, iid, sx_, sy_, sz_, \
+                                      jx, jy, jz)
             if (clamp) {
                 if (masked) {
                     if (near) CUCO_SYM(true, true, true);
@@ -913,7 +918,7 @@ int launch_build_tiles(const uint32_t *off, const BoxSet &bs, uint32_t nkeys,
 }
 
 P32 make_p32(const PairParams &pp, const GridInfo &g, double uniform_mass,
-             uint32_t n_box)
+             double mass_max, uint32_t n_box)
 {
     const double sc = pp.asmthfac;
     P32 p;
@@ -931,9 +936,9 @@ P32 make_p32(const PairParams &pp, const GridInfo &g, double uniform_mass,
     p.mass = (float)uniform_mass;
     p.M = g.M;
     // Fixed-point scale of the each-pair-once j sums: a pair term is at most
-    // m * 10.67 / h^2 (scaled units; the spline's peak, the table is <= 1),
+    // m_max * 10.67 / h^2 (scaled units; the spline's peak, the table is <= 1),
     // so n such terms stay below 2^62 / scale.
-    const double pair_max = std::max(uniform_mass, 1e-30) * 10.67 / ((double)p.h * p.h);
+    const double pair_max = std::max(mass_max, 1e-30) * 10.67 / ((double)p.h * p.h);
     const double bound = std::max(1.0, (double)n_box) * pair_max;
     int e2 = 0;
     std::frexp(std::ldexp(1.0, 62) / bound, &e2);
@@ -968,16 +973,19 @@ int launch_pp32(const float4 *spos, const uint32_t *skeys,
     int rc = launch_fill_u32(t.counters + 1, 0u, 1, s);
     if (rc) return rc;
     const int nwg = num_workgroups > 0 ? num_workgroups : 2048;
-    // Each pair once: the uniform-mass, table-cut configuration, all targets.
-    const bool symmetric = sym.on && sym.iacc && sym.jacc && !generic && uniform && !active_end;
+    // Each pair once: the table-cut configuration, all targets.
+    const bool symmetric = sym.on && sym.iacc && sym.jacc && !generic && !active_end;
     if (symmetric) {
         rc = launch_fill_u32(reinterpret_cast<uint32_t *>(sym.jacc), 0u,
                              (size_t)6 * sym.n_sorted, s);
         if (rc) return rc;
-        k_pp32<false, true, true><<<nwg, kThreads, 0, s>>>(
-            spos, skeys, perm, off, tv, p32s, mass, table32, t.tile_start,
-            t.tile_count, t.counters, acc, cand_counter, nullptr, 0, sym.iacc,
-            sym.jacc, sym.n_sorted, nullptr);
+#define CUCO_SYM32(UNIFORM)                                                             \
+    k_pp32<false, UNIFORM, true><<<nwg, kThreads, 0, s>>>(                              \
+        spos, skeys, perm, off, tv, p32s, mass, table32, t.tile_start, t.tile_count,    \
+        t.counters, acc, cand_counter, nullptr, 0, sym.iacc, sym.jacc, sym.n_sorted,   \
+        nullptr)
+        if (uniform) CUCO_SYM32(true); else CUCO_SYM32(false);
+#undef CUCO_SYM32
         CUCO_LAUNCH_CHECK();
         k_pp32_combine<<<xcd_grid(bs.total[kMapPart]), 256, 0, s>>>(
             sym.iacc, sym.jacc, sym.n_sorted, perm, bs.view(kKeyMorton, kMapPart), p32s, acc);
