@@ -215,14 +215,16 @@ __device__ inline float rotate_down(float v)
     return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x134, 0xF, 0xF, false));
 }
 
-// Pair sums of one lane against the 64 staged j (cut by the table).  UNI:
-// the common particle mass is folded into the table; otherwise the target
-// side takes j's mass (the staged w) and the j side the target's (mi).  With MASKED a pair counts only when j's
+// Pair sums of one lane against the 64 staged j.  GEN: rcut is not on a
+// table bin edge, so pairs past it are masked by r^2 and the bin clamped;
+// otherwise the table's zero tail makes the cut.  UNI: the common particle
+// mass is folded into the table; otherwise the target side takes j's mass
+// (the staged w) and the j side the target's (mi).  With MASKED a pair counts only when j's
 // sorted index exceeds i's (iid = 0xffffffff for lanes without a target;
 // unused slots carry index 0 and a position inside the tile).
 // rot / rotid hold the 64 staged j twice (slots 64..127 repeat 0..63), so
 // lane L reads slot L + s with no wrap.
-template <bool UNI, bool MASKED, bool NEAR, bool CLAMP>
+template <bool GEN, bool UNI, bool MASKED, bool NEAR, bool CLAMP>
 __device__ inline void interact_sym(const float4 *__restrict__ rot,
                                     const uint32_t *__restrict__ rotid,
                                     const P32 &pp, const float *__restrict__ tab,
@@ -235,7 +237,7 @@ __device__ inline void interact_sym(const float4 *__restrict__ rot,
     const uint32_t *il = rotid + lane;
     for (int s = 0; s < 64; s += 4) {
         float dx[4], dy[4], dz[4], r[4], fac[4], t[4], mj[4];
-        bool keep[4];
+        bool keep[4], inside[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const float4 p = rl[s + q];
@@ -248,10 +250,11 @@ __device__ inline void interact_sym(const float4 *__restrict__ rot,
             const float rinv = __builtin_amdgcn_rsqf(r2);
             r[q] = r2 * rinv;
             fac[q] = rinv * rinv * rinv;
+            if (GEN) inside[q] = r2 <= pp.rcut2;
             // The table's zero tail from the cut bin on.  CLAMP: bins past
-            // the 1024-entry copy read its last (zero) entry; without it the
-            // tile's extent keeps every bin inside (tile_clamp).
-            t[q] = tab[CLAMP ? min((int)r[q], kTabSym - 1) : (int)r[q]];
+            // the LDS copy read its last (zero) entry; without it the tile's
+            // extent keeps every bin inside (the per-tile clamp test).
+            t[q] = tab[(CLAMP || GEN) ? min((int)r[q], kTabSym - 1) : (int)r[q]];
         }
         if (NEAR) {
             const bool close = (r[0] < pp.h) | (r[1] < pp.h) | (r[2] < pp.h) | (r[3] < pp.h);
@@ -264,6 +267,7 @@ __device__ inline void interact_sym(const float4 *__restrict__ rot,
         for (int q = 0; q < 4; ++q) {
             float f = fac[q] * t[q];
             if (MASKED) f = keep[q] ? f : 0.0f;
+            if (GEN) f = inside[q] ? f : 0.0f;
             const float fi = UNI ? f : f * mj[q];
             const float fj = UNI ? f : f * mi;
             sx = fmaf(dx[q], fi, sx);
@@ -440,8 +444,8 @@ __global__ __launch_bounds__(kThreads, SYM ? 5 : 1) void k_pp32(
                                          bmaxz, pp.h);
             const bool masked = nj < 64 || cnt < 64u || inside;
 #define CUCO_SYM(MK, NR, CL) \
-    interact_sym<UNIFORM, MK, NR, CL>(rot, rotid, pp, s_tab, xi, yi, zi, pi.w, iid, sx_, sy_, sz_, \
-                                      jx, jy, jz)
+    interact_sym<GENERIC, UNIFORM, MK, NR, CL>(rot, rotid, pp, s_tab, xi, yi, zi, pi.w, iid, sx_, \
+                                               sy_, sz_, jx, jy, jz)
             if (clamp) {
                 if (masked) {
                     if (near) CUCO_SYM(true, true, true);
@@ -973,18 +977,22 @@ int launch_pp32(const float4 *spos, const uint32_t *skeys,
     int rc = launch_fill_u32(t.counters + 1, 0u, 1, s);
     if (rc) return rc;
     const int nwg = num_workgroups > 0 ? num_workgroups : 2048;
-    // Each pair once: the table-cut configuration, all targets.
-    const bool symmetric = sym.on && sym.iacc && sym.jacc && !generic && !active_end;
+    // Each pair once: all targets (not an active set).
+    const bool symmetric = sym.on && sym.iacc && sym.jacc && !active_end;
     if (symmetric) {
         rc = launch_fill_u32(reinterpret_cast<uint32_t *>(sym.jacc), 0u,
                              (size_t)6 * sym.n_sorted, s);
         if (rc) return rc;
-#define CUCO_SYM32(UNIFORM)                                                             \
-    k_pp32<false, UNIFORM, true><<<nwg, kThreads, 0, s>>>(                              \
+#define CUCO_SYM32(GENERIC, UNIFORM)                                                    \
+    k_pp32<GENERIC, UNIFORM, true><<<nwg, kThreads, 0, s>>>(                            \
         spos, skeys, perm, off, tv, p32s, mass, table32, t.tile_start, t.tile_count,    \
         t.counters, acc, cand_counter, nullptr, 0, sym.iacc, sym.jacc, sym.n_sorted,   \
         nullptr)
-        if (uniform) CUCO_SYM32(true); else CUCO_SYM32(false);
+        if (generic) {
+            if (uniform) CUCO_SYM32(true, true); else CUCO_SYM32(true, false);
+        } else {
+            if (uniform) CUCO_SYM32(false, true); else CUCO_SYM32(false, false);
+        }
 #undef CUCO_SYM32
         CUCO_LAUNCH_CHECK();
         k_pp32_combine<<<xcd_grid(bs.total[kMapPart]), 256, 0, s>>>(

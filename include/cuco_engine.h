@@ -172,8 +172,9 @@ typedef struct cuco_tuning {
     int32_t pm_overlap;           /* 1: the PM chain runs on a second stream,
                                      concurrently with the pair force */
     int32_t pp_symmetric;         /* 1: FP32 pair force evaluates each pair
-                                     once for both particles (rcut on a table
-                                     bin edge; else ignored) */
+                                     once for both particles (full
+                                     evaluations; active sets of the integer
+                                     timeline use the default kernel) */
 } cuco_tuning;
 
 int cuco_get_tuning(cuco_engine *e, cuco_tuning *t);

@@ -254,3 +254,33 @@ def test_each_pair_once_with_unequal_masses():
     eng.set_tuning(3, 8, 0, 0)
     _, cand_both = eng.count_pairs()
     assert 0 < cand_once < 0.7 * cand_both
+
+
+@pytest.mark.parametrize("unequal", [False, True])
+def test_rcut_off_the_table_edge(unequal):
+    """rcut = 4 asmth puts the cut inside table bin 666 (the reference's 4.5
+    asmth sits on the edge of bin 750): both pair variants then mask pairs
+    past rcut explicitly and clamp the bin.  Pair force within FP32 rounding
+    of the FP64 path, equal and unequal masses."""
+    box = halo_box(seed=16)
+    pos, vel, mass = ics.generate(box)
+    if unequal:
+        mass = mass * np.random.default_rng(16).uniform(0.5, 1.5, box.N)
+    kw = box.engine_kwargs()
+    kw["rcut"] = 4.0 * box.asmth
+    eng = E.Engine(max_particles=box.N, **kw)
+    eng.upload(pos, None, mass)
+    eng.compute_accelerations(E.FP64, True)
+    ref_pp, ref_pm = eng.accelerations()
+    for sym in (0, 1):
+        eng.set_tuning(3, 8, 0, sym)
+        eng.compute_accelerations(E.FP32, True)
+        err = rel_err(eng.accelerations()[0], ref_pp, ref_pp + ref_pm)
+        print(unequal, sym, np.median(err), np.percentile(err, 99), err.max())
+        assert np.percentile(err, 99) <= 2e-5, sym
+        assert np.median(err) <= 1e-6, sym
+    eng.set_tuning(3, 8, 0, 1)
+    _, cand_once = eng.count_pairs()
+    eng.set_tuning(3, 8, 0, 0)
+    _, cand_both = eng.count_pairs()
+    assert 0 < cand_once < 0.7 * cand_both
