@@ -850,6 +850,12 @@ int create_engine(const cuco_params *params, const uint32_t *caps, int nboxes,
     ALLOC(e->bs.dev[1], sizeof(BoxDev) * nboxes);
     ALLOC(e->bs.maps, sizeof(uint32_t) * kMapCount * (nboxes + 1));
     ALLOC(e->bs.tile_map, sizeof(uint32_t) * (nboxes + 1));
+    // FP32 pair force evaluates each pair once by default (40 bytes per
+    // particle of own and fixed-point j sums); cuco_set_tuning can switch to
+    // the kernel that visits every ordered pair.
+    ALLOC(e->psym.iacc, sizeof(float4) * cap);
+    ALLOC(e->psym.jacc, sizeof(unsigned long long) * 3 * cap);
+    e->psym.on = true;
     // Tile arrays sized for the smallest tunable Morton block (most tiles).
     // A Morton block never straddles two boxes: every box's key range is a
     // multiple of 8^bits with bits > shift.
@@ -1842,10 +1848,6 @@ int cuco_set_tuning(cuco_engine *e, const cuco_tuning *t)
         destroy_graph(e);
     }
     if ((t->pp_symmetric != 0) != e->psym.on) {
-        if (t->pp_symmetric && !e->psym.iacc) {
-            TRY(dev_alloc(e->psym.iacc, sizeof(float4) * e->capacity));
-            TRY(dev_alloc(e->psym.jacc, sizeof(unsigned long long) * 3 * (size_t)e->capacity));
-        }
         e->psym.on = t->pp_symmetric != 0;
         destroy_graph(e);
     }

@@ -567,7 +567,7 @@ class Engine:
                 "pm_overlap": t.pm_overlap, "pp_symmetric": t.pp_symmetric}
 
     def set_tuning(self, pp_block_shift: int, pp_workgroups_per_cu: int,
-                   pm_overlap: int = 0, pp_symmetric: int = 0) -> None:
+                   pm_overlap: int = 0, pp_symmetric: int = 1) -> None:
         t = Tuning(pp_block_shift, pp_workgroups_per_cu, int(pm_overlap),
                    int(pp_symmetric))
         _check(lib().cuco_set_tuning(self.h, C.byref(t)), "cuco_set_tuning")

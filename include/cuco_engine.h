@@ -171,10 +171,11 @@ typedef struct cuco_tuning {
                                      per compute unit (1..32) */
     int32_t pm_overlap;           /* 1: the PM chain runs on a second stream,
                                      concurrently with the pair force */
-    int32_t pp_symmetric;         /* 1: FP32 pair force evaluates each pair
-                                     once for both particles (full
-                                     evaluations; active sets of the integer
-                                     timeline use the default kernel) */
+    int32_t pp_symmetric;         /* 1 (the default): FP32 pair force
+                                     evaluates each pair once for both
+                                     particles (full evaluations; active sets
+                                     of the integer timeline visit every
+                                     ordered pair); 0: every ordered pair */
 } cuco_tuning;
 
 int cuco_get_tuning(cuco_engine *e, cuco_tuning *t);
@@ -374,7 +375,8 @@ int cuco_get_shortrange_table(cuco_engine *e, float *table);
  * box's mesh in box order). */
 int cuco_get_pm_grid(cuco_engine *e, int32_t which, double *out);
 /* Exact count of ordered pairs i != j with r^2 <= rcut^2 (FP64 test) and
- * of candidate pairs the FP32 kernel evaluates, for the current positions. */
+ * of candidate pairs the FP32 kernel evaluates, for the current positions
+ * (with pp_symmetric a pair evaluated for both particles counts once). */
 int cuco_count_pairs(cuco_engine *e, uint64_t *pairs_in,
                      uint64_t *fp32_candidates);
 

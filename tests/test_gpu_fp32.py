@@ -15,9 +15,11 @@ pytestmark = pytest.mark.gpu
 P99_BAR = 1e-3
 
 
-def fp32_errors(box, pos, mass, orc=None):
+def fp32_errors(box, pos, mass, orc=None, sym=None):
     eng = make_engine(box)
     eng.upload(pos, None, mass)
+    if sym is not None:
+        eng.set_tuning(**dict(eng.tuning(), pp_symmetric=sym))
     if orc is not None:
         p = oracle_params(box)
         ref_pp, _ = orc.pp_force(p, pos, mass)
@@ -36,20 +38,22 @@ def fp32_errors(box, pos, mass, orc=None):
     return eng, errs
 
 
-def test_lattice_error_percentiles(orc):
+@pytest.mark.parametrize("sym", [0, 1])
+def test_lattice_error_percentiles(orc, sym):
     box = ics.WORKLOADS["lattice32"]
     pos, vel, mass = ics.generate(box)
-    eng, errs = fp32_errors(box, pos, mass, orc)
+    eng, errs = fp32_errors(box, pos, mass, orc, sym)
     print(errs)
     for k in ("pp", "pm", "total"):
         assert errs[k]["p99"] <= P99_BAR, (k, errs)
         assert errs[k]["median"] <= 1e-4, (k, errs)
 
 
-def test_clustered_error_percentiles(orc):
+@pytest.mark.parametrize("sym", [0, 1])
+def test_clustered_error_percentiles(orc, sym):
     box = halo_box()
     pos, vel, mass = ics.generate(box)
-    eng, errs = fp32_errors(box, pos, mass, orc)
+    eng, errs = fp32_errors(box, pos, mass, orc, sym)
     print(errs)
     assert errs["total"]["p99"] <= P99_BAR, errs
 
@@ -145,8 +149,12 @@ def test_fp32_is_deterministic_and_counts_cover_pairs():
     eng.compute_accelerations(E.FP32, True)
     a2 = eng.accelerations()
     assert np.array_equal(a1[0], a2[0]) and np.array_equal(a1[1], a2[1])
-    pairs, cand = eng.count_pairs()
-    assert cand >= pairs > 0
+    assert eng.tuning()["pp_symmetric"] == 1        # the default variant
+    pairs, cand_once = eng.count_pairs()
+    assert 2 * cand_once >= pairs > 0               # unordered pairs, once each
+    eng.set_tuning(**dict(eng.tuning(), pp_symmetric=0))
+    pairs0, cand = eng.count_pairs()
+    assert pairs0 == pairs and cand >= pairs        # every ordered pair
 
 
 def test_every_tuning_gives_the_same_forces():
