@@ -142,6 +142,7 @@ struct cuco_engine {
     // Integer timeline (individual power-of-two timesteps).
     int32_t *ti_beg = nullptr, *ti_end = nullptr;
     int32_t *ti_scalars = nullptr;  // [0] min Ti_endstep, [1] error flags, [2] active count
+    unsigned long long *tl_work = nullptr;  // pair-work estimate: all, active
     double *kick_table = nullptr;
     bool tl_init = false;
     cuco_timeline tl{};
@@ -994,7 +995,7 @@ int cuco_destroy(cuco_engine *e)
     free_ptr(e->dfactors); free_ptr(e->dstep);
     if (e->stream2) (void)hipStreamSynchronize(e->stream2);
     free_ptr(e->ti_beg); free_ptr(e->ti_end);
-    free_ptr(e->ti_scalars); free_ptr(e->kick_table);
+    free_ptr(e->ti_scalars); free_ptr(e->tl_work); free_ptr(e->kick_table);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     if (e->stream2) (void)hipStreamDestroy(e->stream2);
     if (e->ev_fork) (void)hipEventDestroy(e->ev_fork);
@@ -1339,16 +1340,6 @@ double timeline_time(const cuco_engine *e, int ti)
     return e->tl.time_begin * exp(ti * e->tl_interval);  // run.c:57
 }
 
-// Device min of Ti_endstep, then the number of particles at that time.
-__global__ void k_count_eq(const int32_t *__restrict__ ti_end, uint32_t n,
-                           int32_t *__restrict__ scalars)
-{
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool eq = i < n && ti_end[i] == scalars[0];
-    const uint64_t b = __ballot(eq);
-    if ((threadIdx.x & 63) == 0 && b) atomicAdd(&scalars[2], (int32_t)__popcll(b));
-}
-
 // Softening of gravlist.cpp:28-45 at scale factor a: comoving SofteningHalo
 // until it exceeds SofteningHaloMaxPhys physically.
 double softening_table(const cuco_timeline &t, double a)
@@ -1389,9 +1380,7 @@ int timeline_step(cuco_engine *e, int precision, int32_t ti_stop, bool &stopped)
     s.time = timeline_time(e, s.ti_current);
 
     // Active particles of this step.
-    TRY(launch_fill_u32(reinterpret_cast<uint32_t *>(e->ti_scalars) + 2, 0u, 1, e->stream));
-    k_count_eq<<<grid_for(e->n, 256), 256, 0, e->stream>>>(e->ti_end, e->n, e->ti_scalars);
-    CUCO_LAUNCH_CHECK();
+    TRY(launch_count_active(e->ti_end, e->n, s.ti_current, e->ti_scalars + 2, e->stream));
 
     // set_softenings (gravlist.cpp:28-45) at the new time.
     const double soft = softening_table(e->tl, s.time);
@@ -1405,7 +1394,29 @@ int timeline_step(cuco_engine *e, int precision, int32_t ti_stop, bool &stopped)
     // compute_accelerations (accel.c:8-23)
     const bool pm_step = s.pm_ti_endstep == s.ti_current;
     TRY(begin_eval(e));
-    int rc = do_forces(e, precision, pm_step, e->ti_end, s.ti_current);
+    // FP32: when the active particles carry at least 4/5 of the estimated
+    // pair work, the short-range force is evaluated for every particle, each
+    // pair once, and only the active ones are kicked below (the reference
+    // computes every particle's force on every step, accel.c:8-23); the
+    // each-pair-once kernel costs about 4/5 of the one that visits every
+    // ordered pair, whose active-set tiles take the other steps.
+    bool whole = false;
+    int rc = CUCO_OK;
+    if (precision == CUCO_FP32 && e->psym.on) {
+        rc = sort_particles(e, kKeyMorton);
+        if (!rc) rc = launch_active_work(e->skeys, e->perm, e->off, e->n,
+                                         3 * std::min(4, e->min_bits), e->ti_end,
+                                         s.ti_current, e->tl_work, e->stream);
+        unsigned long long w[2] = {0, 0};
+        if (!rc && (hipMemcpyAsync(w, e->tl_work, sizeof(w), hipMemcpyDeviceToHost,
+                                   e->stream) != hipSuccess ||
+                    hipStreamSynchronize(e->stream) != hipSuccess)) {
+            set_error("reading the pair-work estimate failed");
+            rc = CUCO_ERR_HIP;
+        }
+        whole = w[1] * 5 >= w[0] * 4;
+    }
+    if (!rc) rc = do_forces(e, precision, pm_step, whole ? nullptr : e->ti_end, s.ti_current);
     end_eval(e);
     if (rc) return rc;
 
@@ -1473,6 +1484,7 @@ int cuco_timeline_init(cuco_engine *e, const cuco_timeline *t)
         TRY(dev_alloc(e->ti_beg, sizeof(int32_t) * e->capacity));
         TRY(dev_alloc(e->ti_end, sizeof(int32_t) * e->capacity));
         TRY(dev_alloc(e->ti_scalars, sizeof(int32_t) * 4));
+        TRY(dev_alloc(e->tl_work, sizeof(unsigned long long) * 2));
         TRY(dev_alloc(e->kick_table, sizeof(double) * kDriftTable));
     }
     if (!e->atiles.scan) {

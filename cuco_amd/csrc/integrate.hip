@@ -157,18 +157,99 @@ __global__ void k_advance(double4 *__restrict__ vel, const A *__restrict__ app,
     vel[i] = v;
 }
 
-// Smallest Ti_endstep (find_next_sync_point_and_drift, run.c:35-45).
-__global__ void k_ti_min(const int32_t *__restrict__ ti_end, uint32_t n,
-                         int32_t *__restrict__ out)
+// Reductions over Ti_endstep: a fixed grid of 256-thread workgroups strides
+// over int4 loads (ti_end is a whole device allocation, so 16-byte
+// aligned), reduces in registers, across the wave and across the four waves
+// in LDS, and makes one atomic per workgroup.  One atomic per wave on a
+// single address serialises in L2 (3 ms at 16.7M particles).
+constexpr int kRedThreads = 256;
+constexpr uint32_t kRedMaxBlocks = 1024;
+
+inline uint32_t red_blocks(uint32_t n)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    int32_t v = i < n ? ti_end[i] : 0x7fffffff;
+    const uint32_t b = ((n >> 2) + kRedThreads - 1) / kRedThreads;
+    return b < 1 ? 1 : b > kRedMaxBlocks ? kRedMaxBlocks : b;
+}
+
+template <bool COUNT>
+__global__ __launch_bounds__(kRedThreads) void k_ti_reduce(
+    const int32_t *__restrict__ ti_end, uint32_t n, int32_t ti,
+    int32_t *__restrict__ out)
+{
+    __shared__ int32_t sh[kRedThreads / 64];
+    auto op = [](int32_t a, int32_t b) { return COUNT ? a + b : min(a, b); };
+    auto elem = [ti](int32_t x) { return COUNT ? (int32_t)(x == ti) : x; };
+    int32_t v = COUNT ? 0 : 0x7fffffff;
+    const uint32_t n4 = n >> 2, stride = gridDim.x * kRedThreads;
+    const int4 *t4 = reinterpret_cast<const int4 *>(ti_end);
+    for (uint32_t i = blockIdx.x * kRedThreads + threadIdx.x; i < n4; i += stride) {
+        const int4 q = t4[i];
+        v = op(v, op(op(elem(q.x), elem(q.y)), op(elem(q.z), elem(q.w))));
+    }
+    for (uint32_t i = 4 * n4 + blockIdx.x * kRedThreads + threadIdx.x; i < n; i += stride)
+        v = op(v, elem(ti_end[i]));
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
-    if ((threadIdx.x & 63) == 0) atomicMin(out, v);
+    for (int o = 32; o > 0; o >>= 1) v = op(v, __shfl_xor(v, o, 64));
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        v = op(op(sh[0], sh[1]), op(sh[2], sh[3]));
+        if (COUNT) atomicAdd(out, v); else atomicMin(out, v);
+    }
+}
+
+// Pair-work estimate of an active set: every particle weighs the number of
+// particles in its aligned block of 2^shift Morton keys (shift 12: 16^3
+// cells, about the size of the short-range sphere, so the weight tracks the
+// candidates the particle meets); out[0] += weights of all particles,
+// out[1] += weights of the active ones (sorted order: skeys, perm; off is
+// the CSR cell table).
+__global__ __launch_bounds__(kRedThreads) void k_active_work(
+    const uint32_t *__restrict__ skeys, const uint32_t *__restrict__ perm,
+    const uint32_t *__restrict__ off, uint32_t n, int shift,
+    const int32_t *__restrict__ ti_end, int32_t ti,
+    unsigned long long *__restrict__ out)
+{
+    __shared__ unsigned long long sh[2][kRedThreads / 64];
+    unsigned long long wa = 0, wx = 0;
+    const uint32_t stride = gridDim.x * kRedThreads;
+    for (uint32_t j = blockIdx.x * kRedThreads + threadIdx.x; j < n; j += stride) {
+        const uint32_t b0 = (skeys[j] >> shift) << shift;
+        const uint32_t w = off[b0 + (1u << shift)] - off[b0];
+        wa += w;
+        if (ti_end[perm[j]] == ti) wx += w;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        wa += __shfl_xor(wa, o, 64);
+        wx += __shfl_xor(wx, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        sh[0][threadIdx.x >> 6] = wa;
+        sh[1][threadIdx.x >> 6] = wx;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        atomicAdd(&out[0], sh[0][0] + sh[0][1] + sh[0][2] + sh[0][3]);
+        atomicAdd(&out[1], sh[1][0] + sh[1][1] + sh[1][2] + sh[1][3]);
+    }
 }
 
 }  // namespace
+
+int launch_active_work(const uint32_t *skeys, const uint32_t *perm, const uint32_t *off,
+                       uint32_t n, int shift, const int32_t *ti_end, int32_t ti_current,
+                       unsigned long long *out, hipStream_t s)
+{
+    int rc = launch_fill_u32(reinterpret_cast<uint32_t *>(out), 0u, 4, s);
+    if (rc) return rc;
+    if (n == 0) return CUCO_OK;
+    const uint32_t blocks = std::min<uint32_t>(kRedMaxBlocks, (n + kRedThreads - 1) / kRedThreads);
+    k_active_work<<<blocks, kRedThreads, 0, s>>>(skeys, perm, off, n, shift, ti_end,
+                                                  ti_current, out);
+    CUCO_LAUNCH_CHECK();
+    return CUCO_OK;
+}
 
 int launch_drift(double4 *pos, const double4 *vel, const BoxSet &bs,
                  double drift, hipStream_t s)
@@ -258,7 +339,18 @@ int launch_ti_min(const int32_t *ti_end, uint32_t n, int32_t *out, hipStream_t s
     int rc = launch_fill_u32(reinterpret_cast<uint32_t *>(out), 0x7fffffffu, 1, s);
     if (rc) return rc;
     if (n == 0) return CUCO_OK;
-    k_ti_min<<<grid_for(n, 256), 256, 0, s>>>(ti_end, n, out);
+    k_ti_reduce<false><<<red_blocks(n), kRedThreads, 0, s>>>(ti_end, n, 0, out);
+    CUCO_LAUNCH_CHECK();
+    return CUCO_OK;
+}
+
+int launch_count_active(const int32_t *ti_end, uint32_t n, int32_t ti_current,
+                        int32_t *out, hipStream_t s)
+{
+    int rc = launch_fill_u32(reinterpret_cast<uint32_t *>(out), 0u, 1, s);
+    if (rc) return rc;
+    if (n == 0) return CUCO_OK;
+    k_ti_reduce<true><<<red_blocks(n), kRedThreads, 0, s>>>(ti_end, n, ti_current, out);
     CUCO_LAUNCH_CHECK();
     return CUCO_OK;
 }

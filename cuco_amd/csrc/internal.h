@@ -234,7 +234,18 @@ int launch_advance(double4 *vel, const void *app, const void *apm, int fp32,
                    int32_t *ti_beg, int32_t *ti_end, uint32_t n,
                    const TimelineDev &t, const double *kick_table,
                    int32_t *flags, hipStream_t s);
-// out[0] = min(ti_end[0..n)) (out is cleared first).
+// out[0] = min(ti_end[0..n)) (out is cleared first).  ti_end must be
+// 16-byte aligned.
 int launch_ti_min(const int32_t *ti_end, uint32_t n, int32_t *out, hipStream_t s);
+// Pair-work estimate of the active set (sorted order, CSR cell offsets):
+// out[0] = sum over particles of the occupancy of their aligned block of
+// 2^shift keys, out[1] = the same over the active ones (out cleared first;
+// every box's key range must be a multiple of 2^shift).
+int launch_active_work(const uint32_t *skeys, const uint32_t *perm, const uint32_t *off,
+                       uint32_t n, int shift, const int32_t *ti_end, int32_t ti_current,
+                       unsigned long long *out, hipStream_t s);
+// out[0] = number of i with ti_end[i] == ti_current (out is cleared first).
+int launch_count_active(const int32_t *ti_end, uint32_t n, int32_t ti_current,
+                        int32_t *out, hipStream_t s);
 
 }  // namespace cuco

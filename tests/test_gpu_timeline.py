@@ -77,7 +77,7 @@ def test_active_set_force_equals_full_evaluation(orc):
         st = eng.timeline_run(precision, max_steps=1)
         b, _ = eng.timesteps()
         active = b == st["ti_current"]        # kicked in this step: begstep moved here
-        assert 0 < active.sum()
+        assert 0 < active.sum() == st["active"]
         app = eng.accelerations(pp=True, pm=False)
         pos, _ = eng.download_particles()
         full = make_engine(box)
@@ -92,6 +92,23 @@ def test_active_set_force_equals_full_evaluation(orc):
             # candidates are chunked differently: equal to FP32 rounding.
             scale = np.linalg.norm(fa, axis=1).max()
             assert np.abs(app[active] - fa[active]).max() <= 1e-5 * scale
+
+
+def test_mostly_active_step_is_a_whole_evaluation(orc):
+    """A step whose active particles carry at least 4/5 of the estimated
+    pair work (the first: all particles active) evaluates the FP32
+    short-range force of every particle with the each-pair-once kernel:
+    bit-identical to a full evaluation."""
+    box, tl, ref, eng = setup(orc, seed=11)
+    st = eng.timeline_run(E.FP32, max_steps=1)
+    assert st["active"] == box.N
+    pos, _ = eng.download_particles()
+    full = make_engine(box)
+    full.upload(pos, None, ics.generate(box)[2])
+    full.set_softening(eng_soft(tl, st["time"]))
+    full.gravity_linklist(E.FP32)
+    assert np.array_equal(eng.accelerations(pp=True, pm=False),
+                          full.accelerations(pp=True, pm=False))
 
 
 def eng_soft(tl, a):
