@@ -422,6 +422,11 @@ def main() -> int:
         "tuning": tuning,
         "fp32_ms_other_overlap": other_ms,
         "stage_ms_fp32": {k: v / max(nev, 1) for k, v in st32.items()},
+        "stage_ms_note": ("HIP-event time per stage; with pm_overlap=1 the PM stages "
+                          "(deposit .. interp) run on a second stream beside the pair "
+                          "force, so their times include waiting for the compute units "
+                          "the pair kernel holds" if tuning.get("pm_overlap") else
+                          "HIP-event time per stage, one stream"),
         "stage_ms_fp64": {k: v / max(args.fp64_steps, 1) for k, v in st64.items()},
         "roofline": roofline,
         "cpu_baseline": None,

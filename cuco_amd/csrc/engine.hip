@@ -1414,7 +1414,7 @@ int timeline_step(cuco_engine *e, int precision, int32_t ti_stop, bool &stopped)
             set_error("reading the pair-work estimate failed");
             rc = CUCO_ERR_HIP;
         }
-        whole = w[1] * 5 >= w[0] * 4;
+        whole = (double)w[1] >= 0.8 * (double)w[0];
     }
     if (!rc) rc = do_forces(e, precision, pm_step, whole ? nullptr : e->ti_end, s.ti_current);
     end_eval(e);
