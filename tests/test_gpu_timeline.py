@@ -133,6 +133,57 @@ def test_fp32_timeline_tracks_fp64(orc):
     assert s64["steps"] == 8
 
 
+def active_work_share(box, pos, active):
+    """Share of the active particles in the pair-work estimate: every
+    particle weighs the occupancy of its aligned 16^3-cell block of the
+    Morton key cube (cells sharing ix >> 4, iy >> 4, iz >> 4)."""
+    c = np.minimum((pos * (box.M / box.L)).astype(np.int64), box.M - 1) >> 4
+    k = (c[:, 0] * 64 + c[:, 1]) * 64 + c[:, 2]
+    w = np.bincount(k)[k].astype(np.float64)
+    return w[active].sum() / w.sum()
+
+
+def test_force_path_follows_the_pair_work_rule(orc):
+    """FP32 steps whose active set carries less than 4/5 of the estimated
+    pair work leave the inactive particles' short-range accelerations
+    untouched (active-set tiles); the others recompute them (whole
+    evaluation)."""
+    box, tl, ref, eng = setup(orc, seed=13)
+    eng.timeline_run(E.FP32, max_steps=1)          # all active
+    seen = set()
+    for _ in range(10):
+        before = eng.accelerations(pp=True, pm=False)
+        st = eng.timeline_run(E.FP32, max_steps=1)
+        b, _ = eng.timesteps()
+        active = b == st["ti_current"]
+        if active.all():
+            continue
+        pos, _ = eng.download_particles()
+        whole = active_work_share(box, pos, active) >= 0.8
+        after = eng.accelerations(pp=True, pm=False)
+        untouched = np.array_equal(after[~active], before[~active])
+        assert untouched != whole, (st["ti_current"], active.mean())
+        seen.add(whole)
+    assert False in seen
+
+
+def test_fp32_timeline_repeats_bit_for_bit(orc):
+    """Every FP32 force on the timeline is deterministic (fixed-point j sums,
+    fixed per-target order, fixed-point mesh deposit) and the choice between
+    active-set tiles and a whole evaluation depends on the particles only:
+    two handles given the same particles agree bit for bit."""
+    runs = []
+    for _ in range(2):
+        box, tl, ref, eng = setup(orc, seed=12)
+        st = eng.timeline_run(E.FP32, max_steps=10)
+        pos, vel = eng.download_particles()
+        runs.append((st, pos, vel, *eng.timesteps()))
+    (s0, *a0), (s1, *a1) = runs
+    assert s0 == s1
+    for x, y in zip(a0, a1):
+        assert np.array_equal(x, y)
+
+
 def test_records_and_snapshot_round_trip(orc, tmp_path):
     box, tl, ref, eng = setup(orc)
     st = eng.timeline_run(E.FP64, max_steps=4)
