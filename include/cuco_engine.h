@@ -90,7 +90,9 @@ const char *cuco_last_error(void);
 /* Replaces cudaInit (gravlist.cu:25-31), linklist_init's device set-up
  * (gravlist.cpp:48-115), pm_init_periodic (pm_periodic.c:25-43) and the
  * RadixSort constructor (radixsort.h:140-150).  Allocates every device
- * buffer for up to max_particles particles on HIP device `device`. */
+ * buffer for up to max_particles particles on HIP device `device` (about
+ * 265 bytes per particle plus the cell table; the PM meshes and deposit
+ * scratch of each precision are allocated on its first use). */
 int cuco_create(const cuco_params *params, int64_t max_particles,
                 int32_t device, cuco_engine **out);
 /* Frees everything the handle owns (the reference never frees). */
