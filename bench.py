@@ -256,8 +256,8 @@ def main() -> int:
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not "
-              "supported (the engine is single-GPU; sharding is described in "
-              "DESIGN.md)", file=sys.stderr)
+              "supported (the engine runs one GPU per handle; the whole job "
+              "here is one GPU's share)", file=sys.stderr)
         return 2
 
     from cuco_amd import engine as E
