@@ -210,6 +210,186 @@ def timeline_bench(E, ics, box, pos, vel, mass, steps):
     }
 
 
+def stage_bytes(N: int, M: int, key_bits: int, nkeys: int) -> dict:
+    """Compulsory HBM bytes of one FP32 evaluation's stages (every input
+    read once, every output written once) for N particles and an M^3 mesh:
+    R = M^3 4-byte reals, C = M^2 (M/2+1) 8-byte complex values."""
+    R = 4 * M ** 3
+    C = 8 * M * M * (M // 2 + 1)
+    passes = -(-key_bits // 8)
+    return {
+        # positions (double4) in, 4-byte Morton key out
+        "keys": 36 * N,
+        # per 8-bit pass: key read for the digit counts, key/value read and
+        # written by the scatter
+        "sort": passes * 20 * N,
+        # CSR cell offsets (keys in, one entry per key written) + reorder
+        # (perm and key in, double4 gathered, cell-relative float4 out)
+        "cells": 4 * N + 4 * (nkeys + 1) + 56 * N,
+        # sorted float4 + key in, density mesh out
+        "deposit": 20 * N + R,
+        # z rows r2c (R in, C out) + y lines (C in, C out)
+        "fft_fwd": R + 3 * C,
+        # x lines: forward FFT, Green's function, inverse FFT in one pass
+        "green_x": 2 * C,
+        # y lines + z rows c2r
+        "fft_inv": 3 * C + R,
+        # potential mesh in, sorted float4 + key + perm in, float4 out
+        "interp": R + 40 * N,
+    }
+
+
+def hbm_roofline(stage_ms: dict, nbytes: dict, stages) -> dict:
+    """Achieved GB/s and the fraction of the 8 TB/s HBM peak per stage and
+    summed over `stages`, from HIP-event stage times."""
+    out = {}
+    tb, tm = 0, 0.0
+    for k in stages:
+        ms = stage_ms.get(k, 0.0)
+        if ms <= 0:
+            continue
+        gbs = nbytes[k] / (ms * 1e-3) / 1e9
+        out[k] = {"bytes": nbytes[k], "ms": ms, "achieved": gbs,
+                  "frac": gbs / PEAK_HBM_GBS}
+        tb += nbytes[k]
+        tm += ms
+    if tm > 0:
+        out["total"] = {"bytes": tb, "ms": tm, "achieved": tb / (tm * 1e-3) / 1e9,
+                        "frac": tb / (tm * 1e-3) / 1e9 / PEAK_HBM_GBS}
+    return out
+
+
+def morton_bits(M: int) -> int:
+    return 3 * max(1, math.ceil(math.log2(M)))
+
+
+def pm_and_sort_roofline(E, eng, box, stage_fp32, reps=3) -> dict:
+    """HBM roofline of the sort and PM stages of the FP32 path.  Sort stages
+    (keys, sort, cell table) from the timed evaluations, which run them on
+    the main stream before anything else; PM stages from PM-only
+    evaluations (nothing runs beside them)."""
+    eng.set_timing(True)
+    eng.pmforce_periodic(E.FP32)
+    eng.synchronize()
+    eng.stage_times(reset=True)
+    for _ in range(reps):
+        eng.pmforce_periodic(E.FP32)
+    eng.synchronize()
+    st, n = eng.stage_times(reset=True)
+    eng.set_timing(False)
+    pm_ms = {k: v / max(n, 1) for k, v in st.items()}
+    kb = morton_bits(box.M)
+    nbytes = stage_bytes(box.N, box.M, kb, 1 << kb)
+    return {
+        "bound": "hbm",
+        "unit": "GB/s",
+        "peak": PEAK_HBM_GBS,
+        "note": ("compulsory bytes per stage (each input read once, each output "
+                 "written once) / HIP-event stage time; PM stages timed in "
+                 "PM-only evaluations"),
+        "sort": hbm_roofline(stage_fp32, nbytes, ["keys", "sort", "cells"]),
+        "pm": hbm_roofline(pm_ms, nbytes, ["deposit", "fft_fwd", "green_x", "fft_inv",
+                                           "interp"]),
+    }
+
+
+def halo512_bench(E, ics, tuning, steps=5, n_oracle=1_000_000, seed=5006,
+                  with_oracle=True):
+    """The large single box: 512^3 particles, 1024^3 FP32 mesh, 1000 Mpc/h,
+    20000 halos.  `steps` FP32 KDK steps (one hipGraph per step, after a
+    warm-up run that captures it), the pair kernel's interactions/s against
+    the FP32 vector peak, the PM chain's HBM GB/s, and the FP32 force error
+    at the initial conditions against the FP64 oracle on a random subset of
+    n_oracle particles (PP for those targets over all particles, PM from
+    the oracle's full FP64 mesh)."""
+    box = ics.WORKLOADS["halo512"]
+    t0 = time.perf_counter()
+    pos, vel, mass = ics.generate(box)
+    log(f"[bench] halo512: N={box.N} generated in {time.perf_counter() - t0:.1f}s")
+    eng = E.Engine(max_particles=box.N, **box.engine_kwargs())
+    eng.upload(pos, vel, mass)
+    eng.set_tuning(**tuning)
+    eng.compute_accelerations(E.FP32, True)
+    eng.synchronize()
+    a32, p32 = eng.accelerations()
+    pairs_in, candidates = eng.count_pairs()
+    eng.set_timing(True)
+    eng.gravity_linklist(E.FP32)
+    eng.synchronize()
+    eng.stage_times(reset=True)
+    for _ in range(2):
+        eng.gravity_linklist(E.FP32)
+    eng.synchronize()
+    st_pp, n_pp = eng.stage_times(reset=True)
+    eng.set_timing(False)
+    pp_ms = st_pp["pp"] / max(n_pp, 1)
+    pm = pm_and_sort_roofline(E, eng, box, {}, reps=2)["pm"]
+    b0 = box
+    d, k = E.init_drift_table(b0.a_start, b0.a_end, ics.OMEGA_M, ics.OMEGA_L,
+                              ics.HUBBLE_INTERNAL)
+    f = E.kdk_factors(d, k, b0.a_start, b0.a_end, b0.a_start, b0.a_end, steps)
+    ts = []
+    for rep in range(2):                  # the first run captures the graph
+        eng.upload(pos, vel, mass)
+        eng.compute_accelerations(E.FP32, True)
+        eng.synchronize()
+        t = time.perf_counter()
+        eng.kdk_steps(f, E.FP32, use_graph=True)
+        ts.append(time.perf_counter() - t)
+    eng.close()
+    del vel
+    step_ms = ts[-1] / steps * 1e3
+    log(f"[bench] halo512: {step_ms:.1f} ms per KDK step, pair kernel {pp_ms:.1f} ms")
+    achieved = FLOPS_PER_INTERACTION * pairs_in / (pp_ms * 1e-3) / 1e12
+    out = {
+        "workload": (f"halo512: {box.n_side}^3 particles, {box.M}^3 FP32 mesh, "
+                     f"{box.L / 1000:g} Mpc/h, 50% in {box.n_halos} NFW halos + 50% "
+                     f"uniform; {steps} FP32 KDK steps from a = {box.a_start}, one "
+                     "hipGraph per step"),
+        "n_particles": box.N,
+        "steps": steps,
+        "ms_per_step": step_ms,
+        "evals_per_s": box.N / (step_ms * 1e-3),
+        "tuning": tuning,
+        "pp": {"ms": pp_ms, "pairs_in": pairs_in, "candidates": candidates,
+               "interactions_per_s": pairs_in / (pp_ms * 1e-3),
+               "achieved_tflops": achieved, "peak_tflops": PEAK_FP32_TFLOPS,
+               "frac": achieved / PEAK_FP32_TFLOPS},
+        "pm_roofline": pm,
+        "error_vs_oracle": None,
+    }
+    if with_oracle:
+        import oracle
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
+        threads = min(threads, os.cpu_count() or threads)
+        p = oracle.params(box.M, box.L, ics.G_INTERNAL, box.asmth, box.rcut,
+                          box.soft_h, nthreads=threads)
+        rng = np.random.default_rng(seed)
+        targets = np.sort(rng.choice(box.N, n_oracle, replace=False))
+        t0 = time.perf_counter()
+        ref_pp, _ = oracle.pp_force(p, pos, mass, targets)
+        t_pp = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        ref_pm = oracle.pm_force(p, pos, mass)[targets]
+        t_pm = time.perf_counter() - t0
+        tot = ref_pp + ref_pm
+        den = np.linalg.norm(tot, axis=1)
+        den = np.where(den > 0, den, 1.0)
+        a, q = a32[targets], p32[targets]
+        out["error_vs_oracle"] = {
+            "targets": n_oracle,
+            "seed": seed,
+            "denominator": "|a_total| of the oracle",
+            "pp": percentiles(np.linalg.norm(a - ref_pp, axis=1) / den),
+            "pm": percentiles(np.linalg.norm(q - ref_pm, axis=1) / den),
+            "total": percentiles(np.linalg.norm(a + q - tot, axis=1) / den),
+            "oracle_s": {"pp": t_pp, "pm": t_pm, "threads": threads},
+        }
+        log(f"[bench] halo512 error vs oracle {out['error_vs_oracle']['total']} "
+            f"(oracle PP {t_pp:.1f}s, PM {t_pm:.1f}s)")
+    return out
+
+
 def pmc_traffic(kernel: str, workload: str, key: str = "traffic_bytes"):
     """HBM bytes per dispatch of `kernel` on `workload` from the committed
     rocprofv3 PMC summary (profiles/<kernel>_<workload>_pmc.md, the line
@@ -240,6 +420,11 @@ def main() -> int:
     ap.add_argument("--ensemble-steps", type=int, default=20)
     ap.add_argument("--timeline-steps", type=int, default=12,
                     help="individual-timestep steps timed after the first (0: skip)")
+    ap.add_argument("--halo512-steps", type=int, default=5,
+                    help="FP32 KDK steps of the 512^3 / 1024^3 leg (0: skip)")
+    ap.add_argument("--halo512-oracle", type=int, default=1000000,
+                    help="random particles of the 512^3 leg checked against the "
+                         "FP64 oracle (0: no oracle)")
     ap.add_argument("--pm-overlap", choices=["auto", "0", "1"], default="auto",
                     help="PM chain on a second stream beside the pair force: "
                          "the autotuner's choice, or forced off / on")
@@ -372,11 +557,12 @@ def main() -> int:
     pp_ms = st_pp["pp"] / max(n_pp, 1)
     achieved = FLOPS_PER_INTERACTION * pairs_in / (pp_ms * 1e-3) / 1e12
     roofline = {
-        "bound": "mfma",
-        "note": ("compute roofline of the FP32 pair kernel; it runs on the FP32 "
-                 "vector ALUs (a pair force is no dense contraction, so no MFMA) "
-                 "against MI355X's published FP32 peak; packed FP32 is disabled "
-                 "in every kernel, which caps this kernel at half of that peak"),
+        "bound": "valu_fp32",
+        "note": ("compute roofline of the FP32 pair kernel: FP32 vector (VALU) "
+                 "instructions, no MFMA (a pair force is no dense contraction), "
+                 "against MI355X's published FP32 vector peak; packed FP32 is "
+                 "disabled in every kernel, which caps this kernel at half of "
+                 "that peak"),
         "kernel": ("k_pp32 (FP32 pair force, each pair once for both particles) + "
                    "k_pp32_combine" if tuning.get("pp_symmetric")
                    else "k_pp32 (FP32 pair force)"),
@@ -429,6 +615,8 @@ def main() -> int:
                           "HIP-event time per stage, one stream"),
         "stage_ms_fp64": {k: v / max(args.fp64_steps, 1) for k, v in st64.items()},
         "roofline": roofline,
+        "pm_roofline": pm_and_sort_roofline(E, eng, box, {k: v / max(nev, 1)
+                                                          for k, v in st32.items()}),
         "cpu_baseline": None,
     }
     eng.close()
@@ -437,6 +625,10 @@ def main() -> int:
     if args.ensemble_boxes > 0:
         out["ensemble"] = ensemble_bench(E, ics, args.ensemble_boxes,
                                          args.ensemble_steps)
+    if args.halo512_steps > 0 and box.name != "halo512":
+        out["halo512"] = halo512_bench(E, ics, tuning, args.halo512_steps,
+                                       args.halo512_oracle,
+                                       with_oracle=args.halo512_oracle > 0)
     if not args.no_cpu_baseline:
         t0 = time.perf_counter()
         parity, out["cpu_baseline"] = cpu_baseline(box, pos, mass, args.cpu_targets,

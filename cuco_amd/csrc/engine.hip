@@ -190,6 +190,18 @@ int set_device(cuco_engine *e)
         if (rc_) return rc_;   \
     } while (0)
 
+// Handler of the function-try-blocks around the C entry points that build
+// host arrays: no C++ exception crosses the C boundary.
+#define CUCO_CATCH                                                          \
+    catch (const std::bad_alloc &) {                                        \
+        ::cuco::set_error("host memory allocation failed");                 \
+        return CUCO_ERR_OUT_OF_MEMORY;                                      \
+    }                                                                       \
+    catch (const std::exception &ex_) {                                     \
+        ::cuco::set_error("host exception: %s", ex_.what());                \
+        return CUCO_ERR_INVALID_ARG;                                        \
+    }
+
 // ------------------------------------------------------------ timing
 int begin_eval(cuco_engine *e)
 {
@@ -606,7 +618,7 @@ int kick_current(cuco_engine *e, double kpp, double kpm)
     }
     const bool pp32 = e->pp_prec == CUCO_FP32, pm32 = e->pm_prec == CUCO_FP32;
     if (e->pp_prec >= 0 && e->pm_prec >= 0 && pp32 != pm32) {
-        // Mixed precisions: two kicks, PP first (timestep.c:223-226 then :253-255).
+        // Mixed precisions: two kicks, PP first (timestep.c:56-59 then :86-92).
         if (pp32) TRY(launch_kick32(e->vel, e->acc_pp32, nullptr, e->n, kpp, 0, e->stream));
         else TRY(launch_kick64(e->vel, e->acc_pp64, nullptr, e->n, kpp, 0, e->stream));
         if (pm32) TRY(launch_kick32(e->vel, nullptr, e->acc_pm32, e->n, 0, kpm, e->stream));
@@ -906,7 +918,7 @@ extern "C" {
 
 int cuco_create(const cuco_params *params, int64_t max_particles, int32_t device,
                 cuco_engine **out)
-{
+try {
     if (!params || !out) {
         set_error("null argument");
         return CUCO_ERR_INVALID_ARG;
@@ -918,11 +930,11 @@ int cuco_create(const cuco_params *params, int64_t max_particles, int32_t device
     }
     const uint32_t cap = (uint32_t)max_particles;
     return create_engine(params, &cap, 1, false, device, out);
-}
+} CUCO_CATCH
 
 int cuco_ensemble_create(const cuco_params *params, const int64_t *n_per_box,
                          int32_t nboxes, int32_t device, cuco_engine **out)
-{
+try {
     if (!params || !n_per_box || !out) {
         set_error("null argument");
         return CUCO_ERR_INVALID_ARG;
@@ -941,7 +953,7 @@ int cuco_ensemble_create(const cuco_params *params, const int64_t *n_per_box,
         caps[b] = (uint32_t)n_per_box[b];
     }
     return create_engine(params, caps.data(), nboxes, true, device, out);
-}
+} CUCO_CATCH
 
 int cuco_ensemble_size(cuco_engine *e, int32_t *nboxes, int64_t *n_per_box)
 {
@@ -1058,11 +1070,31 @@ int cuco_set_params(cuco_engine *e, int32_t box, const cuco_params *params)
     return update_pair_constants(e);
 }
 
-int cuco_upload_particles(cuco_engine *e, int64_t n, const double *pos,
-                          const double *vel, const double *mass,
-                          double uniform_mass)
+}  // extern "C"
+
+namespace {
+
+// Checks a particle count against the handle: [1, capacity], and exactly the
+// capacity for an ensemble.
+int check_count(const cuco_engine *e, int64_t n)
 {
-    CHECK_ENGINE(e);
+    if (e->ensemble && n != (int64_t)e->capacity) {
+        set_error("ensemble needs exactly %u particles (got %lld)", e->capacity, (long long)n);
+        return CUCO_ERR_INVALID_ARG;
+    }
+    if (n <= 0 || n > (int64_t)e->capacity) {
+        set_error("n=%lld outside [1, %u]", (long long)n, e->capacity);
+        return CUCO_ERR_INVALID_ARG;
+    }
+    return CUCO_OK;
+}
+
+// keep_vel: positions (and masses) only.  When the handle already holds
+// these n particles their device velocities and integer-timeline state stay;
+// otherwise the velocities start at zero.
+int upload_impl(cuco_engine *e, int64_t n, const double *pos, const double *vel,
+                const double *mass, double uniform_mass, bool keep_vel)
+{
     if (e->ensemble) {
         if (n != (int64_t)e->capacity || !pos) {
             set_error("ensemble upload needs exactly %u particles (got %lld) and positions",
@@ -1101,8 +1133,10 @@ int cuco_upload_particles(cuco_engine *e, int64_t n, const double *pos,
         if (mmax != e->box[b].mass_max) mass_max_changed = true;
         e->box[b].mass_max = mmax;
     }
+    const bool keep = keep_vel && e->have_particles && (uint32_t)n == e->n;
     CUCO_HIP(hipMemcpyAsync(e->pos, hp.data(), sizeof(double4) * n, hipMemcpyHostToDevice, e->stream));
-    CUCO_HIP(hipMemcpyAsync(e->vel, hv.data(), sizeof(double4) * n, hipMemcpyHostToDevice, e->stream));
+    if (!keep)
+        CUCO_HIP(hipMemcpyAsync(e->vel, hv.data(), sizeof(double4) * n, hipMemcpyHostToDevice, e->stream));
     CUCO_HIP(hipStreamSynchronize(e->stream));
     if ((uint32_t)n != e->n) destroy_graph(e);
     e->n = (uint32_t)n;
@@ -1118,12 +1152,24 @@ int cuco_upload_particles(cuco_engine *e, int64_t n, const double *pos,
     e->have_particles = true;
     e->sorted_mode = -1;
     e->pp_prec = e->pm_prec = -1;
-    e->tl_init = false;
+    if (!keep) e->tl_init = false;
     return CUCO_OK;
 }
 
+}  // namespace
+
+extern "C" {
+
+int cuco_upload_particles(cuco_engine *e, int64_t n, const double *pos,
+                          const double *vel, const double *mass,
+                          double uniform_mass)
+try {
+    CHECK_ENGINE(e);
+    return upload_impl(e, n, pos, vel, mass, uniform_mass, false);
+} CUCO_CATCH
+
 int cuco_download_particles(cuco_engine *e, double *pos, double *vel)
-{
+try {
     CHECK_ENGINE(e);
     TRY(check_ready(e));
     std::vector<double4> h(e->n);
@@ -1140,10 +1186,10 @@ int cuco_download_particles(cuco_engine *e, double *pos, double *vel)
         }
     }
     return CUCO_OK;
-}
+} CUCO_CATCH
 
 int cuco_download_accel(cuco_engine *e, double *acc_pp, double *acc_pm)
-{
+try {
     CHECK_ENGINE(e);
     TRY(check_ready(e));
     for (int which = 0; which < 2; ++which) {
@@ -1177,7 +1223,7 @@ int cuco_download_accel(cuco_engine *e, double *acc_pp, double *acc_pm)
         }
     }
     return CUCO_OK;
-}
+} CUCO_CATCH
 
 int cuco_pp_force(cuco_engine *e, int32_t precision)
 {
@@ -1470,7 +1516,7 @@ int timeline_step(cuco_engine *e, int precision, int32_t ti_stop, bool &stopped)
 extern "C" {
 
 int cuco_timeline_init(cuco_engine *e, const cuco_timeline *t)
-{
+try {
     CHECK_ENGINE(e);
     TRY(check_ready(e));
     if (!t || !(t->time_begin > 0) || !(t->time_max > t->time_begin) ||
@@ -1524,7 +1570,7 @@ int cuco_timeline_init(cuco_engine *e, const cuco_timeline *t)
     s.timebase_interval = e->tl_interval;
     e->tl_init = true;
     return CUCO_OK;
-}
+} CUCO_CATCH
 
 int cuco_timeline_run(cuco_engine *e, int32_t precision, int32_t max_steps,
                       int32_t ti_stop, cuco_timeline_state *state)
@@ -1577,7 +1623,7 @@ int cuco_get_timesteps(cuco_engine *e, int32_t *ti_begstep, int32_t *ti_endstep)
 }
 
 int cuco_download_records(cuco_engine *e, cuco_particle *P)
-{
+try {
     CHECK_ENGINE(e);
     TRY(check_ready(e));
     if (!P) {
@@ -1610,10 +1656,10 @@ int cuco_download_records(cuco_engine *e, cuco_particle *P)
         P[i].Ti_begstep = tb[i];
     }
     return CUCO_OK;
-}
+} CUCO_CATCH
 
 int cuco_upload_records(cuco_engine *e, const cuco_particle *P, int64_t n)
-{
+try {
     CHECK_ENGINE(e);
     if (!P || n <= 0) {
         set_error("null particle array or n <= 0");
@@ -1628,49 +1674,51 @@ int cuco_upload_records(cuco_engine *e, const cuco_particle *P, int64_t n)
         mass[i] = P[i].Mass;
     }
     return cuco_upload_particles(e, n, pos.data(), vel.data(), mass.data(), 0.0);
-}
+} CUCO_CATCH
 
 int cuco_pmforce_periodic(cuco_engine *e, cuco_particle *P, int64_t n, int32_t precision)
-{
+try {
     CHECK_ENGINE(e);
     if (!P) {
         set_error("null particle array");
         return CUCO_ERR_INVALID_ARG;
     }
+    TRY(check_count(e, n));
     std::vector<double> pos((size_t)n * 3), mass((size_t)n);
     for (int64_t i = 0; i < n; ++i) {
         for (int d = 0; d < 3; ++d) pos[3 * i + d] = P[i].Pos[d];
         mass[i] = P[i].Mass;
     }
-    TRY(cuco_upload_particles(e, n, pos.data(), nullptr, mass.data(), 0.0));
+    TRY(upload_impl(e, n, pos.data(), nullptr, mass.data(), 0.0, true));
     TRY(cuco_pm_force(e, precision));
     std::vector<double> acc((size_t)n * 3);
     TRY(cuco_download_accel(e, nullptr, acc.data()));
     for (int64_t i = 0; i < n; ++i)
         for (int d = 0; d < 3; ++d) P[i].GravPM[d] = (float)acc[3 * i + d];
     return CUCO_OK;
-}
+} CUCO_CATCH
 
 int cuco_gravity_linklist(cuco_engine *e, cuco_particle *P, int64_t n, int32_t precision)
-{
+try {
     CHECK_ENGINE(e);
     if (!P) {
         set_error("null particle array");
         return CUCO_ERR_INVALID_ARG;
     }
+    TRY(check_count(e, n));
     std::vector<double> pos((size_t)n * 3), mass((size_t)n);
     for (int64_t i = 0; i < n; ++i) {
         for (int d = 0; d < 3; ++d) pos[3 * i + d] = P[i].Pos[d];
         mass[i] = P[i].Mass;
     }
-    TRY(cuco_upload_particles(e, n, pos.data(), nullptr, mass.data(), 0.0));
+    TRY(upload_impl(e, n, pos.data(), nullptr, mass.data(), 0.0, true));
     TRY(cuco_pp_force(e, precision));
     std::vector<double> acc((size_t)n * 3);
     TRY(cuco_download_accel(e, acc.data(), nullptr));
     for (int64_t i = 0; i < n; ++i)
         for (int d = 0; d < 3; ++d) P[i].GravAccel[d] = (float)acc[3 * i + d];
     return CUCO_OK;
-}
+} CUCO_CATCH
 
 int cuco_device_alloc(void **dptr, size_t bytes)
 {
@@ -1711,7 +1759,7 @@ int cuco_copy_device_to_device(void *dst, const void *src, size_t bytes)
 }
 
 int cuco_radix_sort(uint32_t *d_keys, uint32_t *d_values, uint32_t n, uint32_t key_bits)
-{
+try {
     if (n == 0) return CUCO_OK;
     if (!d_keys || key_bits == 0 || key_bits > 32) {
         set_error("null keys or key_bits %u outside 1..32", key_bits);
@@ -1738,10 +1786,10 @@ int cuco_radix_sort(uint32_t *d_keys, uint32_t *d_values, uint32_t n, uint32_t k
     }
     free_ptr(sc.keys_tmp); free_ptr(sc.vals_tmp); free_ptr(sc.counts); free_ptr(sc.totals);
     return rc;
-}
+} CUCO_CATCH
 
 int cuco_get_sorted(cuco_engine *e, uint32_t *keys, uint32_t *index)
-{
+try {
     CHECK_ENGINE(e);
     TRY(check_ready(e));
     if (e->sorted_mode < 0) {
@@ -1752,10 +1800,10 @@ int cuco_get_sorted(cuco_engine *e, uint32_t *keys, uint32_t *index)
     if (index) CUCO_HIP(hipMemcpyAsync(index, e->perm, sizeof(uint32_t) * e->n, hipMemcpyDeviceToHost, e->stream));
     CUCO_HIP(hipStreamSynchronize(e->stream));
     return CUCO_OK;
-}
+} CUCO_CATCH
 
 int cuco_get_cell_table(cuco_engine *e, uint32_t *cell_start, uint32_t *cell_end)
-{
+try {
     CHECK_ENGINE(e);
     TRY(check_ready(e));
     if (e->sorted_mode != kKeyXMajor) {
@@ -1776,7 +1824,7 @@ int cuco_get_cell_table(cuco_engine *e, uint32_t *cell_start, uint32_t *cell_end
     free_ptr(ds);
     free_ptr(de);
     return rc;
-}
+} CUCO_CATCH
 
 int cuco_get_shortrange_table(cuco_engine *e, float *table)
 {
@@ -1786,7 +1834,7 @@ int cuco_get_shortrange_table(cuco_engine *e, float *table)
 }
 
 int cuco_get_pm_grid(cuco_engine *e, int32_t which, double *out)
-{
+try {
     CHECK_ENGINE(e);
     if (e->pm_prec < 0 || (which != 0 && which != 1) || !out) {
         set_error("no PM evaluation, or which not in {0,1}");
@@ -1804,7 +1852,7 @@ int cuco_get_pm_grid(cuco_engine *e, int32_t which, double *out)
         for (size_t i = 0; i < M3; ++i) out[i] = h[i];
     }
     return CUCO_OK;
-}
+} CUCO_CATCH
 
 int cuco_count_pairs(cuco_engine *e, uint64_t *pairs_in, uint64_t *fp32_candidates)
 {
@@ -1867,7 +1915,7 @@ int cuco_set_tuning(cuco_engine *e, const cuco_tuning *t)
 }
 
 int cuco_autotune(cuco_engine *e, cuco_tuning *chosen, double *best_ms)
-{
+try {
     CHECK_ENGINE(e);
     TRY(check_ready(e));
     hipEvent_t a, b;
@@ -1946,7 +1994,7 @@ int cuco_autotune(cuco_engine *e, cuco_tuning *chosen, double *best_ms)
     if (chosen) *chosen = best;
     if (best_ms) *best_ms = best_t;
     return CUCO_OK;
-}
+} CUCO_CATCH
 
 int cuco_set_timing(cuco_engine *e, int32_t enable)
 {

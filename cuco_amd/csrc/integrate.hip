@@ -1,5 +1,5 @@
-// Device-resident drift and kick (predict.c:10-41, timestep.c:223-226 and
-// :253-255).  Positions and velocities stay FP64 on the device.  Compiled
+// Device-resident drift and kick (predict.c:10-41, timestep.c:51-59 and
+// :86-92).  Positions and velocities stay FP64 on the device.  Compiled
 // with -ffp-contract=off: x += v*d and v += a*k round exactly as the C code.
 //
 // The *_table variants read their factor from a device array indexed by a

@@ -156,4 +156,25 @@ int32_t cuco_find_next_output_time(const double *times, int32_t count,
     return ti_next;
 }
 
+int cuco_init_records(cuco_particle *P, int64_t n, double time_begin)
+{
+    if ((!P && n > 0) || n < 0 || !(time_begin > 0)) {
+        set_error("null particle array, n < 0 or time_begin <= 0");
+        return CUCO_ERR_INVALID_ARG;
+    }
+    // init.c:27-36: the IC holds peculiar velocities; the run integrates
+    // a^(3/2) times them.  Vel is float, the factor double (float *= double).
+    const double fac = sqrt(time_begin) * time_begin;
+    for (int64_t i = 0; i < n; ++i) {
+        for (int j = 0; j < 3; ++j) {
+            P[i].Vel[j] = (float)(P[i].Vel[j] * fac);
+            P[i].GravAccel[j] = 0;
+            P[i].GravPM[j] = 0;
+        }
+        P[i].Ti_endstep = 0;
+        P[i].Ti_begstep = 0;
+    }
+    return CUCO_OK;
+}
+
 }  // extern "C"

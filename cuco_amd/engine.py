@@ -10,7 +10,7 @@ The names mirror the reference host interface of the gravity path:
 * ``Engine.compute_accelerations`` -- accel.c:8-23
 * ``Engine.move_particles``  -- move_particles + do_box_wrapping (predict.c)
 * ``Engine.kick``            -- the kicks of advance_and_find_timesteps
-                                (timestep.c:223-226, 253-255)
+                                (timestep.c:51-59, 86-92)
 
 Errors surface as :class:`CucoError` carrying the library's status and
 message; nothing falls back to a CPU path.  If the shared library is absent
@@ -49,7 +49,7 @@ EXPORTED = [
     "cuco_ensemble_create", "cuco_ensemble_size", "cuco_write_snapshot",
     "cuco_read_snapshot", "cuco_read_output_list", "cuco_find_next_output_time",
     "cuco_download_records", "cuco_upload_records", "cuco_timeline_init",
-    "cuco_timeline_run", "cuco_get_timesteps",
+    "cuco_timeline_run", "cuco_get_timesteps", "cuco_init_records",
 ]
 
 
@@ -157,6 +157,7 @@ def lib():
         "cuco_timeline_run": (C.c_int, [vp, i32, i32, i32, C.POINTER(TimelineState)]),
         "cuco_get_timesteps": (C.c_int, [vp, C.POINTER(i32), C.POINTER(i32)]),
         "cuco_upload_records": (C.c_int, [vp, vp, i64]),
+        "cuco_init_records": (C.c_int, [vp, i64, C.c_double]),
         "cuco_upload_particles": (C.c_int, [vp, i64, dp, dp, dp, C.c_double]),
         "cuco_download_particles": (C.c_int, [vp, dp, dp]),
         "cuco_download_accel": (C.c_int, [vp, dp, dp]),
@@ -272,6 +273,15 @@ def read_snapshot(path: str, header_only: bool = False):
                                     P.ctypes.data_as(C.c_void_p), P.shape[0]),
            "cuco_read_snapshot")
     return h, P
+
+
+def init_records(P: np.ndarray, time_begin: float) -> None:
+    """The particle part of init() (init.c:27-36), in place: Vel *= a^(3/2)
+    at a = time_begin; GravAccel, GravPM and the Ti fields set to 0."""
+    if P.dtype != PARTICLE_DTYPE or not P.flags.c_contiguous:
+        raise ValueError("expected a contiguous PARTICLE array")
+    _check(lib().cuco_init_records(P.ctypes.data_as(C.c_void_p), P.shape[0],
+                                   time_begin), "cuco_init_records")
 
 
 def read_output_list(path: str, max_times: int = 500) -> np.ndarray:
@@ -515,6 +525,7 @@ class Engine:
         _check(lib().cuco_pmforce_periodic(self.h, P.ctypes.data_as(C.c_void_p),
                                            P.shape[0], precision),
                "cuco_pmforce_periodic")
+        self.n = P.shape[0]
 
     def gravity_particles(self, P: np.ndarray, precision: int = FP64) -> None:
         """gravity_linklist_cuda() on a PARTICLE array (writes GravAccel)."""
@@ -522,6 +533,7 @@ class Engine:
         _check(lib().cuco_gravity_linklist(self.h, P.ctypes.data_as(C.c_void_p),
                                            P.shape[0], precision),
                "cuco_gravity_linklist")
+        self.n = P.shape[0]
 
     # -------------------------------------------------------- inspection
     def sorted_order(self):

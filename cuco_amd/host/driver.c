@@ -18,8 +18,10 @@
  *
  * Units and cosmology are the reference's (begrun.c:33-70): kpc/h,
  * 1e10 Msun/h, km/s, Omega0 = 0.3, OmegaLambda = 0.7, h = 0.7.  The IC
- * velocities are taken as internal velocities (init.c:40-44 already
- * applied).  Exit status 0 on success; every engine error is printed. */
+ * holds peculiar velocities: run mode applies init()'s Vel *= a^(3/2) at
+ * a_begin (init.c:27-36) before the first step, as the reference does;
+ * forces mode does not use velocities.  Exit status 0 on success; every
+ * engine error is printed. */
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -144,6 +146,7 @@ static int run(int argc, char **argv)
     All.SofteningHalo = t.softening_halo;
     All.SofteningHaloMaxPhys = t.softening_halo_max_phys;
     All.Time = t.time_begin;
+    check(cuco_init_records(P, NumPart, All.Time), "init"); /* init.c:27-36 */
     linklist_init();
     check(cuco_upload_records(engine, P, NumPart), "upload");
     check(cuco_timeline_init(engine, &t), "init");

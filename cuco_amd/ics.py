@@ -2,7 +2,7 @@
 
 Units are the reference's (begrun.c:33-35,65-70): kpc/h, 1e10 Msun/h, km/s,
 so G = 43007.1 and H0 = 0.1 internal.  Internal velocities are a^2 dx/dt,
-the quantity the reference holds after init.c:133 (Vel *= a^(3/2)).
+the quantity the reference holds after init.c:30 (Vel *= a^(3/2)).
 
 Workloads (lengths quoted in Mpc/h, stored in kpc/h):
 

@@ -195,7 +195,7 @@ int cuco_autotune(cuco_engine *e, cuco_tuning *chosen, double *best_ms);
 
 /* move_particles + do_box_wrapping (predict.c:10-41): x += v*drift; wrap. */
 int cuco_drift(cuco_engine *e, double drift_factor);
-/* Kicks of advance_and_find_timesteps (timestep.c:223-226, 253-255):
+/* Kicks of advance_and_find_timesteps (timestep.c:51-59, 86-92):
  * v += a_pp*kick_pp; v += a_pm*kick_pm, for all particles. */
 int cuco_kick(cuco_engine *e, double kick_pp, double kick_pm);
 /* nsteps global KDK steps; factors is nsteps*3 (k1, drift, k2) per step:
@@ -224,7 +224,7 @@ double cuco_table_factor(const double *table, double a_begin, double a_max,
                          double loga0, double loga1);
 /* Factors of nsteps global KDK steps uniform in ln a from a_start to a_end:
  * per step (kick to the midpoint, drift over the step, kick from the
- * midpoint), the kicks of timestep.c:215-218 for a global step. */
+ * midpoint), the kicks of timestep.c:48-59 for a global step. */
 int cuco_kdk_factors(const double *drift, const double *kick, double a_begin,
                      double a_max, double a_start, double a_end, int32_t nsteps,
                      double *factors);
@@ -232,6 +232,12 @@ int cuco_kdk_factors(const double *drift, const double *kick, double a_begin,
 /* ---------------------------------------------------------------- */
 /* Compatibility shims on PARTICLE-layout arrays                     */
 /* ---------------------------------------------------------------- */
+
+/* The two shims take n in [1, capacity] (an ensemble: exactly its
+ * capacity) and upload positions and masses only: when the handle already
+ * holds n particles, their device velocities and integer-timeline state are
+ * kept (as the reference's copyPosToDevice leaves velocities on the host);
+ * otherwise the device velocities start at zero. */
 
 /* pmforce_periodic() on a caller's P[] (pm_periodic.c:67-314): uploads
  * Pos/Mass, runs the PM force, writes P[i].GravPM. */
@@ -324,11 +330,20 @@ int32_t cuco_find_next_output_time(const double *times, int32_t count,
 /* The handle's particles as PARTICLE records in original order (P[] of
  * savepositions): Pos, Mass, internal Vel, the last GravAccel (PP) and
  * GravPM, and the integer timeline's Ti_endstep/Ti_begstep (0 before
- * cuco_timeline_init). */
+ * cuco_timeline_init).  After cuco_timeline_run, GravAccel is current for
+ * the particles active on the last step only (the ones it kicked); an
+ * inactive particle's value is from its own last active step or from the
+ * last whole evaluation, whichever came later (the reference recomputes
+ * every particle's GravAccel each step, accel.c:22, but kicks only the
+ * active ones, so the dynamics are the same). */
 int cuco_download_records(cuco_engine *e, cuco_particle *P);
 /* Uploads Pos, Vel (internal velocities, as P[] holds them after init.c:
- * 40-44) and Mass of n PARTICLE records; as cuco_upload_particles. */
+ * 27-36) and Mass of n PARTICLE records; as cuco_upload_particles. */
 int cuco_upload_records(cuco_engine *e, const cuco_particle *P, int64_t n);
+/* The particle part of init() (init.c:27-36), host only: Vel *= a^(3/2)
+ * with a = time_begin (IC files hold peculiar velocities; the run holds
+ * a^2 dx/dt), GravAccel, GravPM, Ti_endstep and Ti_begstep set to 0. */
+int cuco_init_records(cuco_particle *P, int64_t n, double time_begin);
 
 /* ---------------------------------------------------------------- */
 /* Device-pointer API (mirrors proto.cuh:26-58)                      */

@@ -741,8 +741,8 @@ int orc_kdk_run(const orc_params *p, double *pos, double *vel,
     for (int32_t s = 0; s < nsteps && rc == 0; ++s) {
         double k1 = factors[3 * s + 0], d = factors[3 * s + 1], k2 = factors[3 * s + 2];
         for (int64_t i = 0; i < 3 * n; ++i) {
-            vel[i] += acc_pp[i] * k1; /* timestep.c:223-226 */
-            vel[i] += acc_pm[i] * k1; /* timestep.c:253-255 */
+            vel[i] += acc_pp[i] * k1; /* timestep.c:56-59 */
+            vel[i] += acc_pm[i] * k1; /* timestep.c:86-92 */
         }
         for (int64_t i = 0; i < 3 * n; ++i) pos[i] += vel[i] * d; /* predict.c:17-21 */
         wrap_positions(pos, n, p->L);

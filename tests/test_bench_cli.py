@@ -25,3 +25,24 @@ def test_traffic_figures_come_from_the_committed_profile():
     # each-pair-once variant: the fixed-point j-side atomics dominate
     assert 1e10 < plain < sym < 1e12
     assert bench.pmc_traffic("pp32", "no_such_workload") is None
+
+
+def test_compulsory_stage_bytes_at_256_cubed():
+    """The HBM roofline's byte counts for 256^3 particles on a 512^3 mesh:
+    the three forward axis passes (z rows, y lines, and the x pass that also
+    holds the Green's function and the inverse x transform) move
+    (R + C) + 2C + 2C, R = 512^3 floats, C = 512^2*257 complex floats."""
+    N, M = 256 ** 3, 512
+    kb = bench.morton_bits(M)
+    assert kb == 27
+    b = bench.stage_bytes(N, M, kb, 1 << kb)
+    R, C = 4 * M ** 3, 8 * M * M * (M // 2 + 1)
+    assert b["fft_fwd"] + b["green_x"] == (R + C) + 2 * C + 2 * C
+    assert abs(b["fft_fwd"] + b["green_x"] - 3.23e9) < 0.01e9
+    assert b["fft_inv"] == b["fft_fwd"]
+    assert b["sort"] == 4 * 20 * N
+    assert b["deposit"] == 20 * N + R
+    r = bench.hbm_roofline({"deposit": 1.0, "interp": 0.0}, b, ["deposit", "interp"])
+    assert set(r) == {"deposit", "total"}
+    assert abs(r["deposit"]["achieved"] - b["deposit"] / 1e-3 / 1e9) < 1e-6
+    assert abs(r["deposit"]["frac"] - r["deposit"]["achieved"] / 8000.0) < 1e-12

@@ -81,7 +81,12 @@ def test_run_mode_writes_the_scheduled_snapshots(tmp_path):
     eng = E.Engine(pm_grid=box.M, box_size=L, G=ics.G_INTERNAL,
                    asmth=1.25 * L / box.M, rcut=4.5 * (1.25 * L / box.M),
                    soft_h=2.8 * soft, max_particles=box.N)
-    eng.upload_records(P)
+    # The driver applies init()'s velocity rescale (init.c:27-36) to the
+    # IC's peculiar velocities; the binding side does the same host step.
+    R0 = P.copy()
+    E.init_records(R0, 0.2)
+    assert not np.array_equal(R0["Vel"], P["Vel"])
+    eng.upload_records(R0)
     tl = dict(time_begin=0.2, time_max=0.3, omega0=0.3, omega_lambda=0.7,
               hubble=ics.HUBBLE_INTERNAL, err_tol_int_accuracy=0.025,
               max_size_timestep=max_dt, min_size_timestep=0.0,

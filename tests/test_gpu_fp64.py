@@ -151,6 +151,30 @@ def test_particle_record_shims(lattice, orc):
     assert (np.linalg.norm(P["GravPM"] - ref_pm, axis=1) / tot).max() < 1e-6
 
 
+def test_shims_keep_device_velocities(lattice):
+    """The PARTICLE shims upload positions and masses only: with the same
+    particle count the device velocities stay; a different count starts
+    them at zero; a count beyond the capacity is an error, not a crash."""
+    box, pos, vel, mass = lattice
+    eng = make_engine(box)
+    eng.upload(pos, vel, mass)
+    P = np.zeros(box.N, dtype=E.PARTICLE_DTYPE)
+    P["Pos"] = pos.astype(np.float32)
+    P["Mass"] = mass.astype(np.float32)
+    eng.gravity_particles(P, E.FP64)
+    eng.pmforce_particles(P, E.FP64)
+    _, v = eng.download_particles()
+    assert np.array_equal(v, vel)
+    Q = P[:100].copy()
+    eng.gravity_particles(Q, E.FP64)
+    _, v = eng.download_particles()
+    assert v.shape == (100, 3) and not v.any()
+    with pytest.raises(E.CucoError) as ei:
+        eng.gravity_particles(np.zeros(box.N + 1, dtype=E.PARTICLE_DTYPE), E.FP64)
+    assert ei.value.status == 1
+    eng.close()
+
+
 def test_errors_are_reported_not_fatal(lattice):
     box, pos, vel, mass = lattice
     eng = make_engine(box, capacity=100)

@@ -1,0 +1,163 @@
+"""Full-size parity at the three large workloads.
+
+* halo256: 256^3 particles, 512^3 mesh, 500 Mpc/h, 2000 NFW halos + uniform
+  (the workload the throughput is quoted on).  FP64: sorted cell order
+  identical to the oracle's for all 16.8M particles, PP bit-identical on
+  20000 sampled targets, PM within 1e-10 of |a_total| on every particle.
+  FP32: median / 99th percentile / max of the per-particle error against the
+  FP64 path, for PP, PM and the total force (denominator |a_total|).
+* ensemble64: the 64 boxes of 32^3..96^3 particles (seed 4004) in one
+  handle.  One box of each of the five sizes: batched forces (FP64 and
+  FP32) and a two-step graph-replayed FP32 KDK trajectory bit-identical to a
+  single-box handle, FP64 against the oracle (PP bit for bit, PM 1e-10),
+  FP32 error percentiles against FP64.
+* halo512: 512^3 particles, 1024^3 mesh, 1000 Mpc/h, 20000 halos.  FP64 PP
+  bit-identical to the oracle on 5000 sampled targets; FP32 error
+  percentiles against the FP64 path.
+
+FP32 bar: 99th percentile <= 1e-3 for PP, PM and total.  The oracle is the
+checker only (FP64 C restatement of gravlist.cpp:216-338 and
+pm_periodic.c:67-314)."""
+import numpy as np
+import pytest
+
+from cuco_amd import engine as E
+from cuco_amd import ics
+
+from helpers import make_engine, oracle_params, percentiles, rel_err
+
+pytestmark = pytest.mark.gpu
+
+FP32_P99 = 1e-3
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def fp32_errors(app, apm, a32, p32):
+    tot = app + apm
+    err = {
+        "pp": percentiles(rel_err(a32, app, tot)),
+        "pm": percentiles(rel_err(p32, apm, tot)),
+        "total": percentiles(rel_err(a32 + p32, tot, tot)),
+    }
+    for k, v in err.items():
+        assert v["p99"] <= FP32_P99, (k, v)
+    return err
+
+
+@pytest.mark.timeout(900)
+def test_halo256_full_size(orc):
+    box = ics.WORKLOADS["halo256"]
+    say(f"halo256: N={box.N} M={box.M}")
+    pos, vel, mass = ics.generate(box)
+    eng = make_engine(box)
+    eng.upload(pos, None, mass)
+    eng.compute_accelerations(E.FP64, True)
+    app, apm = eng.accelerations()
+    keys, idx = eng.sorted_order()
+    p = oracle_params(box)
+    ok, oi = orc.calc_keys(p, pos)
+    ok, oi = orc.sort_pairs(ok, oi)
+    assert np.array_equal(keys, ok) and np.array_equal(idx, oi), "sorted cell order"
+    del ok, oi, keys, idx
+    rng = np.random.default_rng(3003)
+    targets = np.sort(rng.choice(box.N, 20000, replace=False))
+    ref_pp, _ = orc.pp_force(p, pos, mass, targets)
+    assert np.array_equal(app[targets], ref_pp), "FP64 PP bit-identical"
+    ref_pm = orc.pm_force(p, pos, mass)
+    pm_err = rel_err(apm, ref_pm, app + apm).max()
+    say(f"halo256 FP64: order identical, PP bit-identical on 20000 targets, "
+        f"PM max rel {pm_err:.2e}")
+    assert pm_err <= 1e-10
+    del ref_pm
+    eng.compute_accelerations(E.FP32, True)
+    a32, p32 = eng.accelerations()
+    err = fp32_errors(app, apm, a32, p32)
+    say(f"halo256 FP32 vs FP64: {err}")
+    eng.close()
+
+
+def _ensemble_representatives(boxes):
+    """First box of each particle count."""
+    rep = {}
+    for b, box in enumerate(boxes):
+        rep.setdefault(box.n_side, b)
+    return [rep[s] for s in sorted(rep)]
+
+
+@pytest.mark.timeout(900)
+def test_ensemble64_full_size(orc):
+    boxes = ics.ensemble_boxes(64)
+    data = [ics.generate(b) for b in boxes]
+    reps = _ensemble_representatives(boxes)
+    assert sorted({boxes[b].n_side for b in reps}) == [32, 48, 64, 80, 96]
+    say(f"ensemble64: {len(boxes)} boxes, {sum(b.N for b in boxes)} particles, "
+        f"checked boxes {reps}")
+    pos = np.concatenate([d[0] for d in data])
+    vel = np.concatenate([d[1] for d in data])
+    mass = np.concatenate([d[2] for d in data])
+    ens = E.Ensemble([b.engine_kwargs() for b in boxes], [b.N for b in boxes])
+    off = np.concatenate([[0], np.cumsum([b.N for b in boxes])])
+    ens.upload(pos, vel, mass)
+    ens.compute_accelerations(E.FP64, True)
+    e64 = ens.accelerations()
+    ens.compute_accelerations(E.FP32, True)
+    e32 = ens.accelerations()
+    b0 = boxes[0]
+    d, k = E.init_drift_table(b0.a_start, b0.a_end, ics.OMEGA_M, ics.OMEGA_L,
+                              ics.HUBBLE_INTERNAL)
+    f = E.kdk_factors(d, k, b0.a_start, b0.a_end, b0.a_start, b0.a_end, 20)[:2]
+    ens.kdk_steps(f, E.FP32, use_graph=True)
+    epos, evel = ens.download_particles()
+    ens.close()
+    for b in reps:
+        box = boxes[b]
+        s = slice(off[b], off[b + 1])
+        bp, bv, bm = data[b]
+        eng = make_engine(box)
+        eng.upload(bp, bv, bm)
+        eng.compute_accelerations(E.FP64, True)
+        app, apm = eng.accelerations()
+        assert np.array_equal(app, e64[0][s]) and np.array_equal(apm, e64[1][s]), b
+        eng.compute_accelerations(E.FP32, True)
+        a32, p32 = eng.accelerations()
+        assert np.array_equal(a32, e32[0][s]) and np.array_equal(p32, e32[1][s]), b
+        eng.kdk_steps(f, E.FP32, use_graph=True)
+        sp, sv = eng.download_particles()
+        assert np.array_equal(sp, epos[s]) and np.array_equal(sv, evel[s]), b
+        eng.close()
+        p = oracle_params(box)
+        ref_pp, _ = orc.pp_force(p, bp, bm)
+        assert np.array_equal(app, ref_pp), b
+        ref_pm = orc.pm_force(p, bp, bm)
+        pm_err = rel_err(apm, ref_pm, ref_pp + ref_pm).max()
+        assert pm_err <= 1e-10, (b, pm_err)
+        err = fp32_errors(app, apm, a32, p32)
+        say(f"ensemble box {b} ({box.n_side}^3, mesh {box.M}): batched == single "
+            f"(forces, 2 KDK steps); FP64 PP bit-identical, PM {pm_err:.2e}; "
+            f"FP32 vs FP64 {err}")
+
+
+@pytest.mark.timeout(900)
+def test_halo512_full_size(orc):
+    box = ics.WORKLOADS["halo512"]
+    say(f"halo512: N={box.N} M={box.M}")
+    pos, vel, mass = ics.generate(box)
+    del vel
+    eng = make_engine(box)
+    eng.upload(pos, None, mass)
+    eng.compute_accelerations(E.FP64, True)
+    app, apm = eng.accelerations()
+    eng.compute_accelerations(E.FP32, True)
+    a32, p32 = eng.accelerations()
+    eng.close()
+    err = fp32_errors(app, apm, a32, p32)
+    say(f"halo512 FP32 vs FP64: {err}")
+    del a32, p32, apm
+    rng = np.random.default_rng(5006)
+    targets = np.sort(rng.choice(box.N, 5000, replace=False))
+    ref_pp, _ = orc.pp_force(oracle_params(box), pos, mass, targets)
+    assert np.array_equal(app[targets], ref_pp), "FP64 PP bit-identical"
+    say("halo512 FP64: PP bit-identical on 5000 targets")

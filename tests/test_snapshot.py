@@ -116,3 +116,26 @@ def test_next_output_time_brackets_each_listed_time():
     assert E.find_next_output_time(sh, a0, a1, dti, seen[2]) == seen[3]
     # times outside [TimeBegin, TimeMax] are ignored
     assert E.find_next_output_time(np.array([0.05, 2.0]), a0, a1, dti, 0) == 2 * E.TIMEBASE
+
+
+def test_init_records_rescales_peculiar_velocities():
+    """init.c:27-36: an IC file's peculiar velocities become a^(3/2) times
+    themselves (float *= double, rounded once to float), and the force and
+    timeline fields start at zero.  Checked against the closed form in
+    float64 and against the rounding of a single product."""
+    P = random_records(1000, seed=11)
+    Q = P.copy()
+    a = 0.02
+    E.init_records(Q, a)
+    fac = math.sqrt(a) * a
+    expect = (P["Vel"].astype(np.float64) * fac).astype(np.float32)
+    assert np.array_equal(Q["Vel"], expect)
+    # within one float rounding of the exact scaled value
+    assert np.allclose(Q["Vel"], P["Vel"].astype(np.float64) * a ** 1.5,
+                       rtol=2 ** -23, atol=0)
+    for f in ("GravAccel", "GravPM", "Ti_endstep", "Ti_begstep"):
+        assert not Q[f].any(), f
+    assert np.array_equal(Q["Pos"], P["Pos"]) and np.array_equal(Q["Mass"], P["Mass"])
+    E.init_records(Q[:0].copy(), a)          # empty array: nothing to do
+    with pytest.raises(E.CucoError):
+        E.init_records(Q, 0.0)
