@@ -97,12 +97,13 @@ def cpu_baseline(box, pos, mass, n_targets, seed=5006, gpu_fp64=None):
     }
 
 
-def ensemble_bench(E, ics, nboxes, steps):
+def ensemble_bench(E, ics, nboxes, steps, tune=True):
     """The 64-box ensemble: FP32 KDK steps (each one hipGraph launch) of all
     boxes batched in one handle against the same boxes run one after
-    another through single-box handles.  Both sides replay an already
-    captured graph from the same initial state; the final states must be
-    bit-identical."""
+    another through single-box handles.  The ensemble handle is autotuned
+    on its own particles and the single-box handles take the same settings.
+    Both sides replay an already captured graph from the same initial
+    state; the final states must be bit-identical."""
     boxes = ics.ensemble_boxes(nboxes)
     t0 = time.perf_counter()
     data = [ics.generate(b) for b in boxes]
@@ -117,6 +118,14 @@ def ensemble_bench(E, ics, nboxes, steps):
     mass = np.concatenate([x[2] for x in data])
 
     ens = E.Ensemble([b.engine_kwargs() for b in boxes], [b.N for b in boxes])
+    tuning = None
+    if tune:
+        ens.upload(pos, vel, mass)
+        t0 = time.perf_counter()
+        tuning, _ = ens.autotune()
+        log(f"[bench] ensemble autotune {tuning} ({time.perf_counter() - t0:.1f}s)")
+    else:
+        tuning = ens.tuning()
 
     def run_batched():
         ens.upload(pos, vel, mass)
@@ -135,6 +144,7 @@ def ensemble_bench(E, ics, nboxes, steps):
     t_looped, identical = 0.0, True
     for b, (box, (p, v, m)) in enumerate(zip(boxes, data)):
         eng = E.Engine(max_particles=box.N, **box.engine_kwargs())
+        eng.set_tuning(**tuning)
         for rep in range(2):            # the first run captures the graph
             eng.upload(p, v, m)
             eng.compute_accelerations(E.FP32, True)
@@ -157,6 +167,7 @@ def ensemble_bench(E, ics, nboxes, steps):
                      f"Zel'dovich ICs; FP32 KDK steps, one hipGraph per step"),
         "n_boxes": len(boxes),
         "n_particles": n_total,
+        "tuning": tuning,
         "steps": steps,
         "batched_ms_per_step": t_batched * 1e3 / steps,
         "looped_ms_per_step": t_looped * 1e3 / steps,
@@ -432,8 +443,9 @@ def main() -> int:
                     help="counter pass: one FP32 evaluation of the workload and "
                          "a 1 GiB device copy of known size, nothing else")
     ap.add_argument("--tuning", default=None,
-                    help="skip the autotuner and use SHIFT,WG,OVERLAP,SYMMETRIC "
-                         "(e.g. 3,8,0,1): a run for a clean kernel trace")
+                    help="skip the autotuner and use SHIFT,WG,OVERLAP,SYMMETRIC"
+                         "[,CELL_LEVEL,FFT_SPLIT] (e.g. 3,8,0,1): a run for a clean "
+                         "kernel trace")
     ap.add_argument("--pp-symmetric", choices=["auto", "0", "1"], default="auto",
                     help="FP32 pair force evaluating each pair once: the "
                          "autotuner's choice, or forced off / on")
@@ -487,9 +499,10 @@ def main() -> int:
     # FP32 chosen path: autotune (outside the timed region), warmup, steps.
     t0 = time.perf_counter()
     if args.tuning:
-        v = [int(x) for x in args.tuning.split(",")]
+        v = [int(x) for x in args.tuning.split(",")] + [0, 0]
         tuning = dict(pp_block_shift=v[0], pp_workgroups_per_cu=v[1],
-                      pm_overlap=v[2], pp_symmetric=v[3])
+                      pm_overlap=v[2], pp_symmetric=v[3], pp_cell_level=v[4],
+                      fft_split=v[5])
         eng.set_tuning(**tuning)
         log(f"[bench] tuning {tuning} (given)")
     else:

@@ -130,6 +130,7 @@ struct cuco_engine {
     PairSym psym;  // each-pair-once variant of the FP32 pair force
     int pp32_workgroups = 0;
     int pp32_wg_per_cu = 8;
+    int pp_cell_level = 0;
     int ncu = 256;
     int min_block_shift = 2;
 
@@ -144,6 +145,7 @@ struct cuco_engine {
     int32_t *ti_scalars = nullptr;  // [0] min Ti_endstep, [1] error flags, [2] active count
     unsigned long long *tl_work = nullptr;  // pair-work estimate: all, active
     double *kick_table = nullptr;
+    double *drift_table = nullptr;
     bool tl_init = false;
     cuco_timeline tl{};
     std::vector<double> drift_tab, kick_tab;
@@ -352,7 +354,7 @@ int update_boxes(cuco_engine *e)
         uint32_t *mp = e->hmaps.data();
         mp[kMapPart * stride + b + 1] = (h.n + 255) / 256;
         for (int map = kMapGatherX; map < kMapCount; ++map)
-            mp[map * stride + b + 1] = pm_blocks(M, map);
+            mp[map * stride + b + 1] = pm_blocks(M, map, e->bs.fft_split);
         p0 += h.n;
         wave0 += (h.cap + 63) / 64;
         grid0 += (uint64_t)M * M * M;
@@ -457,6 +459,7 @@ int ensure_pm(cuco_engine *e)
     TRY(dev_alloc(e->phi[k], sizeof(T) * e->grid_elems));
     TRY(dev_alloc(e->cgrid[k], sizeof(Cx<T>) * e->cgrid_elems));
     TRY(dev_alloc(e->dep_scratch[k], pm_deposit_scratch_bytes(e->bs, k == 0)));
+    if (k == 1) TRY(pm_deposit_scratch_init(e->bs, e->dep_scratch[k], e->stream));
     return make_pm_tables<T>(e, e->tb[k]);
 }
 
@@ -568,7 +571,8 @@ int do_pm_t(cuco_engine *e, int precision, hipStream_t st)
     TRY(pm_fft_inverse<T>(c, phi, e->bs, e->tb[k], st));
     TRY(stage_stop(e, CUCO_STAGE_FFT_INV, st));
     TRY(stage_start(e, CUCO_STAGE_INTERP, st));
-    TRY(pm_interp<T>(phi, spos, e->skeys, e->perm, e->off, e->bs, mode, acc, st));
+    TRY(pm_interp<T>(phi, spos, e->skeys, e->perm, e->off, e->bs, mode, e->dep_scratch[k],
+                     acc, st));
     TRY(stage_stop(e, CUCO_STAGE_INTERP, st));
     e->pm_prec = precision;
     return CUCO_OK;
@@ -1008,6 +1012,7 @@ int cuco_destroy(cuco_engine *e)
     if (e->stream2) (void)hipStreamSynchronize(e->stream2);
     free_ptr(e->ti_beg); free_ptr(e->ti_end);
     free_ptr(e->ti_scalars); free_ptr(e->tl_work); free_ptr(e->kick_table);
+    free_ptr(e->drift_table);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     if (e->stream2) (void)hipStreamDestroy(e->stream2);
     if (e->ev_fork) (void)hipEventDestroy(e->ev_fork);
@@ -1532,6 +1537,7 @@ try {
         TRY(dev_alloc(e->ti_scalars, sizeof(int32_t) * 4));
         TRY(dev_alloc(e->tl_work, sizeof(unsigned long long) * 2));
         TRY(dev_alloc(e->kick_table, sizeof(double) * kDriftTable));
+        TRY(dev_alloc(e->drift_table, sizeof(double) * kDriftTable));
     }
     if (!e->atiles.scan) {
         // Tiles of the active targets (FP32 active-set pair force).
@@ -1551,12 +1557,16 @@ try {
         TRY(dev_alloc(a.t.scan_scratch, sizeof(uint32_t) * scan_scratch_words(a.t.nblocks)));
     }
     e->tl = *t;
+    // The tables are built on the device; the kick table stays there for
+    // the per-particle kicks, and host copies serve the drift factors.
     e->drift_tab.assign(kDriftTable, 0.0);
     e->kick_tab.assign(kDriftTable, 0.0);
-    TRY(cuco_init_drift_table(t->time_begin, t->time_max, t->omega0, t->omega_lambda,
-                              t->hubble, e->drift_tab.data(), e->kick_tab.data()));
-    CUCO_HIP(hipMemcpy(e->kick_table, e->kick_tab.data(), sizeof(double) * kDriftTable,
-                       hipMemcpyHostToDevice));
+    TRY(launch_drift_tables(t->time_begin, t->time_max, t->omega0, t->omega_lambda, t->hubble,
+                            e->drift_table, e->kick_table, e->stream));
+    CUCO_HIP(hipMemcpyAsync(e->drift_tab.data(), e->drift_table, sizeof(double) * kDriftTable,
+                            hipMemcpyDeviceToHost, e->stream));
+    CUCO_HIP(hipMemcpyAsync(e->kick_tab.data(), e->kick_table, sizeof(double) * kDriftTable,
+                            hipMemcpyDeviceToHost, e->stream));
     e->tl_log_begin = log(t->time_begin);
     e->tl_log_max = log(t->time_max);
     e->tl_interval = (log(t->time_max) - log(t->time_begin)) / kTimebase;  // init.c:23
@@ -1788,6 +1798,29 @@ try {
     return rc;
 } CUCO_CATCH
 
+int cuco_device_drift_table(int32_t device, double a_begin, double a_max, double omega0,
+                            double omega_lambda, double hubble, double *drift, double *kick)
+{
+    if (!(a_begin > 0) || !(a_max > a_begin) || !(hubble > 0) || !drift || !kick) {
+        set_error("a_begin, a_max, hubble out of range or null tables");
+        return CUCO_ERR_INVALID_ARG;
+    }
+    CUCO_HIP(hipSetDevice(device));
+    double *d = nullptr;
+    TRY(dev_alloc(d, sizeof(double) * 2 * kDriftTable));
+    int rc = launch_drift_tables(a_begin, a_max, omega0, omega_lambda, hubble, d,
+                                 d + kDriftTable, nullptr);
+    if (!rc && (hipMemcpy(drift, d, sizeof(double) * kDriftTable, hipMemcpyDeviceToHost) !=
+                    hipSuccess ||
+                hipMemcpy(kick, d + kDriftTable, sizeof(double) * kDriftTable,
+                          hipMemcpyDeviceToHost) != hipSuccess)) {
+        set_error("copy of the drift tables failed");
+        rc = CUCO_ERR_HIP;
+    }
+    free_ptr(d);
+    return rc;
+}
+
 int cuco_get_sorted(cuco_engine *e, uint32_t *keys, uint32_t *index)
 try {
     CHECK_ENGINE(e);
@@ -1880,6 +1913,8 @@ int cuco_get_tuning(cuco_engine *e, cuco_tuning *t)
     t->pp_workgroups_per_cu = e->pp32_wg_per_cu;
     t->pm_overlap = e->pm_overlap ? 1 : 0;
     t->pp_symmetric = e->psym.on ? 1 : 0;
+    t->pp_cell_level = e->pp_cell_level;
+    t->fft_split = e->bs.fft_split;
     return CUCO_OK;
 }
 
@@ -1889,9 +1924,20 @@ int cuco_set_tuning(cuco_engine *e, const cuco_tuning *t)
     if (!t || t->pp_block_shift < e->min_block_shift || t->pp_block_shift > 4 ||
         t->pp_block_shift >= e->min_bits || t->pp_workgroups_per_cu < 1 ||
         t->pp_workgroups_per_cu > 32 || t->pm_overlap < 0 || t->pm_overlap > 1 ||
-        t->pp_symmetric < 0 || t->pp_symmetric > 1) {
+        t->pp_symmetric < 0 || t->pp_symmetric > 1 || t->pp_cell_level < 0 ||
+        t->pp_cell_level > 2 || t->fft_split < 0 || t->fft_split > 1) {
         set_error("tuning out of range");
         return CUCO_ERR_INVALID_ARG;
+    }
+    if (t->pp_cell_level != e->pp_cell_level) {
+        e->pp_cell_level = t->pp_cell_level;
+        e->tiles.coarse_cells = t->pp_cell_level == 0 ? 4.0f : t->pp_cell_level == 1 ? 1e30f : 0.0f;
+        destroy_graph(e);
+    }
+    if (t->fft_split != e->bs.fft_split) {
+        e->bs.fft_split = t->fft_split;
+        TRY(update_boxes(e));  // the register-FFT block maps
+        destroy_graph(e);
     }
     if (t->pp_block_shift != e->tiles.block_shift) {
         e->tiles.block_shift = t->pp_block_shift;
@@ -1923,13 +1969,13 @@ try {
     CUCO_HIP(hipEventCreate(&b));
     const bool timing = e->timing;
     e->timing = false;
-    cuco_tuning best{e->tiles.block_shift, e->pp32_wg_per_cu, 0, 0};
+    cuco_tuning best{e->tiles.block_shift, e->pp32_wg_per_cu, 0, 0, 0, 0};
     float best_t = 1e30f;
     int rc = CUCO_OK;
     for (int shift = e->min_block_shift; shift <= std::min(4, e->min_bits - 1) && !rc; ++shift) {
         for (int cfg = 0; cfg < 6; ++cfg) {
             const int wg = cfg % 3 == 0 ? 4 : cfg % 3 == 1 ? 8 : 16;
-            cuco_tuning t{shift, wg, 0, cfg / 3};
+            cuco_tuning t{shift, wg, 0, cfg / 3, 0, 0};
             rc = cuco_set_tuning(e, &t);
             if (!rc) rc = sort_particles(e, kKeyMorton);
             float ms = 1e30f;
@@ -1952,6 +1998,59 @@ try {
             if (rc) break;
         }
     }
+    // Candidate enumeration level of the each-pair-once kernel (mesh cells
+    // or 2x2x2-cell blocks for wide tiles, or either for every tile).
+    if (!rc && best.pp_symmetric) {
+        const cuco_tuning base = best;
+        for (int level = 0; level < 3 && !rc; ++level) {
+            cuco_tuning t = base;
+            t.pp_cell_level = level;
+            rc = cuco_set_tuning(e, &t);
+            if (!rc) rc = sort_particles(e, kKeyMorton);
+            float ms = 1e30f;
+            for (int rep = 0; rep < 2 && !rc; ++rep) {
+                if (hipEventRecord(a, e->stream) != hipSuccess) rc = CUCO_ERR_HIP;
+                e->psym.n_sorted = e->n;
+                if (!rc) rc = launch_pp32(e->spos32, e->skeys, e->perm, e->off, e->bs, e->p32,
+                                          e->table_cut, e->uniform_mass, e->table32, e->tiles,
+                                          e->acc_pp32, nullptr, e->pp32_workgroups,
+                                          nullptr, 0, e->psym, nullptr, e->n, e->stream);
+                if (!rc && (hipEventRecord(b, e->stream) != hipSuccess ||
+                            hipEventSynchronize(b) != hipSuccess)) rc = CUCO_ERR_HIP;
+                float t_ms = 0;
+                if (!rc && hipEventElapsedTime(&t_ms, a, b) == hipSuccess) ms = std::min(ms, t_ms);
+            }
+            if (!rc && ms < best_t) {
+                best_t = ms;
+                best = t;
+            }
+        }
+    }
+    // Register-FFT line split: the FP32 PM chain alone (particles sorted).
+    if (!rc && e->bs.fft32_mask) {
+        const cuco_tuning base = best;
+        float best_pm = 1e30f;
+        for (int split = 0; split < 2 && !rc; ++split) {
+            cuco_tuning t = base;
+            t.fft_split = split;
+            rc = cuco_set_tuning(e, &t);
+            if (!rc) rc = ensure_pm<float>(e);
+            if (!rc) rc = sort_particles(e, kKeyMorton);
+            float ms = 1e30f;
+            for (int rep = 0; rep < 3 && !rc; ++rep) {
+                if (hipEventRecord(a, e->stream) != hipSuccess) rc = CUCO_ERR_HIP;
+                if (!rc) rc = do_pm(e, CUCO_FP32);
+                if (!rc && (hipEventRecord(b, e->stream) != hipSuccess ||
+                            hipEventSynchronize(b) != hipSuccess)) rc = CUCO_ERR_HIP;
+                float t_ms = 0;
+                if (!rc && hipEventElapsedTime(&t_ms, a, b) == hipSuccess) ms = std::min(ms, t_ms);
+            }
+            if (!rc && ms < best_pm * 0.98f) {
+                best_pm = ms;
+                best = t;
+            }
+        }
+    }
     // Whole FP32 evaluations (sort, PM, PP) with and without the PM chain
     // on its own stream; for each overlap setting, the pair kernel is also
     // retimed at 4 and 8 workgroups per CU (fewer resident pair workgroups
@@ -1962,7 +2061,9 @@ try {
         for (int ov = 0; ov < 2 && !rc; ++ov) {
             for (int wg : {best.pp_workgroups_per_cu, 4}) {
                 if (ov == 0 && wg != best.pp_workgroups_per_cu) continue;
-                cuco_tuning t{best.pp_block_shift, wg, ov, best.pp_symmetric};
+                cuco_tuning t = best;
+                t.pp_workgroups_per_cu = wg;
+                t.pm_overlap = ov;
                 rc = cuco_set_tuning(e, &t);
                 float ms = 1e30f;
                 for (int rep = 0; rep < 3 && !rc; ++rep) {

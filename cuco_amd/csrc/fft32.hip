@@ -174,9 +174,11 @@ struct RegDft<4, INV> {
 
 // ------------------------------------------------------------ shapes
 // Threads per line N1 (the rest, N2 = N/N1 >= N1, in registers).
-template <int N>
+// S = 1: half as many threads per line (more values each, more lines per
+// block).
+template <int N, int S = 0>
 struct Shape {
-    static constexpr int N1 = N <= 32 ? 4 : N <= 128 ? 8 : N <= 512 ? 16 : 32;
+    static constexpr int N1 = (N <= 32 ? 4 : N <= 128 ? 8 : N <= 512 ? 16 : 32) >> S;
     static constexpr int N2 = N / N1;
     static constexpr int B = kRegThreads / N1;  // lines per block
     // LDS exchange swizzle: slot k2*N1 + (g ^ ((k2 + Q*l) mod N1)) of line l
@@ -186,11 +188,11 @@ struct Shape {
     static_assert(N2 % N1 == 0, "N2 must be a multiple of N1");
 };
 
-template <int N>
+template <int N, int S>
 __device__ __forceinline__ int xslot(int l, int g, int k2)
 {
-    constexpr int N1 = Shape<N>::N1;
-    return l * N + k2 * N1 + (g ^ ((k2 + Shape<N>::Q * l) & (N1 - 1)));
+    constexpr int N1 = Shape<N, S>::N1;
+    return l * N + k2 * N1 + (g ^ ((k2 + Shape<N, S>::Q * l) & (N1 - 1)));
 }
 
 __device__ __forceinline__ C conj_if(C w, bool inv) { return inv ? C{w.x, -w.y} : w; }
@@ -199,11 +201,11 @@ __device__ __forceinline__ C conj_if(C w, bool inv) { return inv ? C{w.x, -w.y} 
 // Runs the length-N2 DFTs, applies W_N^(g*k2) and writes the partial sums
 // to the exchange slots of line l.  tw: W_N^k, k < N, at stride ts (the
 // box's table holds W_M^k; rows use N = M/2, ts = 2).
-template <int N, bool INV>
+template <int N, bool INV, int S>
 __device__ __forceinline__ void step_a(C *x, C *lds, int l, int g,
                                        const C *__restrict__ tw, int ts)
 {
-    constexpr int N1 = Shape<N>::N1, N2 = Shape<N>::N2;
+    constexpr int N1 = Shape<N, S>::N1, N2 = Shape<N, S>::N2;
     RegDft<N2, INV>::run(x);
 #pragma unroll
     for (int k2 = 1; k2 < N2; ++k2) {
@@ -211,24 +213,24 @@ __device__ __forceinline__ void step_a(C *x, C *lds, int l, int g,
         x[k2] = {x[k2].x * w.x - x[k2].y * w.y, x[k2].x * w.y + x[k2].y * w.x};
     }
 #pragma unroll
-    for (int k2 = 0; k2 < N2; ++k2) lds[xslot<N>(l, g, k2)] = x[k2];
+    for (int k2 = 0; k2 < N2; ++k2) lds[xslot<N, S>(l, g, k2)] = x[k2];
 }
 
 // The full four-step.  Out: y[u*N1 + k1] = X[k2 + N2*k1] with
 // k2 = g + N1*u.  The block synchronises inside.
-template <int N, bool INV>
+template <int N, bool INV, int S>
 __device__ __forceinline__ void four_step(C *x, C *y, C *lds, int l, int g,
                                           const C *__restrict__ tw, int ts)
 {
-    constexpr int N1 = Shape<N>::N1, N2 = Shape<N>::N2;
-    step_a<N, INV>(x, lds, l, g, tw, ts);
+    constexpr int N1 = Shape<N, S>::N1, N2 = Shape<N, S>::N2;
+    step_a<N, INV, S>(x, lds, l, g, tw, ts);
     __syncthreads();
 #pragma unroll
     for (int u = 0; u < N2 / N1; ++u) {
         const int k2 = g + N1 * u;
         C z[N1];
 #pragma unroll
-        for (int gp = 0; gp < N1; ++gp) z[gp] = lds[xslot<N>(l, gp, k2)];
+        for (int gp = 0; gp < N1; ++gp) z[gp] = lds[xslot<N, S>(l, gp, k2)];
         RegDft<N1, INV>::run(z);
 #pragma unroll
         for (int k1 = 0; k1 < N1; ++k1) y[u * N1 + k1] = z[k1];
@@ -245,12 +247,12 @@ __device__ __forceinline__ C *lds_area()
 // ------------------------------------------------------------ z rows
 // Real rows of length M = 2N -> N+1 complex, forward.  Rows mapping: lane
 // group of N1 threads per row.
-template <int N>
+template <int N, int S>
 __global__ __launch_bounds__(kRegThreads) void k_rrows_r2c(const float *__restrict__ in_all,
                                                            C *__restrict__ out_all, BoxView v,
                                                            const C *__restrict__ tw_all)
 {
-    constexpr int N1 = Shape<N>::N1, N2 = Shape<N>::N2, B = Shape<N>::B;
+    constexpr int N1 = Shape<N, S>::N1, N2 = Shape<N, S>::N2, B = Shape<N, S>::B;
     constexpr int M = 2 * N, Zc = N + 1, nrows = M * M;
     int bi;
     uint32_t lb;
@@ -268,7 +270,7 @@ __global__ __launch_bounds__(kRegThreads) void k_rrows_r2c(const float *__restri
 #pragma unroll
     for (int m = 0; m < N2; ++m)
         x[m] = ok ? in[(size_t)row * N + g + N1 * m] : C{0.f, 0.f};
-    four_step<N, false>(x, y, lds, l, g, tw, 2);
+    four_step<N, false, S>(x, y, lds, l, g, tw, 2);
     __syncthreads();
     // Natural order per row, then the split post-process.
     C *rl = lds + l * N;
@@ -291,12 +293,12 @@ __global__ __launch_bounds__(kRegThreads) void k_rrows_r2c(const float *__restri
 }
 
 // N+1 complex -> real rows of length 2N, inverse (unnormalized).
-template <int N>
+template <int N, int S>
 __global__ __launch_bounds__(kRegThreads) void k_rrows_c2r(const C *__restrict__ in_all,
                                                            float *__restrict__ out_all, BoxView v,
                                                            const C *__restrict__ tw_all)
 {
-    constexpr int N1 = Shape<N>::N1, N2 = Shape<N>::N2, B = Shape<N>::B;
+    constexpr int N1 = Shape<N, S>::N1, N2 = Shape<N, S>::N2, B = Shape<N, S>::B;
     constexpr int M = 2 * N, Zc = N + 1, nrows = M * M;
     int bi;
     uint32_t lb;
@@ -330,7 +332,7 @@ __global__ __launch_bounds__(kRegThreads) void k_rrows_c2r(const C *__restrict__
         }
         x[m] = z;
     }
-    four_step<N, true>(x, y, lds, l, g, tw, 2);
+    four_step<N, true, S>(x, y, lds, l, g, tw, 2);
     if (!ok) return;
 #pragma unroll
     for (int u = 0; u < N2 / N1; ++u)
@@ -349,11 +351,11 @@ __device__ __forceinline__ size_t rline_base(int L, int M, int Zc)
     return XAXIS ? (size_t)L : (size_t)(L / Zc) * M * Zc + (L % Zc);
 }
 
-template <int N, bool INV, bool XAXIS>
+template <int N, bool INV, bool XAXIS, int S>
 __global__ __launch_bounds__(kRegThreads) void k_rlines(C *__restrict__ data_all, BoxView v,
                                                         const C *__restrict__ tw_all)
 {
-    constexpr int N1 = Shape<N>::N1, N2 = Shape<N>::N2, B = Shape<N>::B;
+    constexpr int N1 = Shape<N, S>::N1, N2 = Shape<N, S>::N2, B = Shape<N, S>::B;
     constexpr int M = N, Zc = M / 2 + 1, nlines = M * Zc;
     constexpr size_t stride = XAXIS ? (size_t)M * Zc : (size_t)Zc;
     int bi;
@@ -372,7 +374,7 @@ __global__ __launch_bounds__(kRegThreads) void k_rlines(C *__restrict__ data_all
 #pragma unroll
     for (int m = 0; m < N2; ++m)
         x[m] = ok ? data[base + (size_t)(g + N1 * m) * stride] : C{0.f, 0.f};
-    four_step<N, INV>(x, y, lds, l, g, tw, 1);
+    four_step<N, INV, S>(x, y, lds, l, g, tw, 1);
     if (!ok) return;
 #pragma unroll
     for (int u = 0; u < N2 / N1; ++u)
@@ -388,14 +390,14 @@ __global__ __launch_bounds__(kRegThreads) void k_rlines(C *__restrict__ data_all
 // length-N1 DFTs and applies W_N^(-g'*k2) in place -- the same exchange
 // slots it read, so no other thread's slots are touched -- and after one
 // more barrier reads its own g back for the inverse length-N2 DFTs.
-template <int N>
-__global__ __launch_bounds__(kRegThreads, N >= 1024 ? 1 : 2) void k_rgreen(C *__restrict__ data_all, BoxView v,
+template <int N, int S>
+__global__ __launch_bounds__(kRegThreads, (N >= 1024 || (S && N >= 512)) ? 1 : 2) void k_rgreen(C *__restrict__ data_all, BoxView v,
                                                         const C *__restrict__ tw_all,
                                                         const float *__restrict__ gexp_all,
                                                         const float *__restrict__ gsinc_all,
                                                         const float *__restrict__ gprod_all)
 {
-    constexpr int N1 = Shape<N>::N1, N2 = Shape<N>::N2, B = Shape<N>::B;
+    constexpr int N1 = Shape<N, S>::N1, N2 = Shape<N, S>::N2, B = Shape<N, S>::B;
     constexpr int M = N, Zc = M / 2 + 1, nlines = M * Zc;
     constexpr size_t stride = (size_t)M * Zc;
     int bi;
@@ -417,7 +419,7 @@ __global__ __launch_bounds__(kRegThreads, N >= 1024 ? 1 : 2) void k_rgreen(C *__
 #pragma unroll
     for (int m = 0; m < N2; ++m)
         x[m] = ok ? data[(size_t)Lc + (size_t)(g + N1 * m) * stride] : C{0.f, 0.f};
-    step_a<N, false>(x, lds, l, g, tw, 1);
+    step_a<N, false, S>(x, lds, l, g, tw, 1);
     const int yy = Lc / Zc, z = Lc - yy * Zc;
     const int ky = yy > M / 2 ? yy - M : yy;
     // -exp(-k^2 a^2) / sinc^4 / k^2, the y and z factors taken once per line.
@@ -429,7 +431,7 @@ __global__ __launch_bounds__(kRegThreads, N >= 1024 ? 1 : 2) void k_rgreen(C *__
         const int k2 = g + N1 * u;
         C q[N1];
 #pragma unroll
-        for (int gp = 0; gp < N1; ++gp) q[gp] = lds[xslot<N>(l, gp, k2)];
+        for (int gp = 0; gp < N1; ++gp) q[gp] = lds[xslot<N, S>(l, gp, k2)];
         RegDft<N1, false>::run(q);
 #pragma unroll
         for (int k1 = 0; k1 < N1; ++k1) {
@@ -444,13 +446,13 @@ __global__ __launch_bounds__(kRegThreads, N >= 1024 ? 1 : 2) void k_rgreen(C *__
 #pragma unroll
         for (int gp = 0; gp < N1; ++gp) {
             const C w = conj_if(tw[(gp * k2) & (N - 1)], true);
-            lds[xslot<N>(l, gp, k2)] = {q[gp].x * w.x - q[gp].y * w.y,
+            lds[xslot<N, S>(l, gp, k2)] = {q[gp].x * w.x - q[gp].y * w.y,
                                         q[gp].x * w.y + q[gp].y * w.x};
         }
     }
     __syncthreads();
 #pragma unroll
-    for (int k2 = 0; k2 < N2; ++k2) x[k2] = lds[xslot<N>(l, g, k2)];
+    for (int k2 = 0; k2 < N2; ++k2) x[k2] = lds[xslot<N, S>(l, g, k2)];
     RegDft<N2, true>::run(x);
     if (!ok) return;
 #pragma unroll
@@ -467,42 +469,42 @@ int launch_reg(K kernel, size_t lds, unsigned blocks, hipStream_t s, Args... arg
     return CUCO_OK;
 }
 
-template <int M>
+template <int M, int S>
 int forward_m(const float *rho, C *c, const BoxSet &bs, const C *tw, hipStream_t s)
 {
-    int rc = launch_reg(k_rrows_r2c<M / 2>, Shape<M / 2>::lds, bs.total[kMapRowsR], s, rho, c,
-                        bs.view(0, kMapRowsR), tw);
+    int rc = launch_reg(k_rrows_r2c<M / 2, S>, Shape<M / 2, S>::lds, bs.total[kMapRowsR], s,
+                        rho, c, bs.view(0, kMapRowsR), tw);
     if (rc) return rc;
-    return launch_reg(k_rlines<M, false, false>, Shape<M>::lds, bs.total[kMapLinesR], s, c,
-                      bs.view(0, kMapLinesR), tw);
+    return launch_reg(k_rlines<M, false, false, S>, Shape<M, S>::lds, bs.total[kMapLinesR], s,
+                      c, bs.view(0, kMapLinesR), tw);
 }
 
-template <int M>
+template <int M, int S>
 int inverse_m(C *c, float *phi, const BoxSet &bs, const C *tw, hipStream_t s)
 {
-    int rc = launch_reg(k_rlines<M, true, false>, Shape<M>::lds, bs.total[kMapLinesR], s, c,
-                        bs.view(0, kMapLinesR), tw);
+    int rc = launch_reg(k_rlines<M, true, false, S>, Shape<M, S>::lds, bs.total[kMapLinesR], s,
+                        c, bs.view(0, kMapLinesR), tw);
     if (rc) return rc;
-    return launch_reg(k_rrows_c2r<M / 2>, Shape<M / 2>::lds, bs.total[kMapRowsR], s,
+    return launch_reg(k_rrows_c2r<M / 2, S>, Shape<M / 2, S>::lds, bs.total[kMapRowsR], s,
                       (const C *)c, phi, bs.view(0, kMapRowsR), tw);
 }
 
-template <int M>
+template <int M, int S>
 int green_m(C *c, const BoxSet &bs, const PmTables &tb, hipStream_t s)
 {
-    return launch_reg(k_rgreen<M>, Shape<M>::lds, bs.total[kMapLinesR], s, c,
+    return launch_reg(k_rgreen<M, S>, Shape<M, S>::lds, bs.total[kMapLinesR], s, c,
                       bs.view(0, kMapLinesR), static_cast<const C *>(tb.tw),
                       static_cast<const float *>(tb.gexp), static_cast<const float *>(tb.gsinc),
                       static_cast<const float *>(tb.gprod));
 }
 
-template <int M>
+template <int M, int S>
 int x_m(C *c, const BoxSet &bs, bool inverse, const C *tw, hipStream_t s)
 {
     if (inverse)
-        return launch_reg(k_rlines<M, true, true>, Shape<M>::lds, bs.total[kMapLinesR], s, c,
-                          bs.view(0, kMapLinesR), tw);
-    return launch_reg(k_rlines<M, false, true>, Shape<M>::lds, bs.total[kMapLinesR], s, c,
+        return launch_reg(k_rlines<M, true, true, S>, Shape<M, S>::lds, bs.total[kMapLinesR], s,
+                          c, bs.view(0, kMapLinesR), tw);
+    return launch_reg(k_rlines<M, false, true, S>, Shape<M, S>::lds, bs.total[kMapLinesR], s, c,
                       bs.view(0, kMapLinesR), tw);
 }
 
@@ -544,11 +546,11 @@ uint32_t fft32_mask_bit(int M)
     return 1u << i;
 }
 
-uint32_t fft32_blocks(int M, bool rows)
+uint32_t fft32_blocks(int M, bool rows, int split)
 {
     if (!fft32_supported(M)) return 0;
     const int N = rows ? M / 2 : M;
-    const int N1 = N <= 32 ? 4 : N <= 128 ? 8 : N <= 512 ? 16 : 32;
+    const int N1 = (N <= 32 ? 4 : N <= 128 ? 8 : N <= 512 ? 16 : 32) >> (split ? 1 : 0);
     const uint32_t B = (uint32_t)(kRegThreads / N1);
     const uint32_t lines = rows ? (uint32_t)M * M : (uint32_t)M * (M / 2 + 1);
     return (lines + B - 1) / B;
@@ -559,7 +561,7 @@ int fft32_forward(const float *rho, Cx<float> *c, const BoxSet &bs, const PmTabl
 {
     const C *tw = static_cast<const C *>(tb.tw);
     return each_mesh(bs, [&](int M) -> int {
-#define CALL(m) forward_m<m>(rho, c, bs, tw, s)
+#define CALL(m) (bs.fft_split ? forward_m<m, 1>(rho, c, bs, tw, s) : forward_m<m, 0>(rho, c, bs, tw, s))
         CUCO_FFT32_CASES(M, CALL)
 #undef CALL
     });
@@ -570,7 +572,7 @@ int fft32_inverse(Cx<float> *c, float *phi, const BoxSet &bs, const PmTables &tb
 {
     const C *tw = static_cast<const C *>(tb.tw);
     return each_mesh(bs, [&](int M) -> int {
-#define CALL(m) inverse_m<m>(c, phi, bs, tw, s)
+#define CALL(m) (bs.fft_split ? inverse_m<m, 1>(c, phi, bs, tw, s) : inverse_m<m, 0>(c, phi, bs, tw, s))
         CUCO_FFT32_CASES(M, CALL)
 #undef CALL
     });
@@ -579,7 +581,7 @@ int fft32_inverse(Cx<float> *c, float *phi, const BoxSet &bs, const PmTables &tb
 int fft32_green_x(Cx<float> *c, const BoxSet &bs, const PmTables &tb, hipStream_t s)
 {
     return each_mesh(bs, [&](int M) -> int {
-#define CALL(m) green_m<m>(c, bs, tb, s)
+#define CALL(m) (bs.fft_split ? green_m<m, 1>(c, bs, tb, s) : green_m<m, 0>(c, bs, tb, s))
         CUCO_FFT32_CASES(M, CALL)
 #undef CALL
     });
@@ -589,7 +591,7 @@ int fft32_x(Cx<float> *c, const BoxSet &bs, bool inverse, const PmTables &tb, hi
 {
     const C *tw = static_cast<const C *>(tb.tw);
     return each_mesh(bs, [&](int M) -> int {
-#define CALL(m) x_m<m>(c, bs, inverse, tw, s)
+#define CALL(m) (bs.fft_split ? x_m<m, 1>(c, bs, inverse, tw, s) : x_m<m, 0>(c, bs, inverse, tw, s))
         CUCO_FFT32_CASES(M, CALL)
 #undef CALL
     });

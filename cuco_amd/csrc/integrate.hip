@@ -10,6 +10,63 @@ namespace cuco {
 
 namespace {
 
+// ---- drift and kick tables (driftfac.c:14-42) on the device
+// Entry i is the integral from a_begin to a_i = exp(ln a_begin + (ln a_max -
+// ln a_begin)/1000 (i+1)) of da/(H a^3) (drift) and da/(H a^2) (kick),
+// H = hubble sqrt(om/a^3 + (1-om-ol)/a^2 + ol).  Thread i integrates
+// [a_{i-1}, a_i] (a_{-1} = a_begin) with the 16-point Gauss-Legendre rule
+// (exact for polynomials of degree 31; an interval spans a factor
+// (a_max/a_begin)^(1/1000) in a, so the smooth integrand is resolved to
+// rounding), then one thread adds the pieces in table order.
+constexpr double kGl16[8][2] = {  // Legendre roots x > 0 and weights
+    {0.09501250983763745, 0.18945061045506859}, {0.2816035507792589, 0.1826034150449236},
+    {0.45801677765722737, 0.16915651939500262}, {0.6178762444026438, 0.14959598881657676},
+    {0.755404408355003, 0.12462897125553403},   {0.8656312023878318, 0.09515851168249259},
+    {0.9445750230732326, 0.062253523938647706}, {0.9894009349916499, 0.027152459411754037}};
+
+__device__ inline void drift_kick_integrands(double a, double om, double ol, double hubble,
+                                             double &fd, double &fk)
+{
+    // driftfac.c:48-52 and :59-63
+    double h = om / (a * a * a) + (1 - om - ol) / (a * a) + ol;
+    h = hubble * sqrt(h);
+    fd = 1 / (h * a * a * a);
+    fk = 1 / (h * a * a);
+}
+
+__global__ __launch_bounds__(1024) void k_drift_tables(double a_begin, double lb, double lm,
+                                                       double om, double ol, double hubble,
+                                                       double *__restrict__ drift,
+                                                       double *__restrict__ kick)
+{
+    __shared__ double sd[kDriftTable], sk[kDriftTable];
+    for (int i = threadIdx.x; i < kDriftTable; i += blockDim.x) {
+        const double lo = i == 0 ? a_begin : exp(lb + ((lm - lb) / kDriftTable) * i);
+        const double hi = exp(lb + ((lm - lb) / kDriftTable) * (i + 1));
+        const double c = 0.5 * (lo + hi), r = 0.5 * (hi - lo);
+        double d = 0, k = 0;
+        for (int q = 0; q < 8; ++q) {
+            double fd, fk, gd, gk;
+            drift_kick_integrands(c - r * kGl16[q][0], om, ol, hubble, fd, fk);
+            drift_kick_integrands(c + r * kGl16[q][0], om, ol, hubble, gd, gk);
+            d += kGl16[q][1] * (fd + gd);
+            k += kGl16[q][1] * (fk + gk);
+        }
+        sd[i] = d * r;
+        sk[i] = k * r;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double d = 0, k = 0;
+        for (int i = 0; i < kDriftTable; ++i) {
+            d += sd[i];
+            k += sk[i];
+            drift[i] = d;
+            kick[i] = k;
+        }
+    }
+}
+
 // predict.c:31-36.  A particle moves far less than a box length per step;
 // the bounded loops cannot spin on a non-finite value.
 __device__ inline double wrap_coord(double x, double L)
@@ -247,6 +304,15 @@ int launch_active_work(const uint32_t *skeys, const uint32_t *perm, const uint32
     const uint32_t blocks = std::min<uint32_t>(kRedMaxBlocks, (n + kRedThreads - 1) / kRedThreads);
     k_active_work<<<blocks, kRedThreads, 0, s>>>(skeys, perm, off, n, shift, ti_end,
                                                   ti_current, out);
+    CUCO_LAUNCH_CHECK();
+    return CUCO_OK;
+}
+
+int launch_drift_tables(double a_begin, double a_max, double omega0, double omega_lambda,
+                        double hubble, double *drift, double *kick, hipStream_t s)
+{
+    k_drift_tables<<<1, 1024, 0, s>>>(a_begin, log(a_begin), log(a_max), omega0, omega_lambda,
+                                      hubble, drift, kick);
     CUCO_LAUNCH_CHECK();
     return CUCO_OK;
 }

@@ -32,13 +32,15 @@ struct BoxSet {
     uint32_t total[kMapCount] = {};  // blocks per launch
     uint32_t *tile_map = nullptr;    // nboxes+1: first pair-kernel tile per box
     uint32_t fft32_mask = 0;         // bit i: a box has mesh 32 << i (register FFTs)
+    int fft_split = 0;               // register-FFT line split (fft32_blocks)
     BoxView view(int mode, int map) const
     {
         return BoxView{dev[mode], maps + (size_t)map * (nboxes + 1), nboxes};
     }
 };
-// Blocks of one box of mesh size M for a PM map (kMapGatherX..kMapLinesR).
-uint32_t pm_blocks(int M, int map);
+// Blocks of one box of mesh size M for a PM map (kMapGatherX..kMapLinesR);
+// fft_split selects the register-FFT line split.
+uint32_t pm_blocks(int M, int map, int fft_split = 0);
 
 // ------------------------------------------------------------ sort
 struct SortScratch {
@@ -100,6 +102,9 @@ struct TileInfo {
     uint32_t max_tiles = 0;
     uint32_t nblocks = 0;
     int block_shift = 3;              // Morton block = 2^shift cells per dim
+    // Each-pair-once kernel: tiles at least this many cells wide enumerate
+    // their candidates by 2x2x2-cell blocks instead of cells.
+    float coarse_cells = 4.0f;
 };
 
 // Constants of the FP32 pair kernel for one box.  All lengths are scaled by
@@ -171,6 +176,8 @@ template <typename T>
 int pm_deposit(const void *spos, const uint32_t *skeys, const uint32_t *off,
                const BoxSet &bs, int mode, T *rho, void *scratch, hipStream_t s);
 size_t pm_deposit_scratch_bytes(const BoxSet &bs, int fp64);
+// Clears the parts of a new FP32 deposit scratch that must start at zero.
+int pm_deposit_scratch_init(const BoxSet &bs, void *scratch, hipStream_t s);
 // Fixed-point scale of the FP32 deposit tiles for a box of total mass m_total.
 double deposit_scale(double m_total);
 template <typename T>
@@ -182,10 +189,12 @@ template <typename T>
 int pm_fft_inverse(Cx<T> *c, T *phi, const BoxSet &bs, const PmTables &tb,
                    hipStream_t s);
 // off: the cell table of the sort (FP32: Morton keys, read per brick).
+// dep_scratch: the FP32 deposit scratch of the same evaluation (its work
+// units); unused for FP64.
 template <typename T>
 int pm_interp(const T *phi, const void *spos, const uint32_t *skeys,
               const uint32_t *perm, const uint32_t *off, const BoxSet &bs, int mode,
-              void *acc, hipStream_t s);
+              const void *dep_scratch, void *acc, hipStream_t s);
 template <typename T>
 int pm_fft_x(Cx<T> *c, const BoxSet &bs, bool inverse, const PmTables &tb,
              hipStream_t s);
@@ -196,7 +205,9 @@ bool fft_size_supported(int M);
 // One launch per mesh size present (bs.fft32_mask).
 bool fft32_supported(int M);
 uint32_t fft32_mask_bit(int M);
-uint32_t fft32_blocks(int M, bool rows);
+// split 0: N1 = 4/8/16/32 threads per line of length N <= 32/128/512/1024;
+// split 1: half as many threads per line, each holding twice the values.
+uint32_t fft32_blocks(int M, bool rows, int split = 0);
 int fft32_forward(const float *rho, Cx<float> *c, const BoxSet &bs, const PmTables &tb,
                   hipStream_t s);
 int fft32_green_x(Cx<float> *c, const BoxSet &bs, const PmTables &tb, hipStream_t s);
@@ -205,6 +216,10 @@ int fft32_inverse(Cx<float> *c, float *phi, const BoxSet &bs, const PmTables &tb
 int fft32_x(Cx<float> *c, const BoxSet &bs, bool inverse, const PmTables &tb, hipStream_t s);
 
 // ------------------------------------------------------------ integrator
+// init_drift_table (driftfac.c:14-42) on the device: kDriftTable entries
+// of each table into device arrays.
+int launch_drift_tables(double a_begin, double a_max, double omega0, double omega_lambda,
+                        double hubble, double *drift, double *kick, hipStream_t s);
 int launch_drift(double4 *pos, const double4 *vel, const BoxSet &bs,
                  double drift, hipStream_t s);
 int launch_kick64(double4 *vel, const double4 *app, const double4 *apm,

@@ -28,6 +28,7 @@
 // This file is compiled with -ffp-contract=off so the FP64 kernels keep the
 // reference's rounding sequence for the weights and stencil (the FP32
 // interpolation fuses explicitly with fmaf).
+#include <algorithm>
 #include <cmath>
 
 #include "internal.h"
@@ -575,35 +576,111 @@ __global__ __launch_bounds__(256) void k_cic_gather(
 
 // FP32 deposit over Morton bricks of 16^3 cells (4096 consecutive keys, so
 // a brick's particles are one sorted range).  A brick's cells reach the
-// 17^3 nodes at local coordinates 0..16 of its corner; nodes 1..15 along
-// every axis get nothing from other bricks.
+// nodes at local coordinates 0..u of its corner per axis, u = 16 (at the
+// upper mesh edge, when M is not a multiple of 16, u = M minus the corner:
+// the node at M is node 0 of the next period); nodes strictly between 0
+// and u along every axis get nothing from other bricks.
 //
-//  k_cic_brick: one block per brick.  The block sums its particles' CIC
+//  k_cic_unit: one block per work unit (a brick, or a run of at most H of
+//    its particles when it holds more).  The block sums its particles' CIC
 //    corners into the 17^3-node tile (a wave-level segmented scan by cell
 //    first where a wave holds repeated cells, then the corner sums added in
 //    64-bit fixed point with LDS atomics: integer adds commute, so the tile
 //    does not depend on the order of waves or particles).  Interior nodes
-//    go to the mesh; the shell (a coordinate 0 or 16) to the brick's
-//    partial tile.
+//    go to the mesh; the shell (a coordinate 0 or u) to the brick's six
+//    face planes.
 //  k_cic_shell: one block per brick for its nodes with a local coordinate 0
 //    (the lower faces): the brick's own shell value plus, along every axis
-//    where the coordinate is 0, the brick below at 16 (periodic; at the
-//    mesh edge the last brick, at M minus its corner), summed in a fixed
-//    order.
+//    where the coordinate is 0, the brick below at its u (periodic), summed
+//    in a fixed order.
 constexpr int kDepBrick = 16;
 constexpr int kDepEdge = kDepBrick + 1;
 constexpr int kDepNodes = kDepEdge * kDepEdge * kDepEdge;
 constexpr uint32_t kDepCells = kDepBrick * kDepBrick * kDepBrick;
+constexpr int kDepFace = kDepEdge * kDepEdge;
+constexpr int kDepShell = 6 * kDepFace;  // floats of a brick's face planes
 
-__global__ __launch_bounds__(256) void k_cic_brick(Src32 src, const uint32_t *__restrict__ skeys,
-                                                   const uint32_t *__restrict__ off, BoxView v,
-                                                   float *__restrict__ partial,
-                                                   float *__restrict__ rho_all)
+// Upper node coordinate per axis of the brick at corner o: 16, or M - o at
+// the upper mesh edge.
+__device__ inline int dep_upper(int o, int M) { return min(kDepBrick, M - o); }
+
+// Position of shell node l in a brick's face planes (0: x = 0, 1: x = u,
+// 2: y = 0, 3: y = u, 4: z = 0, 5: z = u, each 17 x 17 over the other two
+// coordinates); a node on several faces is kept on the first.
+__device__ inline int shell_slot(int lx, int ly, int lz, int ux, int uy, int uz)
+{
+    if (lx == 0 || lx == ux) return (lx ? kDepFace : 0) + ly * kDepEdge + lz;
+    if (ly == 0 || ly == uy) return (ly ? 3 : 2) * kDepFace + lx * kDepEdge + lz;
+    return (lz ? 5 : 4) * kDepFace + lx * kDepEdge + ly;
+}
+
+// Work units of the FP32 deposit and interpolation: a brick's particles in
+// runs of at most H (a brick in a halo core can hold 10^5 particles; one
+// block per brick would leave the whole launch waiting for it).  Built per
+// evaluation: per brick the number of units (and, for a brick of more than
+// one, a slot for its 64-bit fixed-point tile), the scan of those counts,
+// then the unit -> brick list.
+struct Units {
+    const uint32_t *uoff;   // nbricks + 1: first unit of each brick (scanned)
+    const uint32_t *brick;  // per unit: its brick (global over the boxes)
+    uint32_t H;             // particles per unit at most
+    uint32_t nbricks;
+};
+
+__global__ void k_brick_units(const uint32_t *__restrict__ off, BoxView v, uint32_t H,
+                              uint32_t *__restrict__ ucount, uint32_t *__restrict__ slot_of,
+                              uint32_t *__restrict__ nheavy)
+{
+    const uint32_t gb = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gb >= v.map[v.nboxes]) return;
+    const int bi = find_box(v.map, v.nboxes, gb);
+    const BoxDev &bx = v.box[bi];
+    const uint32_t k0 = bx.key_base + (gb - v.map[bi]) * kDepCells;
+    const uint32_t n = off[k0 + kDepCells] - off[k0];
+    const uint32_t U = n > H ? (n + H - 1) / H : 1u;
+    ucount[gb] = U;
+    if (U > 1) slot_of[gb] = atomicAdd(nheavy, 1u);
+}
+
+__global__ void k_brick_units_fill(const uint32_t *__restrict__ uoff, uint32_t nbricks,
+                                   uint32_t *__restrict__ brick)
+{
+    const uint32_t gb = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gb >= nbricks) return;
+    for (uint32_t u = uoff[gb]; u < uoff[gb + 1]; ++u) brick[u] = gb;
+}
+
+// Unit u (wave-uniform): its box, brick inside the box, part and number of
+// parts; false past the last unit.
+__device__ inline bool unit_of(const Units &un, const BoxView &v, uint32_t u, int &bi,
+                               uint32_t &lb, uint32_t &part, uint32_t &nparts)
+{
+    if (u >= un.uoff[un.nbricks]) return false;
+    const uint32_t gb = __builtin_amdgcn_readfirstlane(un.brick[u]);
+    bi = __builtin_amdgcn_readfirstlane(find_box(v.map, v.nboxes, gb));
+    lb = gb - v.map[bi];
+    const uint32_t u0 = __builtin_amdgcn_readfirstlane(un.uoff[gb]);
+    part = u - u0;
+    nparts = __builtin_amdgcn_readfirstlane(un.uoff[gb + 1]) - u0;
+    return true;
+}
+
+__global__ __launch_bounds__(256) void k_cic_unit(Src32 src, const uint32_t *__restrict__ skeys,
+                                                  const uint32_t *__restrict__ off, BoxView v,
+                                                  Units un, uint32_t max_units,
+                                                  const uint32_t *__restrict__ slot_of,
+                                                  uint32_t *__restrict__ arrive,
+                                                  unsigned long long *__restrict__ htile,
+                                                  float *__restrict__ partial,
+                                                  float *__restrict__ rho_all)
 {
     __shared__ unsigned long long s_node[kDepNodes];
+    __shared__ uint32_t s_last;
+    const long ul = xcd_block(max_units);
+    if (ul < 0) return;
     int bi;
-    uint32_t lb;
-    if (!box_block(v, bi, lb)) return;
+    uint32_t lb, part, nparts;
+    if (!unit_of(un, v, (uint32_t)ul, bi, lb, part, nparts)) return;
     const BoxDev &bx = v.box[bi];
     const GridInfo &g = bx.g;
     const int M = g.M;
@@ -612,8 +689,10 @@ __global__ __launch_bounds__(256) void k_cic_brick(Src32 src, const uint32_t *__
     key_cell(g, k0, x0, y0, z0);
     if (x0 >= M || y0 >= M || z0 >= M) return;  // outside the mesh: no cells
     for (int e = threadIdx.x; e < kDepNodes; e += 256) s_node[e] = 0ull;
-    const uint32_t p_begin = off[bx.key_base + k0];
-    const uint32_t p_end = off[bx.key_base + k0 + kDepCells];
+    const uint32_t b_begin = off[bx.key_base + k0];
+    const uint32_t b_end = off[bx.key_base + k0 + kDepCells];
+    const uint32_t p_begin = b_begin + part * un.H;
+    const uint32_t p_end = min(b_end, p_begin + un.H);
     __syncthreads();
     const int lane = threadIdx.x & 63;
     const double scale = bx.dep_scale;
@@ -659,25 +738,49 @@ __global__ __launch_bounds__(256) void k_cic_brick(Src32 src, const uint32_t *__
         }
     }
     __syncthreads();
+    if (nparts > 1) {
+        // A brick of several units: every unit adds its tile into the brick's
+        // 64-bit tile in global memory (integer adds: the sum does not depend
+        // on the order); the last unit to arrive takes the sum, clears the
+        // tile and its counter for the next evaluation, and writes the brick.
+        const uint32_t slot = slot_of[v.map[bi] + lb];
+        unsigned long long *tile = htile + (size_t)slot * kDepNodes;
+        for (int e = threadIdx.x; e < kDepNodes; e += 256) {
+            const unsigned long long t = s_node[e];
+            if (t) atomicAdd(&tile[e], t);
+        }
+        __threadfence();
+        __syncthreads();
+        if (threadIdx.x == 0) s_last = atomicAdd(&arrive[slot], 1u) == nparts - 1 ? 1u : 0u;
+        __syncthreads();
+        if (!s_last) return;
+        __threadfence();
+        for (int e = threadIdx.x; e < kDepNodes; e += 256) {
+            s_node[e] = __hip_atomic_load(&tile[e], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            tile[e] = 0ull;
+        }
+        if (threadIdx.x == 0) arrive[slot] = 0u;
+        __syncthreads();
+    }
     const double inv = 1.0 / scale;
-    float *part = partial + (size_t)(v.map[bi] + lb) * kDepNodes;
+    float *pt = partial + (size_t)(v.map[bi] + lb) * kDepShell;
     float *rho = rho_all + bx.grid0;
+    const int ux = dep_upper(x0, M), uy = dep_upper(y0, M), uz = dep_upper(z0, M);
     // A thread owns node columns (ly, lz) and walks them along x.  Interior
-    // nodes inside the mesh are complete; the rest -- the shell and, where M
-    // is not a multiple of 16, the node at M (node 0 of the next period) --
-    // wait for k_cic_shell.
-    for (int col = threadIdx.x; col < kDepEdge * kDepEdge; col += 256) {
+    // nodes are complete; the shell waits for k_cic_shell; nodes past u get
+    // nothing.
+    for (int col = threadIdx.x; col < kDepFace; col += 256) {
         const int ly = col / kDepEdge, lz = col - ly * kDepEdge;
-        const bool yz_in = ly > 0 && lz > 0 && ly < kDepBrick && lz < kDepBrick && y0 + ly < M &&
-                           z0 + lz < M;
+        if (ly > uy || lz > uz) continue;
+        const bool yz_in = ly > 0 && lz > 0 && ly < uy && lz < uz;
         const size_t yz = (size_t)(y0 + ly) * M + (z0 + lz);
-        for (int lx = 0; lx < kDepEdge; ++lx) {
-            const int e = lx * kDepEdge * kDepEdge + col;
+        for (int lx = 0; lx <= ux; ++lx) {
+            const int e = lx * kDepFace + col;
             const float val = (float)((double)(long long)s_node[e] * inv);
-            if (yz_in && lx > 0 && lx < kDepBrick && x0 + lx < M)
+            if (yz_in && lx > 0 && lx < ux)
                 rho[(size_t)(x0 + lx) * M * M + yz] = val;
             else
-                part[e] = val;
+                pt[shell_slot(lx, ly, lz, ux, uy, uz)] = val;
         }
     }
 }
@@ -703,14 +806,16 @@ __global__ __launch_bounds__(256) void k_cic_shell(const float *__restrict__ par
         below[d] = o[d] == 0 ? last : o[d] - kDepBrick;
         lbel[d] = o[d] == 0 ? M - last : kDepBrick;
     }
-    // Tile of the brick below along the set bits of c (bit 2: x, 1: y, 0: z).
+    const int ux = dep_upper(o[0], M), uy = dep_upper(o[1], M), uz = dep_upper(o[2], M);
+    // Face planes of the brick below along the set bits of c (bit 2: x,
+    // 1: y, 0: z).
     const float *tile[8];
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
         const uint32_t brick =
             cell_key(g, (c & 4) ? below[0] : o[0], (c & 2) ? below[1] : o[1],
                      (c & 1) ? below[2] : o[2]) / kDepCells;
-        tile[c] = partial + (size_t)(v.map[bi] + brick) * kDepNodes;
+        tile[c] = partial + (size_t)(v.map[bi] + brick) * kDepShell;
     }
     float *rho = rho_all + bx.grid0;
     // Lower-face nodes: x = 0 (16x16), then y = 0 with x > 0 (15x16), then
@@ -733,9 +838,11 @@ __global__ __launch_bounds__(256) void k_cic_shell(const float *__restrict__ par
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
             if (((c & 4) && l[0]) || ((c & 2) && l[1]) || ((c & 1) && l[2])) continue;
+            // The node in the brick below: its upper face along the axes of c.
             const int ax = (c & 4) ? lbel[0] : l[0], ay = (c & 2) ? lbel[1] : l[1],
                       az = (c & 1) ? lbel[2] : l[2];
-            sum += tile[c][(ax * kDepEdge + ay) * kDepEdge + az];
+            sum += tile[c][shell_slot(ax, ay, az, (c & 4) ? lbel[0] : ux, (c & 2) ? lbel[1] : uy,
+                                      (c & 1) ? lbel[2] : uz)];
         }
         rho[((size_t)(o[0] + l[0]) * M + (o[1] + l[1])) * M + (o[2] + l[2])] = sum;
     }
@@ -877,20 +984,24 @@ __global__ __launch_bounds__(256) void k_interp32(const float *__restrict__ phi_
                                                   const uint32_t *__restrict__ skeys,
                                                   const uint32_t *__restrict__ perm,
                                                   const uint32_t *__restrict__ off, BoxView v,
+                                                  Units un, uint32_t max_units,
                                                   float4 *__restrict__ acc)
 {
     constexpr int E = kBrickNodes;
     constexpr uint32_t kCells = kBrick * kBrick * kBrick;
     __shared__ float s_phi[E * E * E];
+    const long ul = xcd_block(max_units);
+    if (ul < 0) return;
     int bi;
-    uint32_t lb;
-    if (!box_block(v, bi, lb)) return;
+    uint32_t lb, part, nparts;
+    if (!unit_of(un, v, (uint32_t)ul, bi, lb, part, nparts)) return;
     const BoxDev &bx = v.box[bi];
     const GridInfo &g = bx.g;
     const uint32_t k0 = lb * kCells;
-    const uint32_t p_begin = off[bx.key_base + k0];
-    const uint32_t p_end = off[bx.key_base + k0 + kCells];
-    if (p_begin == p_end) return;  // empty brick (or one outside the mesh)
+    const uint32_t b_end = off[bx.key_base + k0 + kCells];
+    const uint32_t p_begin = off[bx.key_base + k0] + part * un.H;
+    const uint32_t p_end = min(b_end, p_begin + un.H);
+    if (p_begin >= p_end) return;  // empty brick (or one outside the mesh)
     const int M = g.M;
     int x0, y0, z0;
     key_cell(g, k0, x0, y0, z0);
@@ -940,6 +1051,50 @@ int launch_fft_kernel(K kernel, unsigned grid, hipStream_t s, Args... args)
     return CUCO_OK;
 }
 
+// Byte layout of the FP32 deposit scratch: the brick partial tiles, the
+// work-unit tables, and the 64-bit tiles of bricks split into several units
+// (at most capacity/H + 1 such bricks: each holds more than H particles).
+struct DepLayout {
+    uint32_t nbricks, H, max_units, max_slots;
+    size_t partial, ucount, slot_of, brick, scan, ctr, arrive, htile, bytes;
+};
+
+DepLayout dep_layout(const BoxSet &bs)
+{
+    DepLayout d{};
+    d.nbricks = bs.total[kMapBrick];
+    const uint32_t cap = std::max(bs.capacity, 1u);
+    // At most 8192 split bricks whatever the particle count, and at least
+    // 2048 particles (8 passes of the block) per unit.
+    uint32_t H = std::max<uint32_t>(2048u, (cap + 8191u) / 8192u);
+    d.H = (H + 255u) / 256u * 256u;
+    d.max_slots = cap / d.H + 1;
+    d.max_units = d.nbricks + d.max_slots;
+    size_t o = 0;
+    auto take = [&o](size_t bytes) {
+        const size_t at = o;
+        o = (o + bytes + 255) / 256 * 256;
+        return at;
+    };
+    d.partial = take(sizeof(float) * kDepShell * (size_t)d.nbricks);
+    d.ucount = take(sizeof(uint32_t) * ((size_t)d.nbricks + 1));
+    d.slot_of = take(sizeof(uint32_t) * (size_t)d.nbricks);
+    d.brick = take(sizeof(uint32_t) * (size_t)d.max_units);
+    d.scan = take(sizeof(uint32_t) * scan_scratch_words(d.nbricks + 1));
+    d.ctr = take(16);
+    d.arrive = take(sizeof(uint32_t) * (size_t)d.max_slots);
+    d.htile = take(sizeof(unsigned long long) * kDepNodes * (size_t)d.max_slots);
+    d.bytes = o;
+    return d;
+}
+
+Units units_view(const DepLayout &d, const void *scratch)
+{
+    const char *base = static_cast<const char *>(scratch);
+    return Units{reinterpret_cast<const uint32_t *>(base + d.ucount),
+                 reinterpret_cast<const uint32_t *>(base + d.brick), d.H, d.nbricks};
+}
+
 }  // namespace
 
 bool fft_size_supported(int M)
@@ -953,10 +1108,31 @@ int pm_deposit(const void *spos, const uint32_t *skeys, const uint32_t *off,
                const BoxSet &bs, int mode, T *rho, void *scratch, hipStream_t s)
 {
     if constexpr (sizeof(T) == 4) {
-        float *partial = static_cast<float *>(scratch);
-        k_cic_brick<<<xcd_grid(bs.total[kMapBrick]), 256, 0, s>>>(
+        const DepLayout d = dep_layout(bs);
+        if (d.nbricks == 0) return CUCO_OK;
+        char *base = static_cast<char *>(scratch);
+        float *partial = reinterpret_cast<float *>(base + d.partial);
+        uint32_t *ucount = reinterpret_cast<uint32_t *>(base + d.ucount);
+        uint32_t *slot_of = reinterpret_cast<uint32_t *>(base + d.slot_of);
+        uint32_t *brick = reinterpret_cast<uint32_t *>(base + d.brick);
+        uint32_t *ctr = reinterpret_cast<uint32_t *>(base + d.ctr);
+        int rc = launch_fill_u32(ctr, 0u, 1, s);
+        if (rc) return rc;
+        k_brick_units<<<grid_for(d.nbricks, 256), 256, 0, s>>>(
+            off, bs.view(mode, kMapBrick), d.H, ucount, slot_of, ctr);
+        CUCO_LAUNCH_CHECK();
+        // counts -> first unit per brick, in place; the total after the last
+        rc = exclusive_scan_u32(ucount, ucount, d.nbricks,
+                                reinterpret_cast<uint32_t *>(base + d.scan), ucount + d.nbricks, s);
+        if (rc) return rc;
+        k_brick_units_fill<<<grid_for(d.nbricks, 256), 256, 0, s>>>(ucount, d.nbricks, brick);
+        CUCO_LAUNCH_CHECK();
+        k_cic_unit<<<xcd_grid(d.max_units), 256, 0, s>>>(
             Src32{static_cast<const float4 *>(spos)}, skeys, off, bs.view(mode, kMapBrick),
-            partial, reinterpret_cast<float *>(rho));
+            units_view(d, scratch), d.max_units, slot_of,
+            reinterpret_cast<uint32_t *>(base + d.arrive),
+            reinterpret_cast<unsigned long long *>(base + d.htile), partial,
+            reinterpret_cast<float *>(rho));
         CUCO_LAUNCH_CHECK();
         k_cic_shell<<<xcd_grid(bs.total[kMapBrick]), 256, 0, s>>>(
             partial, bs.view(mode, kMapBrick), reinterpret_cast<float *>(rho));
@@ -978,9 +1154,19 @@ int pm_deposit(const void *spos, const uint32_t *skeys, const uint32_t *off,
     return CUCO_OK;
 }
 
+int pm_deposit_scratch_init(const BoxSet &bs, void *scratch, hipStream_t s)
+{
+    // The split-brick tiles and their arrival counters start at zero; every
+    // deposit leaves them at zero again.
+    const DepLayout d = dep_layout(bs);
+    char *base = static_cast<char *>(scratch);
+    return launch_fill_u32(reinterpret_cast<uint32_t *>(base + d.arrive), 0u,
+                           (d.bytes - d.arrive) / sizeof(uint32_t), s);
+}
+
 size_t pm_deposit_scratch_bytes(const BoxSet &bs, int fp64)
 {
-    if (!fp64) return sizeof(float) * (size_t)kDepNodes * bs.total[kMapBrick];
+    if (!fp64) return dep_layout(bs).bytes;
     return sizeof(double) * 8 * ((size_t)bs.capacity + (size_t)bs.wave_slots * 2 + 64);
 }
 
@@ -994,7 +1180,7 @@ double deposit_scale(double m_total)
     return std::ldexp(1.0, 61 - ex);
 }
 
-uint32_t pm_blocks(int M, int map)
+uint32_t pm_blocks(int M, int map, int fft_split)
 {
     const uint32_t Zc = (uint32_t)(M / 2 + 1);
     switch (map) {
@@ -1016,8 +1202,8 @@ uint32_t pm_blocks(int M, int map)
         while ((1 << bits) < M) ++bits;
         return bits >= 4 ? 1u << (3 * (bits - 4)) : 1u;
     }
-    case kMapRowsR: return fft32_blocks(M, true);
-    case kMapLinesR: return fft32_blocks(M, false);
+    case kMapRowsR: return fft32_blocks(M, true, fft_split);
+    case kMapLinesR: return fft32_blocks(M, false, fft_split);
     default: return 0;
     }
 }
@@ -1072,7 +1258,7 @@ int pm_fft_inverse(Cx<T> *c, T *phi, const BoxSet &bs, const PmTables &tb,
 template <typename T>
 int pm_interp(const T *phi, const void *spos, const uint32_t *skeys,
               const uint32_t *perm, const uint32_t *off, const BoxSet &bs, int mode,
-              void *acc, hipStream_t s)
+              const void *dep_scratch, void *acc, hipStream_t s)
 {
     if (bs.total[kMapPart] == 0) return CUCO_OK;
     if (sizeof(T) == 8) {
@@ -1080,9 +1266,12 @@ int pm_interp(const T *phi, const void *spos, const uint32_t *skeys,
             reinterpret_cast<const double *>(phi), static_cast<const double4 *>(spos),
             perm, bs.view(mode, kMapPart), static_cast<double4 *>(acc));
     } else {
-        k_interp32<<<xcd_grid(bs.total[kMapBrick]), 256, 0, s>>>(
+        const DepLayout d = dep_layout(bs);
+        if (d.nbricks == 0) return CUCO_OK;
+        k_interp32<<<xcd_grid(d.max_units), 256, 0, s>>>(
             reinterpret_cast<const float *>(phi), static_cast<const float4 *>(spos),
-            skeys, perm, off, bs.view(mode, kMapBrick), static_cast<float4 *>(acc));
+            skeys, perm, off, bs.view(mode, kMapBrick), units_view(d, dep_scratch),
+            d.max_units, static_cast<float4 *>(acc));
     }
     CUCO_LAUNCH_CHECK();
     return CUCO_OK;
@@ -1117,7 +1306,7 @@ int pm_fft_x(Cx<T> *c, const BoxSet &bs, bool inverse, const PmTables &tb,
                                    hipStream_t);                                     \
     template int pm_interp<T>(const T *, const void *, const uint32_t *,             \
                               const uint32_t *, const uint32_t *, const BoxSet &,    \
-                              int, void *, hipStream_t);
+                              int, const void *, void *, hipStream_t);
 CUCO_PM_INST(double)
 CUCO_PM_INST(float)
 #undef CUCO_PM_INST

@@ -39,7 +39,6 @@ constexpr int kRanges = 128;   // cell ranges per wave before the j stream
 constexpr int kJBuf = 128;     // staged j particles per wave
 constexpr int kTab = 4096;     // LDS erfc table; zero from the cut bin on
 constexpr int kTabSym = 2048;  // its copy in the each-pair-once kernel
-constexpr float kCoarseExtent = 4.0f;  // tile extent (cells) for block enumeration
 
 __device__ inline float wave_min(float v)
 {
@@ -201,7 +200,7 @@ __device__ inline void kahan_add(float &sum, float &comp, float v)
 // ---------------------------------------------------------------------------
 // Each pair once (Newton's third law).  A tile takes only the j with a larger
 // sorted index than its first particle, and inside the tile only j > i; the
-// force on j is minus the pair term, summed over the wave's lanes with DPP
+// force on j is minus the pair term, summed over the wave's lanes as it travels with j
 // and added to j's total in 64-bit fixed point (an integer sum, so the result
 // does not depend on the order in which tiles run).
 
@@ -209,10 +208,13 @@ __device__ inline void kahan_add(float &sum, float &comp, float v)
 // staged j number (L + s) mod 64 and subtracts the pair term from its
 // accumulator, which then moves one lane down (lane L takes lane L+1's), so
 // the accumulator of j is always in the lane that meets j next; after 64
-// steps lane L holds the j-side sum of staged j number L.
-__device__ inline float rotate_down(float v)
+// steps lane L holds the j-side sum of staged j number L.  The move is a
+// backward permute through the LDS crossbar (addr = 4 * ((L + 1) mod 64)):
+// it issues on the LDS pipe, not as a vector ALU instruction, so the pair
+// slot's ALU work is the arithmetic alone.
+__device__ inline float rotate_down(float v, int addr)
 {
-    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x134, 0xF, 0xF, false));
+    return __int_as_float(__builtin_amdgcn_ds_bpermute(addr, __float_as_int(v)));
 }
 
 // Pair sums of one lane against the 64 staged j.  GEN: rcut is not on a
@@ -233,6 +235,7 @@ __device__ inline void interact_sym(const float4 *__restrict__ rot,
                                     float &jy, float &jz)
 {
     const int lane = threadIdx.x & 63;
+    const int next = ((lane + 1) & 63) << 2;
     const float4 *rl = rot + lane;
     const uint32_t *il = rotid + lane;
     for (int s = 0; s < 64; s += 4) {
@@ -273,9 +276,9 @@ __device__ inline void interact_sym(const float4 *__restrict__ rot,
             sx = fmaf(dx[q], fi, sx);
             sy = fmaf(dy[q], fi, sy);
             sz = fmaf(dz[q], fi, sz);
-            jx = rotate_down(fmaf(-dx[q], fj, jx));
-            jy = rotate_down(fmaf(-dy[q], fj, jy));
-            jz = rotate_down(fmaf(-dz[q], fj, jz));
+            jx = rotate_down(fmaf(-dx[q], fj, jx), next);
+            jy = rotate_down(fmaf(-dy[q], fj, jy), next);
+            jz = rotate_down(fmaf(-dz[q], fj, jz), next);
         }
     }
 }
@@ -309,7 +312,7 @@ __global__ __launch_bounds__(kThreads, SYM ? 5 : 1) void k_pp32(
     unsigned long long *__restrict__ cand,
     const int32_t *__restrict__ active_end, int32_t ti_current,
     float4 *__restrict__ iacc, unsigned long long *__restrict__ jacc,
-    uint32_t n_sorted, const uint32_t *__restrict__ tlist)
+    uint32_t n_sorted, const uint32_t *__restrict__ tlist, float coarse_cells)
 {
     // SYM trades half of the range list for the doubled rotation buffer.
     constexpr int kR = SYM ? kRanges / 2 : kRanges;
@@ -397,12 +400,13 @@ __global__ __launch_bounds__(kThreads, SYM ? 5 : 1) void k_pp32(
         // contiguous key range: eight times fewer units to test, and the
         // staging takes each candidate's cell from the low bits of its key.
         // Blocks need an even mesh (they wrap onto blocks) and a reach of at
-        // most M cells per axis.  Only the each-pair-once variant uses them
-        // (it runs the halo workloads; the default variant keeps cells).
+        // most M cells per axis; coarse_cells is the tile extent (in cells)
+        // from which they are used.  Only the each-pair-once variant uses
+        // them (it runs the halo workloads; the default variant keeps cells).
         const float ext = fmaxf(bmaxx - bminx, fmaxf(bmaxy - bminy, bmaxz - bminz));
         const bool coarse =
             SYM && uni((uint32_t)(sane && (M & 1) == 0 && nqx <= M && nqy <= M && nqz <= M &&
-                                  ext >= kCoarseExtent * pp.cellf)) != 0u;
+                                  ext >= coarse_cells * pp.cellf)) != 0u;
         const int lv = coarse ? 1 : 0;
         // Units (absolute, unwrapped) covering cells c0 + lq .. c0 + lq + nq - 1.
         const int ux0 = (c0x + lqx) >> lv, uy0 = (c0y + lqy) >> lv, uz0 = (c0z + lqz) >> lv;
@@ -987,7 +991,7 @@ int launch_pp32(const float4 *spos, const uint32_t *skeys,
     k_pp32<GENERIC, UNIFORM, true><<<nwg, kThreads, 0, s>>>(                            \
         spos, skeys, perm, off, tv, p32s, mass, table32, t.tile_start, t.tile_count,    \
         t.counters, acc, cand_counter, nullptr, 0, sym.iacc, sym.jacc, sym.n_sorted,   \
-        nullptr)
+        nullptr, t.coarse_cells)
         if (generic) {
             if (uniform) CUCO_SYM32(true, true); else CUCO_SYM32(true, false);
         } else {
@@ -1015,7 +1019,7 @@ int launch_pp32(const float4 *spos, const uint32_t *skeys,
 #define CUCO_PP32(GENERIC, UNIFORM)                                                     \
     k_pp32<GENERIC, UNIFORM, false><<<nwg, kThreads, 0, s>>>(                           \
         spos, skeys, perm, off, tva, p32s, mass, table32, tt.tile_start, tt.tile_count, \
-        tt.counters, acc, cand_counter, ae, ti_current, nullptr, nullptr, 0u, tl)
+        tt.counters, acc, cand_counter, ae, ti_current, nullptr, nullptr, 0u, tl, 0.f)
     if (generic) {
         if (uniform) CUCO_PP32(true, true); else CUCO_PP32(true, false);
     } else {

@@ -50,12 +50,17 @@ EXPORTED = [
     "cuco_read_snapshot", "cuco_read_output_list", "cuco_find_next_output_time",
     "cuco_download_records", "cuco_upload_records", "cuco_timeline_init",
     "cuco_timeline_run", "cuco_get_timesteps", "cuco_init_records",
+    "cuco_device_drift_table",
 ]
 
 
 class Tuning(C.Structure):
     _fields_ = [("pp_block_shift", C.c_int32), ("pp_workgroups_per_cu", C.c_int32),
-                ("pm_overlap", C.c_int32), ("pp_symmetric", C.c_int32)]
+                ("pm_overlap", C.c_int32), ("pp_symmetric", C.c_int32),
+                ("pp_cell_level", C.c_int32), ("fft_split", C.c_int32)]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
 
 
 class CucoError(RuntimeError):
@@ -191,6 +196,7 @@ def lib():
         "cuco_stage_name": (C.c_char_p, [i32]),
         "cuco_synchronize": (C.c_int, [vp]),
         "cuco_init_drift_table": (C.c_int, [C.c_double] * 5 + [dp, dp]),
+        "cuco_device_drift_table": (C.c_int, [i32] + [C.c_double] * 5 + [dp, dp]),
         "cuco_table_factor": (C.c_double, [dp] + [C.c_double] * 4),
         "cuco_kdk_factors": (C.c_int, [dp, dp] + [C.c_double] * 4 + [i32, dp]),
     }
@@ -229,6 +235,16 @@ def init_drift_table(a_begin, a_max, omega0, omega_lambda, hubble):
     _check(lib().cuco_init_drift_table(a_begin, a_max, omega0, omega_lambda,
                                        hubble, _dp(d), _dp(k)),
            "cuco_init_drift_table")
+    return d, k
+
+
+def device_drift_table(a_begin, a_max, omega0, omega_lambda, hubble, device=0):
+    """init_drift_table (driftfac.c:14-42) built on the device (the tables
+    cuco_timeline_init uses): returns (drift, kick)."""
+    d = np.zeros(DRIFT_TABLE_LENGTH)
+    k = np.zeros(DRIFT_TABLE_LENGTH)
+    _check(lib().cuco_device_drift_table(device, a_begin, a_max, omega0, omega_lambda,
+                                         hubble, _dp(d), _dp(k)), "cuco_device_drift_table")
     return d, k
 
 
@@ -574,25 +590,22 @@ class Engine:
     def tuning(self) -> dict:
         t = Tuning()
         _check(lib().cuco_get_tuning(self.h, C.byref(t)), "cuco_get_tuning")
-        return {"pp_block_shift": t.pp_block_shift,
-                "pp_workgroups_per_cu": t.pp_workgroups_per_cu,
-                "pm_overlap": t.pm_overlap, "pp_symmetric": t.pp_symmetric}
+        return t.as_dict()
 
     def set_tuning(self, pp_block_shift: int, pp_workgroups_per_cu: int,
-                   pm_overlap: int = 0, pp_symmetric: int = 1) -> None:
+                   pm_overlap: int = 0, pp_symmetric: int = 1, pp_cell_level: int = 0,
+                   fft_split: int = 0) -> None:
         t = Tuning(pp_block_shift, pp_workgroups_per_cu, int(pm_overlap),
-                   int(pp_symmetric))
+                   int(pp_symmetric), int(pp_cell_level), int(fft_split))
         _check(lib().cuco_set_tuning(self.h, C.byref(t)), "cuco_set_tuning")
 
     def autotune(self):
-        """Picks the FP32 pair-kernel tuning by timing the candidates."""
+        """Picks the FP32 tuning (pair kernel, FFT split, PM overlap) by
+        timing the candidates on the current particles."""
         t = Tuning()
         ms = C.c_double(0)
         _check(lib().cuco_autotune(self.h, C.byref(t), C.byref(ms)), "cuco_autotune")
-        return ({"pp_block_shift": t.pp_block_shift,
-                 "pp_workgroups_per_cu": t.pp_workgroups_per_cu,
-                 "pm_overlap": t.pm_overlap, "pp_symmetric": t.pp_symmetric},
-                ms.value)
+        return t.as_dict(), ms.value
 
     def set_timing(self, on: bool) -> None:
         _check(lib().cuco_set_timing(self.h, int(on)), "cuco_set_timing")

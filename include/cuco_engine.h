@@ -178,15 +178,28 @@ typedef struct cuco_tuning {
                                      particles (full evaluations; active sets
                                      of the integer timeline visit every
                                      ordered pair); 0: every ordered pair */
+    int32_t pp_cell_level;        /* candidate enumeration of the each-pair-
+                                     once FP32 kernel: 0 (the default) mesh
+                                     cells for tiles narrower than 4 cells,
+                                     2x2x2-cell blocks for wider ones; 1 mesh
+                                     cells for every tile; 2 two-cell blocks
+                                     for every tile (even meshes) */
+    int32_t fft_split;            /* FP32 register FFTs (power-of-two
+                                     meshes): 0 (the default) 4/8/16/32
+                                     threads per line of length <= 32/128/
+                                     512/1024; 1 half as many, each holding
+                                     twice the values */
 } cuco_tuning;
 
 int cuco_get_tuning(cuco_engine *e, cuco_tuning *t);
 int cuco_set_tuning(cuco_engine *e, const cuco_tuning *t);
-/* Times the FP32 pair force on the current particles for every candidate
- * block size and workgroup count, then a whole FP32 evaluation with and
- * without PM/PP overlap; keeps the fastest and returns it; *best_ms
- * (optional) receives the best pair-kernel time.  Leaves the accelerations
- * of the last trial. */
+/* Times, on the handle's current particles: the FP32 pair force for every
+ * Morton block size, workgroup count and pair variant, then its candidate
+ * enumeration level; the FP32 PM chain with each FFT line split; then whole
+ * FP32 evaluations with and without PM/PP overlap.  Keeps the fastest and
+ * returns it; *best_ms (optional) receives the best pair-kernel time.
+ * Every setting gives the same forces up to FP32 rounding.  Leaves the
+ * accelerations of the last trial. */
 int cuco_autotune(cuco_engine *e, cuco_tuning *chosen, double *best_ms);
 
 /* ---------------------------------------------------------------- */
@@ -218,6 +231,14 @@ int cuco_kdk_steps(cuco_engine *e, int32_t precision, int32_t nsteps,
 int cuco_init_drift_table(double a_begin, double a_max, double omega0,
                           double omega_lambda, double hubble, double *drift,
                           double *kick);
+/* init_drift_table (driftfac.c:14-42) built on HIP device `device`, the
+ * way cuco_timeline_init builds the engine's tables: each interval of the
+ * ln a grid integrated with the 16-point Gauss-Legendre rule, the pieces
+ * summed in order; the tables are copied into the caller's host arrays
+ * (CUCO_DRIFT_TABLE_LENGTH doubles each). */
+int cuco_device_drift_table(int32_t device, double a_begin, double a_max,
+                            double omega0, double omega_lambda, double hubble,
+                            double *drift, double *kick);
 /* get_drift_factor / get_gravkick_factor (driftfac.c:66-126) between two
  * values of ln a (the reference takes integer times on the ln a timeline). */
 double cuco_table_factor(const double *table, double a_begin, double a_max,
@@ -283,8 +304,9 @@ typedef struct cuco_timeline_state {
 
 /* init (init.c:20-48) and init_drift_table (driftfac.c:14-42): Time =
  * TimeBegin, Ti_Current = 0, every Ti_begstep = Ti_endstep = 0 and
- * PM_Ti_begstep = PM_Ti_endstep = 0; builds the drift and kick tables.
- * Needs uploaded particles (internal velocities). */
+ * PM_Ti_begstep = PM_Ti_endstep = 0; builds the drift and kick tables on
+ * the device (as cuco_device_drift_table).  Needs uploaded particles
+ * (internal velocities). */
 int cuco_timeline_init(cuco_engine *e, const cuco_timeline *t);
 /* Up to max_steps passes of run()'s body (run.c:9-25): find the next sync
  * point and drift every particle to it (run.c:29-64); set the softening for

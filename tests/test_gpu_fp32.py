@@ -174,11 +174,45 @@ def test_every_tuning_gives_the_same_forces():
                 assert np.abs(a - ref).max() <= 1e-5 * scale, (shift, wg, sym)
     best, ms = eng.autotune()
     assert best["pp_block_shift"] in (2, 3, 4) and ms > 0
+    assert best["pp_cell_level"] in (0, 1, 2) and best["fft_split"] in (0, 1)
     assert eng.tuning() == best
     eng.compute_accelerations(E.FP32, True)
     assert np.abs(eng.accelerations()[0] - ref).max() <= 1e-5 * scale
     with pytest.raises(E.CucoError):
         eng.set_tuning(9, 4)
+    with pytest.raises(E.CucoError):
+        eng.set_tuning(3, 8, 0, 1, pp_cell_level=3)
+    with pytest.raises(E.CucoError):
+        eng.set_tuning(3, 8, 0, 1, fft_split=2)
+
+
+@pytest.mark.parametrize("M", [64, 128, 96])
+def test_enumeration_level_and_fft_split_give_the_same_forces(M):
+    """The autotuner's candidate-enumeration level of the each-pair-once
+    kernel (mesh cells, 2x2x2-cell blocks for wide tiles, blocks for every
+    tile) and the register-FFT line split (96: not a power of two, the
+    Stockham kernels run whatever the split) change only FP32 rounding: PP
+    within 1e-5 and PM within 1e-5 of the force scale."""
+    box = halo_box(32, M, 100.0, n_halos=20, seed=M)
+    pos, vel, mass = ics.generate(box)
+    eng = make_engine(box)
+    eng.upload(pos, None, mass)
+    eng.set_tuning(3, 8, 0, 1)
+    eng.compute_accelerations(E.FP32, True)
+    ref_pp, ref_pm = eng.accelerations()
+    scale = np.linalg.norm(ref_pp + ref_pm, axis=1).max()
+    for level in (0, 1, 2):
+        for split in (0, 1):
+            eng.set_tuning(3, 8, 0, 1, pp_cell_level=level, fft_split=split)
+            assert eng.tuning()["pp_cell_level"] == level
+            assert eng.tuning()["fft_split"] == split
+            eng.compute_accelerations(E.FP32, True)
+            a, p = eng.accelerations()
+            assert np.abs(a - ref_pp).max() <= 1e-5 * scale, (level, split)
+            assert np.abs(p - ref_pm).max() <= 1e-5 * scale, (level, split)
+            if split == 0:
+                assert np.array_equal(p, ref_pm)
+    eng.close()
 
 
 @pytest.mark.parametrize("precision", [E.FP64, E.FP32])

@@ -25,8 +25,9 @@ def setup(orc, seed=9):
               max_size_timestep=math.log(1.5) / 8, min_size_timestep=0.0,
               softening_halo=box.softening,
               softening_halo_max_phys=box.softening * 0.25)
-    d, k = E.init_drift_table(tl["time_begin"], tl["time_max"], tl["omega0"],
-                              tl["omega_lambda"], tl["hubble"])
+    # the engine builds its tables on the device; the oracle takes the same
+    d, k = E.device_drift_table(tl["time_begin"], tl["time_max"], tl["omega0"],
+                                tl["omega_lambda"], tl["hubble"])
     ref = orc.Timeline(oracle_params(box), tl, d, k, pos, vel, mass)
     eng = make_engine(box)
     eng.upload(pos, vel, mass)
@@ -205,3 +206,29 @@ def test_records_and_snapshot_round_trip(orc, tmp_path):
     p2, v2 = other.download_particles()
     assert np.array_equal(p2, np.mod(Q["Pos"].astype(np.float64), box.L))
     assert np.array_equal(v2, Q["Vel"].astype(np.float64))
+
+
+def test_device_drift_tables(golden_dir):
+    """init_drift_table built on the device (driftfac.c:14-42): the
+    Einstein-de Sitter closed forms drift = 2/H0 (a0^-1/2 - a^-1/2) and
+    kick = 2/H0 (a^1/2 - a0^1/2) to 1e-12 (the closed form's own
+    difference of two nearly equal terms costs ~1e-13); for LambdaCDM the host tables
+    (adaptive Simpson) to 1e-12 and the scipy.integrate.quad fixture through
+    the reference's interpolation to 1e-9."""
+    a0, a1, H0 = 0.02, 0.05, 0.1
+    d, k = E.device_drift_table(a0, a1, 1.0, 0.0, H0)
+    a = np.exp(np.log(a0) + (np.log(a1) - np.log(a0)) / 1000 * (np.arange(1000) + 1))
+    np.testing.assert_allclose(d, 2 / H0 * (a0 ** -0.5 - a ** -0.5), rtol=1e-12)
+    np.testing.assert_allclose(k, 2 / H0 * (a ** 0.5 - a0 ** 0.5), rtol=1e-12)
+    g = np.load(f"{golden_dir}/drift_kick.npz")
+    a0, a1 = float(g["a0"]), float(g["a1"])
+    args = (a0, a1, float(g["omega0"]), float(g["omega_lambda"]), float(g["hubble"]))
+    d, k = E.device_drift_table(*args)
+    hd, hk = E.init_drift_table(*args)
+    np.testing.assert_allclose(d, hd, rtol=1e-12)
+    np.testing.assert_allclose(k, hk, rtol=1e-12)
+    for av, dr, kr in zip(g["avals"], g["drift"], g["kick"]):
+        assert E.table_factor(d, a0, a1, math.log(a0), math.log(av)) == pytest.approx(dr, rel=1e-9)
+        assert E.table_factor(k, a0, a1, math.log(a0), math.log(av)) == pytest.approx(kr, rel=1e-9)
+    with pytest.raises(E.CucoError):
+        E.device_drift_table(0.05, 0.02, 0.3, 0.7, 0.1)
