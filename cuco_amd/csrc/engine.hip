@@ -1821,6 +1821,19 @@ int cuco_device_drift_table(int32_t device, double a_begin, double a_max, double
     return rc;
 }
 
+int cuco_radix_sort_noalloc(uint32_t *d_keys, uint32_t *d_values, uint32_t n,
+                            uint32_t key_bits)
+{
+    if (n == 0) return CUCO_OK;
+    if (!d_keys || key_bits == 0 || key_bits > 32) {
+        set_error("null keys or key_bits %u outside 1..32", key_bits);
+        return CUCO_ERR_INVALID_ARG;
+    }
+    TRY(radix_sort_static(d_keys, d_values, n, (int)key_bits, nullptr));
+    CUCO_HIP(hipStreamSynchronize(nullptr));
+    return CUCO_OK;
+}
+
 int cuco_get_sorted(cuco_engine *e, uint32_t *keys, uint32_t *index)
 try {
     CHECK_ENGINE(e);

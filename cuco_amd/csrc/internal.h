@@ -65,6 +65,12 @@ int launch_calc_keys(const double4 *pos, const BoxSet &bs, int mode,
 int radix_sort_pairs(uint32_t *keys, uint32_t *vals, bool vals_identity,
                      uint32_t n, int key_bits, const SortScratch &sc,
                      hipStream_t s, uint32_t **keys_out, uint32_t **vals_out);
+// The same sort with its scratch in a static device array (no allocation):
+// at most static_sort_capacity() pairs; the result is left in keys / vals.
+// Calls that overlap on one device share the array.
+uint32_t static_sort_capacity();
+int radix_sort_static(uint32_t *keys, uint32_t *vals, uint32_t n, int key_bits,
+                      hipStream_t s);
 // off[c] = first sorted index with key >= c, c in [0, nkeys]; off[nkeys] = n.
 int launch_cell_offsets(const uint32_t *skeys, uint32_t n, uint32_t nkeys,
                         uint32_t *off, hipStream_t s);

@@ -14,6 +14,7 @@
 // a wave-level multisplit (eight 64-lane ballots), reorders the tile through
 // LDS by digit, and writes each digit run contiguously.
 #include <algorithm>
+#include <mutex>
 
 #include "internal.h"
 
@@ -355,7 +356,86 @@ __global__ void k_fill_u32(uint32_t *p, uint32_t value, size_t n)
     if (i < n) p[i] = value;
 }
 
+__global__ void k_copy_u32(uint32_t *__restrict__ dst, const uint32_t *__restrict__ src,
+                           uint32_t n)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+// ---- sort scratch without allocation
+// A file-scope device array holds the temporary keys and values, the digit
+// counters and the digit totals of one sort of up to kStaticSortKeys pairs
+// (33 MiB).  Every device the library runs on holds its own copy until the
+// process ends; its address is looked up once per device.
+constexpr uint32_t kStaticSortKeys = 1u << 22;
+constexpr size_t kStaticCountWords =
+    256 * ((size_t)(kStaticSortKeys + kSortTile - 1) / kSortTile + 1);
+constexpr size_t kStaticSortWords = 2 * (size_t)kStaticSortKeys + kStaticCountWords + 256;
+
+static __device__ __align__(256) uint32_t scratch[kStaticSortWords];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+uint32_t *g_scratch_ptr[kMaxDevices];
+
+// Device address of `scratch` on the current device (the host-side name is
+// only a stand-in; kernels must get this address).
+int static_sort_scratch(uint32_t **out)
+{
+    int dev = -1;
+    CUCO_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) {
+        set_error("device %d outside 0..%d", dev, kMaxDevices - 1);
+        return CUCO_ERR_INVALID_ARG;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        CUCO_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[dev] = static_cast<uint32_t *>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return CUCO_OK;
+}
+
 }  // namespace
+
+uint32_t static_sort_capacity() { return kStaticSortKeys; }
+
+int radix_sort_static(uint32_t *keys, uint32_t *vals, uint32_t n, int key_bits,
+                      hipStream_t s)
+{
+    if (n > kStaticSortKeys) {
+        set_error("allocation-free sort: %u keys exceed its maximum of %u", n,
+                  kStaticSortKeys);
+        return CUCO_ERR_INVALID_ARG;
+    }
+    if (n == 0) return CUCO_OK;
+    uint32_t *base = nullptr;
+    int rc = static_sort_scratch(&base);
+    if (rc) return rc;
+    SortScratch sc;
+    sc.keys_tmp = base;
+    sc.vals_tmp = base + kStaticSortKeys;
+    sc.counts = base + 2 * (size_t)kStaticSortKeys;
+    sc.totals = sc.counts + kStaticCountWords;
+    sc.capacity = kStaticSortKeys;
+    uint32_t *ko = nullptr, *vo = nullptr;
+    rc = radix_sort_pairs(keys, vals, false, n, key_bits, sc, s, &ko, &vo);
+    if (rc) return rc;
+    if (ko != keys) {
+        // an odd number of passes leaves the result in the scratch copies
+        const unsigned grid = std::min(grid_for(n, 256), 4096u);
+        k_copy_u32<<<grid, 256, 0, s>>>(keys, ko, n);
+        CUCO_LAUNCH_CHECK();
+        if (vals) {
+            k_copy_u32<<<grid, 256, 0, s>>>(vals, vo, n);
+            CUCO_LAUNCH_CHECK();
+        }
+    }
+    return CUCO_OK;
+}
 
 int launch_calc_keys(const double4 *pos, const BoxSet &bs, int mode,
                      uint32_t *keys, hipStream_t s)

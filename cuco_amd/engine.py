@@ -50,7 +50,7 @@ EXPORTED = [
     "cuco_read_snapshot", "cuco_read_output_list", "cuco_find_next_output_time",
     "cuco_download_records", "cuco_upload_records", "cuco_timeline_init",
     "cuco_timeline_run", "cuco_get_timesteps", "cuco_init_records",
-    "cuco_device_drift_table",
+    "cuco_radix_sort_noalloc", "cuco_device_drift_table",
 ]
 
 
@@ -183,6 +183,7 @@ def lib():
         "cuco_set_tuning": (C.c_int, [vp, C.POINTER(Tuning)]),
         "cuco_autotune": (C.c_int, [vp, C.POINTER(Tuning), dp]),
         "cuco_radix_sort": (C.c_int, [vp, vp, C.c_uint32, C.c_uint32]),
+        "cuco_radix_sort_noalloc": (C.c_int, [vp, vp, C.c_uint32, C.c_uint32]),
         "cuco_fft3d_r2c": (C.c_int, [i32, i32, vp, vp]),
         "cuco_fft3d_c2r": (C.c_int, [i32, i32, vp, vp]),
         "cuco_get_sorted": (C.c_int, [vp, up, up]),

@@ -386,6 +386,17 @@ int cuco_copy_device_to_device(void *dst, const void *src, size_t bytes);
  * allocated per call. */
 int cuco_radix_sort(uint32_t *d_keys, uint32_t *d_values, uint32_t n,
                     uint32_t key_bits);
+/* cuco_radix_sort without any allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments and result; the scratch is a static device array of the
+ * library (33 MiB per device, held until the process ends), so n is at most
+ * CUCO_RADIX_SORT_NOALLOC_MAX (larger n: CUCO_ERR_INVALID_ARG).  All calls
+ * on one device share that array: calls that overlap on one device race.
+ * Synchronous (null stream). */
+#define CUCO_RADIX_SORT_NOALLOC_MAX 4194304u
+int cuco_radix_sort_noalloc(uint32_t *d_keys, uint32_t *d_values, uint32_t n,
+                            uint32_t key_bits);
 
 /* Unnormalized 3D FFTs on device arrays, replacing the cuFFT plans of
  * pm_periodic.c:34-35 and their executions cufftExecR2C (:140) and
