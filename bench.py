@@ -380,6 +380,7 @@ def halo512_bench(E, ics, tuning, steps=5, n_oracle=1_000_000, seed=5006,
         t0 = time.perf_counter()
         ref_pp, _ = oracle.pp_force(p, pos, mass, targets)
         t_pp = time.perf_counter() - t0
+        log(f"[bench] halo512 oracle PP on {n_oracle} targets in {t_pp:.1f}s; PM next")
         t0 = time.perf_counter()
         ref_pm = oracle.pm_force(p, pos, mass)[targets]
         t_pm = time.perf_counter() - t0

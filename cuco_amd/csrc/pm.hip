@@ -629,7 +629,7 @@ struct Units {
 
 __global__ void k_brick_units(const uint32_t *__restrict__ off, BoxView v, uint32_t H,
                               uint32_t *__restrict__ ucount, uint32_t *__restrict__ slot_of,
-                              uint32_t *__restrict__ nheavy)
+                              uint32_t *__restrict__ slot_brick, uint32_t *__restrict__ nsplit)
 {
     const uint32_t gb = blockIdx.x * blockDim.x + threadIdx.x;
     if (gb >= v.map[v.nboxes]) return;
@@ -639,7 +639,13 @@ __global__ void k_brick_units(const uint32_t *__restrict__ off, BoxView v, uint3
     const uint32_t n = off[k0 + kDepCells] - off[k0];
     const uint32_t U = n > H ? (n + H - 1) / H : 1u;
     ucount[gb] = U;
-    if (U > 1) slot_of[gb] = atomicAdd(nheavy, 1u);
+    if (U > 1) {
+        // The slot only says where the brick's tile lives, so the order in
+        // which bricks get slots does not matter.
+        const uint32_t slot = atomicAdd(nsplit, 1u);
+        slot_of[gb] = slot;
+        slot_brick[slot] = gb;
+    }
 }
 
 __global__ void k_brick_units_fill(const uint32_t *__restrict__ uoff, uint32_t nbricks,
@@ -655,7 +661,7 @@ __global__ void k_brick_units_fill(const uint32_t *__restrict__ uoff, uint32_t n
 __device__ inline bool unit_of(const Units &un, const BoxView &v, uint32_t u, int &bi,
                                uint32_t &lb, uint32_t &part, uint32_t &nparts)
 {
-    if (u >= un.uoff[un.nbricks]) return false;
+    if (u >= __builtin_amdgcn_readfirstlane(un.uoff[un.nbricks])) return false;
     const uint32_t gb = __builtin_amdgcn_readfirstlane(un.brick[u]);
     bi = __builtin_amdgcn_readfirstlane(find_box(v.map, v.nboxes, gb));
     lb = gb - v.map[bi];
@@ -665,22 +671,46 @@ __device__ inline bool unit_of(const Units &un, const BoxView &v, uint32_t u, in
     return true;
 }
 
+// The brick's tile (64-bit fixed point, in LDS) to the mesh and the face
+// planes: interior nodes are complete; the shell waits for k_cic_shell;
+// nodes past u get nothing.  A thread owns node columns (ly, lz) and walks
+// them along x.
+__device__ inline void write_brick(const unsigned long long *s_node, double scale, int x0,
+                                   int y0, int z0, int M, float *__restrict__ pt,
+                                   float *__restrict__ rho)
+{
+    const double inv = 1.0 / scale;
+    const int ux = dep_upper(x0, M), uy = dep_upper(y0, M), uz = dep_upper(z0, M);
+    for (int col = threadIdx.x; col < kDepFace; col += 256) {
+        const int ly = col / kDepEdge, lz = col - ly * kDepEdge;
+        if (ly > uy || lz > uz) continue;
+        const bool yz_in = ly > 0 && lz > 0 && ly < uy && lz < uz;
+        const size_t yz = (size_t)(y0 + ly) * M + (z0 + lz);
+        for (int lx = 0; lx <= ux; ++lx) {
+            const int e = lx * kDepFace + col;
+            const float val = (float)((double)(long long)s_node[e] * inv);
+            if (yz_in && lx > 0 && lx < ux)
+                rho[(size_t)(x0 + lx) * M * M + yz] = val;
+            else
+                pt[shell_slot(lx, ly, lz, ux, uy, uz)] = val;
+        }
+    }
+}
+
+// One block per work unit.  Units are taken in launch order (round robin
+// over the XCDs): a split brick's units then spread over the chip, and no
+// two units share data but their split brick's tile.
 __global__ __launch_bounds__(256) void k_cic_unit(Src32 src, const uint32_t *__restrict__ skeys,
                                                   const uint32_t *__restrict__ off, BoxView v,
-                                                  Units un, uint32_t max_units,
-                                                  const uint32_t *__restrict__ slot_of,
-                                                  uint32_t *__restrict__ arrive,
+                                                  Units un, const uint32_t *__restrict__ slot_of,
                                                   unsigned long long *__restrict__ htile,
                                                   float *__restrict__ partial,
                                                   float *__restrict__ rho_all)
 {
     __shared__ unsigned long long s_node[kDepNodes];
-    __shared__ uint32_t s_last;
-    const long ul = xcd_block(max_units);
-    if (ul < 0) return;
     int bi;
     uint32_t lb, part, nparts;
-    if (!unit_of(un, v, (uint32_t)ul, bi, lb, part, nparts)) return;
+    if (!unit_of(un, v, blockIdx.x, bi, lb, part, nparts)) return;
     const BoxDev &bx = v.box[bi];
     const GridInfo &g = bx.g;
     const int M = g.M;
@@ -739,50 +769,46 @@ __global__ __launch_bounds__(256) void k_cic_unit(Src32 src, const uint32_t *__r
     }
     __syncthreads();
     if (nparts > 1) {
-        // A brick of several units: every unit adds its tile into the brick's
-        // 64-bit tile in global memory (integer adds: the sum does not depend
-        // on the order); the last unit to arrive takes the sum, clears the
-        // tile and its counter for the next evaluation, and writes the brick.
-        const uint32_t slot = slot_of[v.map[bi] + lb];
-        unsigned long long *tile = htile + (size_t)slot * kDepNodes;
+        // A brick of several units: every unit adds its tile into the
+        // brick's 64-bit tile in global memory (integer adds: the sum does
+        // not depend on the order); k_cic_split writes the brick.
+        unsigned long long *tile = htile + (size_t)slot_of[v.map[bi] + lb] * kDepNodes;
         for (int e = threadIdx.x; e < kDepNodes; e += 256) {
             const unsigned long long t = s_node[e];
             if (t) atomicAdd(&tile[e], t);
         }
-        __threadfence();
-        __syncthreads();
-        if (threadIdx.x == 0) s_last = atomicAdd(&arrive[slot], 1u) == nparts - 1 ? 1u : 0u;
-        __syncthreads();
-        if (!s_last) return;
-        __threadfence();
-        for (int e = threadIdx.x; e < kDepNodes; e += 256) {
-            s_node[e] = __hip_atomic_load(&tile[e], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            tile[e] = 0ull;
-        }
-        if (threadIdx.x == 0) arrive[slot] = 0u;
-        __syncthreads();
+        return;
     }
-    const double inv = 1.0 / scale;
-    float *pt = partial + (size_t)(v.map[bi] + lb) * kDepShell;
-    float *rho = rho_all + bx.grid0;
-    const int ux = dep_upper(x0, M), uy = dep_upper(y0, M), uz = dep_upper(z0, M);
-    // A thread owns node columns (ly, lz) and walks them along x.  Interior
-    // nodes are complete; the shell waits for k_cic_shell; nodes past u get
-    // nothing.
-    for (int col = threadIdx.x; col < kDepFace; col += 256) {
-        const int ly = col / kDepEdge, lz = col - ly * kDepEdge;
-        if (ly > uy || lz > uz) continue;
-        const bool yz_in = ly > 0 && lz > 0 && ly < uy && lz < uz;
-        const size_t yz = (size_t)(y0 + ly) * M + (z0 + lz);
-        for (int lx = 0; lx <= ux; ++lx) {
-            const int e = lx * kDepFace + col;
-            const float val = (float)((double)(long long)s_node[e] * inv);
-            if (yz_in && lx > 0 && lx < ux)
-                rho[(size_t)(x0 + lx) * M * M + yz] = val;
-            else
-                pt[shell_slot(lx, ly, lz, ux, uy, uz)] = val;
-        }
+    write_brick(s_node, scale, x0, y0, z0, M, partial + (size_t)(v.map[bi] + lb) * kDepShell,
+                rho_all + bx.grid0);
+}
+
+// Bricks split over several units: the summed 64-bit tile (complete once
+// k_cic_unit has finished) to the mesh and face planes; the tile is cleared
+// for the next deposit.
+__global__ __launch_bounds__(256) void k_cic_split(BoxView v, const uint32_t *__restrict__ nsplit,
+                                                   const uint32_t *__restrict__ slot_brick,
+                                                   unsigned long long *__restrict__ htile,
+                                                   float *__restrict__ partial,
+                                                   float *__restrict__ rho_all)
+{
+    __shared__ unsigned long long s_node[kDepNodes];
+    const uint32_t slot = blockIdx.x;
+    if (slot >= __builtin_amdgcn_readfirstlane(nsplit[0])) return;
+    const uint32_t gb = __builtin_amdgcn_readfirstlane(slot_brick[slot]);
+    const int bi = __builtin_amdgcn_readfirstlane(find_box(v.map, v.nboxes, gb));
+    const uint32_t lb = gb - v.map[bi];
+    const BoxDev &bx = v.box[bi];
+    int x0, y0, z0;
+    key_cell(bx.g, lb * kDepCells, x0, y0, z0);
+    unsigned long long *tile = htile + (size_t)slot * kDepNodes;
+    for (int e = threadIdx.x; e < kDepNodes; e += 256) {
+        s_node[e] = tile[e];
+        tile[e] = 0ull;
     }
+    __syncthreads();
+    write_brick(s_node, bx.dep_scale, x0, y0, z0, bx.g.M, partial + (size_t)gb * kDepShell,
+                rho_all + bx.grid0);
 }
 
 __global__ __launch_bounds__(256) void k_cic_shell(const float *__restrict__ partial, BoxView v,
@@ -990,11 +1016,16 @@ __global__ __launch_bounds__(256) void k_interp32(const float *__restrict__ phi_
     constexpr int E = kBrickNodes;
     constexpr uint32_t kCells = kBrick * kBrick * kBrick;
     __shared__ float s_phi[E * E * E];
-    const long ul = xcd_block(max_units);
-    if (ul < 0) return;
+    // Neighbouring bricks stage overlapping potential nodes: XCD k takes the
+    // k-th contiguous eighth of the units actually present, so neighbours
+    // share an L2 (the grid has at least 8 ceil(total/8) blocks).
+    const uint32_t total = __builtin_amdgcn_readfirstlane(un.uoff[un.nbricks]);
+    const uint32_t per = (total + kXcds - 1) / kXcds;
+    const uint32_t ul = (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+    (void)max_units;
     int bi;
     uint32_t lb, part, nparts;
-    if (!unit_of(un, v, (uint32_t)ul, bi, lb, part, nparts)) return;
+    if (!unit_of(un, v, ul, bi, lb, part, nparts)) return;
     const BoxDev &bx = v.box[bi];
     const GridInfo &g = bx.g;
     const uint32_t k0 = lb * kCells;
@@ -1051,12 +1082,12 @@ int launch_fft_kernel(K kernel, unsigned grid, hipStream_t s, Args... args)
     return CUCO_OK;
 }
 
-// Byte layout of the FP32 deposit scratch: the brick partial tiles, the
+// Byte layout of the FP32 deposit scratch: the brick face planes, the
 // work-unit tables, and the 64-bit tiles of bricks split into several units
 // (at most capacity/H + 1 such bricks: each holds more than H particles).
 struct DepLayout {
     uint32_t nbricks, H, max_units, max_slots;
-    size_t partial, ucount, slot_of, brick, scan, ctr, arrive, htile, bytes;
+    size_t partial, ucount, slot_of, brick, scan, ctr, slot_brick, htile, bytes;
 };
 
 DepLayout dep_layout(const BoxSet &bs)
@@ -1082,7 +1113,7 @@ DepLayout dep_layout(const BoxSet &bs)
     d.brick = take(sizeof(uint32_t) * (size_t)d.max_units);
     d.scan = take(sizeof(uint32_t) * scan_scratch_words(d.nbricks + 1));
     d.ctr = take(16);
-    d.arrive = take(sizeof(uint32_t) * (size_t)d.max_slots);
+    d.slot_brick = take(sizeof(uint32_t) * (size_t)d.max_slots);
     d.htile = take(sizeof(unsigned long long) * kDepNodes * (size_t)d.max_slots);
     d.bytes = o;
     return d;
@@ -1118,8 +1149,10 @@ int pm_deposit(const void *spos, const uint32_t *skeys, const uint32_t *off,
         uint32_t *ctr = reinterpret_cast<uint32_t *>(base + d.ctr);
         int rc = launch_fill_u32(ctr, 0u, 1, s);
         if (rc) return rc;
+        uint32_t *slot_brick = reinterpret_cast<uint32_t *>(base + d.slot_brick);
+        unsigned long long *htile = reinterpret_cast<unsigned long long *>(base + d.htile);
         k_brick_units<<<grid_for(d.nbricks, 256), 256, 0, s>>>(
-            off, bs.view(mode, kMapBrick), d.H, ucount, slot_of, ctr);
+            off, bs.view(mode, kMapBrick), d.H, ucount, slot_of, slot_brick, ctr);
         CUCO_LAUNCH_CHECK();
         // counts -> first unit per brick, in place; the total after the last
         rc = exclusive_scan_u32(ucount, ucount, d.nbricks,
@@ -1127,12 +1160,12 @@ int pm_deposit(const void *spos, const uint32_t *skeys, const uint32_t *off,
         if (rc) return rc;
         k_brick_units_fill<<<grid_for(d.nbricks, 256), 256, 0, s>>>(ucount, d.nbricks, brick);
         CUCO_LAUNCH_CHECK();
-        k_cic_unit<<<xcd_grid(d.max_units), 256, 0, s>>>(
+        k_cic_unit<<<d.max_units, 256, 0, s>>>(
             Src32{static_cast<const float4 *>(spos)}, skeys, off, bs.view(mode, kMapBrick),
-            units_view(d, scratch), d.max_units, slot_of,
-            reinterpret_cast<uint32_t *>(base + d.arrive),
-            reinterpret_cast<unsigned long long *>(base + d.htile), partial,
-            reinterpret_cast<float *>(rho));
+            units_view(d, scratch), slot_of, htile, partial, reinterpret_cast<float *>(rho));
+        CUCO_LAUNCH_CHECK();
+        k_cic_split<<<d.max_slots, 256, 0, s>>>(bs.view(mode, kMapBrick), ctr, slot_brick,
+                                                htile, partial, reinterpret_cast<float *>(rho));
         CUCO_LAUNCH_CHECK();
         k_cic_shell<<<xcd_grid(bs.total[kMapBrick]), 256, 0, s>>>(
             partial, bs.view(mode, kMapBrick), reinterpret_cast<float *>(rho));
@@ -1156,12 +1189,12 @@ int pm_deposit(const void *spos, const uint32_t *skeys, const uint32_t *off,
 
 int pm_deposit_scratch_init(const BoxSet &bs, void *scratch, hipStream_t s)
 {
-    // The split-brick tiles and their arrival counters start at zero; every
-    // deposit leaves them at zero again.
+    // The split-brick tiles start at zero; every deposit leaves them at zero
+    // again.
     const DepLayout d = dep_layout(bs);
     char *base = static_cast<char *>(scratch);
-    return launch_fill_u32(reinterpret_cast<uint32_t *>(base + d.arrive), 0u,
-                           (d.bytes - d.arrive) / sizeof(uint32_t), s);
+    return launch_fill_u32(reinterpret_cast<uint32_t *>(base + d.htile), 0u,
+                           (d.bytes - d.htile) / sizeof(uint32_t), s);
 }
 
 size_t pm_deposit_scratch_bytes(const BoxSet &bs, int fp64)
