@@ -445,8 +445,8 @@ def main() -> int:
                          "a 1 GiB device copy of known size, nothing else")
     ap.add_argument("--tuning", default=None,
                     help="skip the autotuner and use SHIFT,WG,OVERLAP,SYMMETRIC"
-                         "[,CELL_LEVEL,FFT_SPLIT] (e.g. 3,8,0,1): a run for a clean "
-                         "kernel trace")
+                         "[,CELL_LEVEL,FFT_SPLIT,WIDE_TILES] (e.g. 3,8,0,1): a run for "
+                         "a clean kernel trace")
     ap.add_argument("--pp-symmetric", choices=["auto", "0", "1"], default="auto",
                     help="FP32 pair force evaluating each pair once: the "
                          "autotuner's choice, or forced off / on")
@@ -500,10 +500,10 @@ def main() -> int:
     # FP32 chosen path: autotune (outside the timed region), warmup, steps.
     t0 = time.perf_counter()
     if args.tuning:
-        v = [int(x) for x in args.tuning.split(",")] + [0, 0]
+        v = [int(x) for x in args.tuning.split(",")] + [0, 0, 0]
         tuning = dict(pp_block_shift=v[0], pp_workgroups_per_cu=v[1],
                       pm_overlap=v[2], pp_symmetric=v[3], pp_cell_level=v[4],
-                      fft_split=v[5])
+                      fft_split=v[5], pp_wide_tiles=v[6])
         eng.set_tuning(**tuning)
         log(f"[bench] tuning {tuning} (given)")
     else:

@@ -157,7 +157,7 @@ struct BoxDev {
     uint32_t tab0;      // first entry of the twiddle / Green tables (M each)
     int32_t pad0;
     double pm_fac;      // G/(pi L) * M/(2L) (pm_periodic.c:85-86)
-    double dep_scale;   // FP32 deposit: 64-bit fixed-point units per unit mass
+    double dep_mmax;    // FP32 deposit: the box's largest particle mass (fixed-point bound)
     FftPlan half, full; // plans for length M/2 (rows) and M (lines)
 };
 

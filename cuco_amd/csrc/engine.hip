@@ -79,7 +79,6 @@ struct HostBox {
     PairParams pp{};
     GridInfo g[2]{};   // per KeyMode
     double pm_fac = 0;
-    double mass_total = 0;  // sum of the box's particle masses
     double mass_max = 0;    // largest particle mass of the box
 };
 
@@ -131,6 +130,7 @@ struct cuco_engine {
     int pp32_workgroups = 0;
     int pp32_wg_per_cu = 8;
     int pp_cell_level = 0;
+    int pp_wide_tiles = 0;  // each-pair-once tiles of 128 targets
     int ncu = 256;
     int min_block_shift = 2;
 
@@ -346,7 +346,7 @@ int update_boxes(cuco_engine *e)
             d.cgrid0 = cgrid0;
             d.tab0 = (uint32_t)tab0;
             d.pm_fac = h.pm_fac;
-            d.dep_scale = deposit_scale(h.mass_total);
+            d.dep_mmax = h.mass_max;
             d.half = make_plan(M / 2);
             d.full = make_plan(M);
             kb[m] += h.g[m].nkeys;
@@ -1117,7 +1117,7 @@ int upload_impl(cuco_engine *e, int64_t n, const double *pos, const double *vel,
     for (size_t b = 0; b < e->box.size(); ++b) {
         const double L = e->box[b].prm.box_size;
         const int64_t nb = e->ensemble ? (int64_t)e->box[b].cap : n;
-        double mtot = 0, mmax = 0;
+        double mmax = 0;
         for (int64_t k = 0; k < nb; ++k, ++i) {
             double x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
             if (!std::isfinite(x) || !std::isfinite(y) || !std::isfinite(z)) {
@@ -1129,12 +1129,10 @@ int upload_impl(cuco_engine *e, int64_t n, const double *pos, const double *vel,
             TRY(wrap_host(&z, L));
             hp[i] = make_double4(x, y, z, mass ? mass[i] : uniform_mass);
             if (hp[i].w != common) common = 0;
-            mtot += hp[i].w;
             mmax = std::max(mmax, hp[i].w);
             hv[i] = vel ? make_double4(vel[3 * i], vel[3 * i + 1], vel[3 * i + 2], 0.0)
                         : make_double4(0, 0, 0, 0);
         }
-        e->box[b].mass_total = mtot;
         if (mmax != e->box[b].mass_max) mass_max_changed = true;
         e->box[b].mass_max = mmax;
     }
@@ -1928,6 +1926,7 @@ int cuco_get_tuning(cuco_engine *e, cuco_tuning *t)
     t->pp_symmetric = e->psym.on ? 1 : 0;
     t->pp_cell_level = e->pp_cell_level;
     t->fft_split = e->bs.fft_split;
+    t->pp_wide_tiles = e->pp_wide_tiles;
     return CUCO_OK;
 }
 
@@ -1938,7 +1937,8 @@ int cuco_set_tuning(cuco_engine *e, const cuco_tuning *t)
         t->pp_block_shift >= e->min_bits || t->pp_workgroups_per_cu < 1 ||
         t->pp_workgroups_per_cu > 32 || t->pm_overlap < 0 || t->pm_overlap > 1 ||
         t->pp_symmetric < 0 || t->pp_symmetric > 1 || t->pp_cell_level < 0 ||
-        t->pp_cell_level > 2 || t->fft_split < 0 || t->fft_split > 1) {
+        t->pp_cell_level > 2 || t->fft_split < 0 || t->fft_split > 1 ||
+        t->pp_wide_tiles < 0 || t->pp_wide_tiles > 1) {
         set_error("tuning out of range");
         return CUCO_ERR_INVALID_ARG;
     }
@@ -1970,6 +1970,14 @@ int cuco_set_tuning(cuco_engine *e, const cuco_tuning *t)
         e->psym.on = t->pp_symmetric != 0;
         destroy_graph(e);
     }
+    // Tiles of 128 serve the each-pair-once kernel only.
+    e->pp_wide_tiles = t->pp_wide_tiles;
+    const uint32_t run = e->psym.on && e->pp_wide_tiles ? 128u : 64u;
+    if (run != e->tiles.run) {
+        e->tiles.run = run;
+        if (e->sorted_mode == kKeyMorton) e->sorted_mode = -1;  // rebuild tiles
+        destroy_graph(e);
+    }
     return CUCO_OK;
 }
 
@@ -1982,13 +1990,13 @@ try {
     CUCO_HIP(hipEventCreate(&b));
     const bool timing = e->timing;
     e->timing = false;
-    cuco_tuning best{e->tiles.block_shift, e->pp32_wg_per_cu, 0, 0, 0, 0};
+    cuco_tuning best{e->tiles.block_shift, e->pp32_wg_per_cu, 0, 0, 0, 0, 0};
     float best_t = 1e30f;
     int rc = CUCO_OK;
     for (int shift = e->min_block_shift; shift <= std::min(4, e->min_bits - 1) && !rc; ++shift) {
         for (int cfg = 0; cfg < 6; ++cfg) {
             const int wg = cfg % 3 == 0 ? 4 : cfg % 3 == 1 ? 8 : 16;
-            cuco_tuning t{shift, wg, 0, cfg / 3, 0, 0};
+            cuco_tuning t{shift, wg, 0, cfg / 3, 0, 0, 0};
             rc = cuco_set_tuning(e, &t);
             if (!rc) rc = sort_particles(e, kKeyMorton);
             float ms = 1e30f;
@@ -2012,12 +2020,14 @@ try {
         }
     }
     // Candidate enumeration level of the each-pair-once kernel (mesh cells
-    // or 2x2x2-cell blocks for wide tiles, or either for every tile).
+    // or 2x2x2-cell blocks for wide tiles, or either for every tile), with
+    // tiles of 64 or 128 targets.
     if (!rc && best.pp_symmetric) {
         const cuco_tuning base = best;
-        for (int level = 0; level < 3 && !rc; ++level) {
+        for (int cfg = 0; cfg < 6 && !rc; ++cfg) {
             cuco_tuning t = base;
-            t.pp_cell_level = level;
+            t.pp_cell_level = cfg % 3;
+            t.pp_wide_tiles = cfg / 3;
             rc = cuco_set_tuning(e, &t);
             if (!rc) rc = sort_particles(e, kKeyMorton);
             float ms = 1e30f;

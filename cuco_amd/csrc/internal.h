@@ -101,7 +101,7 @@ int launch_pp64(const double4 *spos, const uint32_t *perm, const uint32_t *off,
 
 struct TileInfo {
     uint32_t *tile_start = nullptr;   // per tile: first sorted index
-    uint32_t *tile_count = nullptr;   // per tile: particles (<= 64)
+    uint32_t *tile_count = nullptr;   // per tile: particles (<= run)
     uint32_t *block_tiles = nullptr;  // per Morton super-block: tiles, then offsets
     uint32_t *counters = nullptr;     // [0] total tiles, [1] work counter
     uint32_t *scan_scratch = nullptr;
@@ -111,6 +111,9 @@ struct TileInfo {
     // Each-pair-once kernel: tiles at least this many cells wide enumerate
     // their candidates by 2x2x2-cell blocks instead of cells.
     float coarse_cells = 4.0f;
+    // Tile length: 64 targets (one per lane), or 128 (two per lane, the
+    // each-pair-once kernel only).
+    uint32_t run = 64;
 };
 
 // Constants of the FP32 pair kernel for one box.  All lengths are scaled by
@@ -185,7 +188,6 @@ size_t pm_deposit_scratch_bytes(const BoxSet &bs, int fp64);
 // Clears the parts of a new FP32 deposit scratch that must start at zero.
 int pm_deposit_scratch_init(const BoxSet &bs, void *scratch, hipStream_t s);
 // Fixed-point scale of the FP32 deposit tiles for a box of total mass m_total.
-double deposit_scale(double m_total);
 template <typename T>
 int pm_fft_forward(const T *rho, Cx<T> *c, const BoxSet &bs, const PmTables &tb,
                    hipStream_t s);

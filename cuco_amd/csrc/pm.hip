@@ -8,7 +8,7 @@
 //           eight neighbouring cells (sorted by cell, read through the CSR
 //           table) with the reference's CIC weights (pm_periodic.c:97-132);
 //           every node is written once, in a fixed order: no atomics.
-//           FP32: per 16^3-cell Morton brick a node tile summed in 64-bit
+//           FP32: per 16^3-cell Morton brick a node tile summed in 32-bit
 //           fixed point in LDS, brick faces added in a fixed order.
 //  FFT      hand-written Stockham autosort FFT in LDS, radix 4/2/3/5 (FP64,
 //           and FP32 meshes that are not powers of two; the FP32
@@ -578,40 +578,59 @@ __global__ __launch_bounds__(256) void k_cic_gather(
 // a brick's particles are one sorted range).  A brick's cells reach the
 // nodes at local coordinates 0..u of its corner per axis, u = 16 (at the
 // upper mesh edge, when M is not a multiple of 16, u = M minus the corner:
-// the node at M is node 0 of the next period); nodes strictly between 0
-// and u along every axis get nothing from other bricks.
+// the node at M is node 0 of the next period).  A brick owns the nodes
+// 0..u-1 on every axis; its nodes at u on some axis (its upper faces) are
+// owned by the bricks above.
 //
 //  k_cic_unit: one block per work unit (a brick, or a run of at most H of
 //    its particles when it holds more).  The block sums its particles' CIC
 //    corners into the 17^3-node tile (a wave-level segmented scan by cell
 //    first where a wave holds repeated cells, then the corner sums added in
-//    64-bit fixed point with LDS atomics: integer adds commute, so the tile
-//    does not depend on the order of waves or particles).  Interior nodes
-//    go to the mesh; the shell (a coordinate 0 or u) to the brick's six
-//    face planes.
+//    32-bit fixed point with LDS atomics: integer adds commute, so the tile
+//    does not depend on the order of waves or particles).  Owned nodes go
+//    to the mesh (those on a lower face still without the bricks below);
+//    the upper faces to the brick's three face planes.
 //  k_cic_shell: one block per brick for its nodes with a local coordinate 0
-//    (the lower faces): the brick's own shell value plus, along every axis
-//    where the coordinate is 0, the brick below at its u (periodic), summed
-//    in a fixed order.
+//    (the lower faces): the brick's own value from the mesh plus, along
+//    every subset of the axes where the coordinate is 0, the brick below at
+//    its u (periodic) from its planes, summed in a fixed order.
+//
+// Fixed point: a brick of n particles, none heavier than m_max, puts at
+// most n m_max on a node, so with S = the largest power of two where
+// S n m_max <= 2^31, a node's sum (each contribution rounded to the
+// nearest unit, at most n of them) stays below 2^32.  Scaling by S is exact
+// in FP32; a unit is n m_max 2^-31 or less: 4.8e-7 particle masses in a
+// brick of 512.
 constexpr int kDepBrick = 16;
 constexpr int kDepEdge = kDepBrick + 1;
 constexpr int kDepNodes = kDepEdge * kDepEdge * kDepEdge;
 constexpr uint32_t kDepCells = kDepBrick * kDepBrick * kDepBrick;
 constexpr int kDepFace = kDepEdge * kDepEdge;
-constexpr int kDepShell = 6 * kDepFace;  // floats of a brick's face planes
+constexpr int kDepShell = 3 * kDepFace;  // floats of a brick's upper face planes
 
 // Upper node coordinate per axis of the brick at corner o: 16, or M - o at
 // the upper mesh edge.
 __device__ inline int dep_upper(int o, int M) { return min(kDepBrick, M - o); }
 
-// Position of shell node l in a brick's face planes (0: x = 0, 1: x = u,
-// 2: y = 0, 3: y = u, 4: z = 0, 5: z = u, each 17 x 17 over the other two
-// coordinates); a node on several faces is kept on the first.
-__device__ inline int shell_slot(int lx, int ly, int lz, int ux, int uy, int uz)
+// Position of upper-face node l in a brick's face planes (0: x = u, 17 x 17
+// over (y, z); 1: y = u with x < u, over (x, z); 2: z = u with x, y < u,
+// over (x, y)).
+__device__ inline int upper_slot(int lx, int ly, int lz, int ux, int uy)
 {
-    if (lx == 0 || lx == ux) return (lx ? kDepFace : 0) + ly * kDepEdge + lz;
-    if (ly == 0 || ly == uy) return (ly ? 3 : 2) * kDepFace + lx * kDepEdge + lz;
-    return (lz ? 5 : 4) * kDepFace + lx * kDepEdge + ly;
+    if (lx == ux) return ly * kDepEdge + lz;
+    if (ly == uy) return kDepFace + lx * kDepEdge + lz;
+    return 2 * kDepFace + lx * kDepEdge + ly;
+}
+
+// Fixed-point units per unit mass of a brick of n particles (see above);
+// the exponent is clamped to FP32's normal range.
+__device__ inline float brick_scale(uint32_t n, double m_max)
+{
+    const double bound = (double)max(n, 1u) * m_max;
+    if (!(bound > 0.0)) return 1.0f;
+    int ex = 0;
+    (void)frexp(bound, &ex);  // bound < 2^ex
+    return ldexpf(1.0f, min(120, max(-120, 31 - ex)));
 }
 
 // Work units of the FP32 deposit and interpolation: a brick's particles in
@@ -671,29 +690,41 @@ __device__ inline bool unit_of(const Units &un, const BoxView &v, uint32_t u, in
     return true;
 }
 
-// The brick's tile (64-bit fixed point, in LDS) to the mesh and the face
-// planes: interior nodes are complete; the shell waits for k_cic_shell;
-// nodes past u get nothing.  A thread owns node columns (ly, lz) and walks
-// them along x.
-__device__ inline void write_brick(const unsigned long long *s_node, double scale, int x0,
-                                   int y0, int z0, int M, float *__restrict__ pt,
-                                   float *__restrict__ rho)
+// The brick's tile (32-bit fixed point, in LDS; inv = 1/S) to the mesh
+// and the face planes: owned nodes to the mesh, upper faces to the planes;
+// nodes past u get nothing.  Thread t takes the node column (ly, lz) =
+// (t / 16, t % 16) along x (the mesh rows of 16 floats at a time), then the
+// nodes at ly = 16 or lz = 16.  Needs 256 threads.
+__device__ inline void write_brick(const uint32_t *s_node, float inv, int x0, int y0, int z0,
+                                   int M, float *__restrict__ pt, float *__restrict__ rho)
 {
-    const double inv = 1.0 / scale;
     const int ux = dep_upper(x0, M), uy = dep_upper(y0, M), uz = dep_upper(z0, M);
-    for (int col = threadIdx.x; col < kDepFace; col += 256) {
-        const int ly = col / kDepEdge, lz = col - ly * kDepEdge;
-        if (ly > uy || lz > uz) continue;
-        const bool yz_in = ly > 0 && lz > 0 && ly < uy && lz < uz;
-        const size_t yz = (size_t)(y0 + ly) * M + (z0 + lz);
-        for (int lx = 0; lx <= ux; ++lx) {
-            const int e = lx * kDepFace + col;
-            const float val = (float)((double)(long long)s_node[e] * inv);
-            if (yz_in && lx > 0 && lx < ux)
-                rho[(size_t)(x0 + lx) * M * M + yz] = val;
+    const int ly = threadIdx.x >> 4, lz = threadIdx.x & 15;
+    if (ly <= uy && lz <= uz) {
+        const bool yz_own = ly < uy && lz < uz;
+        const size_t mm = (size_t)M * M;
+        float *out = rho + (size_t)x0 * mm + (size_t)(y0 + ly) * M + (z0 + lz);
+        for (int lx = 0; lx <= ux; ++lx, out += mm) {
+            const float val = (float)s_node[(lx * kDepEdge + ly) * kDepEdge + lz] * inv;
+            if (yz_own && lx < ux)
+                *out = val;
             else
-                pt[shell_slot(lx, ly, lz, ux, uy, uz)] = val;
+                pt[upper_slot(lx, ly, lz, ux, uy)] = val;
         }
+    }
+    // ly = 16 (only on the upper y face when uy = 16), then lz = 16 with
+    // ly < 16 (the upper z face when uz = 16).
+    for (int e = threadIdx.x; e < kDepFace + kDepEdge * kDepBrick; e += 256) {
+        int lx, ly2, lz2;
+        if (e < kDepFace) {
+            lx = e / kDepEdge; ly2 = kDepBrick; lz2 = e - lx * kDepEdge;
+        } else {
+            const int f = e - kDepFace;
+            lx = f / kDepBrick; ly2 = f - lx * kDepBrick; lz2 = kDepBrick;
+        }
+        if (lx > ux || ly2 > uy || lz2 > uz) continue;
+        pt[upper_slot(lx, ly2, lz2, ux, uy)] =
+            (float)s_node[(lx * kDepEdge + ly2) * kDepEdge + lz2] * inv;
     }
 }
 
@@ -703,11 +734,11 @@ __device__ inline void write_brick(const unsigned long long *s_node, double scal
 __global__ __launch_bounds__(256) void k_cic_unit(Src32 src, const uint32_t *__restrict__ skeys,
                                                   const uint32_t *__restrict__ off, BoxView v,
                                                   Units un, const uint32_t *__restrict__ slot_of,
-                                                  unsigned long long *__restrict__ htile,
+                                                  uint32_t *__restrict__ htile,
                                                   float *__restrict__ partial,
                                                   float *__restrict__ rho_all)
 {
-    __shared__ unsigned long long s_node[kDepNodes];
+    __shared__ uint32_t s_node[kDepNodes];
     int bi;
     uint32_t lb, part, nparts;
     if (!unit_of(un, v, blockIdx.x, bi, lb, part, nparts)) return;
@@ -718,14 +749,14 @@ __global__ __launch_bounds__(256) void k_cic_unit(Src32 src, const uint32_t *__r
     int x0, y0, z0;
     key_cell(g, k0, x0, y0, z0);
     if (x0 >= M || y0 >= M || z0 >= M) return;  // outside the mesh: no cells
-    for (int e = threadIdx.x; e < kDepNodes; e += 256) s_node[e] = 0ull;
+    for (int e = threadIdx.x; e < kDepNodes; e += 256) s_node[e] = 0u;
     const uint32_t b_begin = off[bx.key_base + k0];
     const uint32_t b_end = off[bx.key_base + k0 + kDepCells];
     const uint32_t p_begin = b_begin + part * un.H;
     const uint32_t p_end = min(b_end, p_begin + un.H);
     __syncthreads();
     const int lane = threadIdx.x & 63;
-    const double scale = bx.dep_scale;
+    const float scale = brick_scale(b_end - b_begin, bx.dep_mmax);
     for (uint32_t base = p_begin; base < p_end; base += 256) {
         const uint32_t i = base + threadIdx.x;
         const bool valid = i < p_end;
@@ -764,35 +795,36 @@ __global__ __launch_bounds__(256) void k_cic_unit(Src32 src, const uint32_t *__r
 #pragma unroll
             for (int c = 0; c < 8; ++c)
                 atomicAdd(&s_node[n0 + ((c >> 2) * kDepEdge + ((c >> 1) & 1)) * kDepEdge + (c & 1)],
-                          (unsigned long long)__double2ll_rn((double)val[c] * scale));
+                          __float2uint_rn(val[c] * scale));
         }
     }
     __syncthreads();
     if (nparts > 1) {
         // A brick of several units: every unit adds its tile into the
-        // brick's 64-bit tile in global memory (integer adds: the sum does
-        // not depend on the order); k_cic_split writes the brick.
-        unsigned long long *tile = htile + (size_t)slot_of[v.map[bi] + lb] * kDepNodes;
+        // brick's tile in global memory (integer adds: the sum does not
+        // depend on the order); k_cic_split writes the brick.
+        uint32_t *tile = htile + (size_t)slot_of[v.map[bi] + lb] * kDepNodes;
         for (int e = threadIdx.x; e < kDepNodes; e += 256) {
-            const unsigned long long t = s_node[e];
+            const uint32_t t = s_node[e];
             if (t) atomicAdd(&tile[e], t);
         }
         return;
     }
-    write_brick(s_node, scale, x0, y0, z0, M, partial + (size_t)(v.map[bi] + lb) * kDepShell,
-                rho_all + bx.grid0);
+    write_brick(s_node, 1.0f / scale, x0, y0, z0, M,
+                partial + (size_t)(v.map[bi] + lb) * kDepShell, rho_all + bx.grid0);
 }
 
-// Bricks split over several units: the summed 64-bit tile (complete once
+// Bricks split over several units: the summed tile (complete once
 // k_cic_unit has finished) to the mesh and face planes; the tile is cleared
 // for the next deposit.
-__global__ __launch_bounds__(256) void k_cic_split(BoxView v, const uint32_t *__restrict__ nsplit,
+__global__ __launch_bounds__(256) void k_cic_split(BoxView v, const uint32_t *__restrict__ off,
+                                                   const uint32_t *__restrict__ nsplit,
                                                    const uint32_t *__restrict__ slot_brick,
-                                                   unsigned long long *__restrict__ htile,
+                                                   uint32_t *__restrict__ htile,
                                                    float *__restrict__ partial,
                                                    float *__restrict__ rho_all)
 {
-    __shared__ unsigned long long s_node[kDepNodes];
+    __shared__ uint32_t s_node[kDepNodes];
     const uint32_t slot = blockIdx.x;
     if (slot >= __builtin_amdgcn_readfirstlane(nsplit[0])) return;
     const uint32_t gb = __builtin_amdgcn_readfirstlane(slot_brick[slot]);
@@ -801,13 +833,15 @@ __global__ __launch_bounds__(256) void k_cic_split(BoxView v, const uint32_t *__
     const BoxDev &bx = v.box[bi];
     int x0, y0, z0;
     key_cell(bx.g, lb * kDepCells, x0, y0, z0);
-    unsigned long long *tile = htile + (size_t)slot * kDepNodes;
+    uint32_t *tile = htile + (size_t)slot * kDepNodes;
     for (int e = threadIdx.x; e < kDepNodes; e += 256) {
         s_node[e] = tile[e];
-        tile[e] = 0ull;
+        tile[e] = 0u;
     }
+    const uint32_t k0 = bx.key_base + lb * kDepCells;
+    const float scale = brick_scale(off[k0 + kDepCells] - off[k0], bx.dep_mmax);
     __syncthreads();
-    write_brick(s_node, bx.dep_scale, x0, y0, z0, bx.g.M, partial + (size_t)gb * kDepShell,
+    write_brick(s_node, 1.0f / scale, x0, y0, z0, bx.g.M, partial + (size_t)gb * kDepShell,
                 rho_all + bx.grid0);
 }
 
@@ -836,8 +870,9 @@ __global__ __launch_bounds__(256) void k_cic_shell(const float *__restrict__ par
     // Face planes of the brick below along the set bits of c (bit 2: x,
     // 1: y, 0: z).
     const float *tile[8];
+    tile[0] = nullptr;
 #pragma unroll
-    for (int c = 0; c < 8; ++c) {
+    for (int c = 1; c < 8; ++c) {
         const uint32_t brick =
             cell_key(g, (c & 4) ? below[0] : o[0], (c & 2) ? below[1] : o[1],
                      (c & 1) ? below[2] : o[2]) / kDepCells;
@@ -859,18 +894,19 @@ __global__ __launch_bounds__(256) void k_cic_shell(const float *__restrict__ par
             const int f = e - kFx - kFy;
             l[0] = 1 + f / (kDepBrick - 1); l[1] = 1 + f % (kDepBrick - 1); l[2] = 0;
         }
-        if (o[0] + l[0] >= M || o[1] + l[1] >= M || o[2] + l[2] >= M) continue;
-        float sum = 0.f;
+        if (l[0] >= ux || l[1] >= uy || l[2] >= uz) continue;
+        float *node = rho + ((size_t)(o[0] + l[0]) * M + (o[1] + l[1])) * M + (o[2] + l[2]);
+        float sum = *node;  // the brick's own particles
 #pragma unroll
-        for (int c = 0; c < 8; ++c) {
+        for (int c = 1; c < 8; ++c) {
             if (((c & 4) && l[0]) || ((c & 2) && l[1]) || ((c & 1) && l[2])) continue;
-            // The node in the brick below: its upper face along the axes of c.
+            // The node in the brick below: on its upper face along the axes of c.
             const int ax = (c & 4) ? lbel[0] : l[0], ay = (c & 2) ? lbel[1] : l[1],
                       az = (c & 1) ? lbel[2] : l[2];
-            sum += tile[c][shell_slot(ax, ay, az, (c & 4) ? lbel[0] : ux, (c & 2) ? lbel[1] : uy,
-                                      (c & 1) ? lbel[2] : uz)];
+            sum += tile[c][upper_slot(ax, ay, az, (c & 4) ? lbel[0] : ux,
+                                      (c & 2) ? lbel[1] : uy)];
         }
-        rho[((size_t)(o[0] + l[0]) * M + (o[1] + l[1])) * M + (o[2] + l[2])] = sum;
+        *node = sum;
     }
 }
 
@@ -1083,7 +1119,7 @@ int launch_fft_kernel(K kernel, unsigned grid, hipStream_t s, Args... args)
 }
 
 // Byte layout of the FP32 deposit scratch: the brick face planes, the
-// work-unit tables, and the 64-bit tiles of bricks split into several units
+// work-unit tables, and the 32-bit tiles of bricks split into several units
 // (at most capacity/H + 1 such bricks: each holds more than H particles).
 struct DepLayout {
     uint32_t nbricks, H, max_units, max_slots;
@@ -1114,7 +1150,7 @@ DepLayout dep_layout(const BoxSet &bs)
     d.scan = take(sizeof(uint32_t) * scan_scratch_words(d.nbricks + 1));
     d.ctr = take(16);
     d.slot_brick = take(sizeof(uint32_t) * (size_t)d.max_slots);
-    d.htile = take(sizeof(unsigned long long) * kDepNodes * (size_t)d.max_slots);
+    d.htile = take(sizeof(uint32_t) * kDepNodes * (size_t)d.max_slots);
     d.bytes = o;
     return d;
 }
@@ -1150,7 +1186,7 @@ int pm_deposit(const void *spos, const uint32_t *skeys, const uint32_t *off,
         int rc = launch_fill_u32(ctr, 0u, 1, s);
         if (rc) return rc;
         uint32_t *slot_brick = reinterpret_cast<uint32_t *>(base + d.slot_brick);
-        unsigned long long *htile = reinterpret_cast<unsigned long long *>(base + d.htile);
+        uint32_t *htile = reinterpret_cast<uint32_t *>(base + d.htile);
         k_brick_units<<<grid_for(d.nbricks, 256), 256, 0, s>>>(
             off, bs.view(mode, kMapBrick), d.H, ucount, slot_of, slot_brick, ctr);
         CUCO_LAUNCH_CHECK();
@@ -1164,7 +1200,7 @@ int pm_deposit(const void *spos, const uint32_t *skeys, const uint32_t *off,
             Src32{static_cast<const float4 *>(spos)}, skeys, off, bs.view(mode, kMapBrick),
             units_view(d, scratch), slot_of, htile, partial, reinterpret_cast<float *>(rho));
         CUCO_LAUNCH_CHECK();
-        k_cic_split<<<d.max_slots, 256, 0, s>>>(bs.view(mode, kMapBrick), ctr, slot_brick,
+        k_cic_split<<<d.max_slots, 256, 0, s>>>(bs.view(mode, kMapBrick), off, ctr, slot_brick,
                                                 htile, partial, reinterpret_cast<float *>(rho));
         CUCO_LAUNCH_CHECK();
         k_cic_shell<<<xcd_grid(bs.total[kMapBrick]), 256, 0, s>>>(
@@ -1203,15 +1239,6 @@ size_t pm_deposit_scratch_bytes(const BoxSet &bs, int fp64)
     return sizeof(double) * 8 * ((size_t)bs.capacity + (size_t)bs.wave_slots * 2 + 64);
 }
 
-double deposit_scale(double m_total)
-{
-    // m_total * scale < 2^61: no node of a tile can reach the int64 range,
-    // with room for the rounding of every contribution.
-    if (!(m_total > 0) || !std::isfinite(m_total)) return 1.0;
-    int ex = 0;
-    (void)std::frexp(m_total, &ex);  // m_total < 2^ex
-    return std::ldexp(1.0, 61 - ex);
-}
 
 uint32_t pm_blocks(int M, int map, int fft_split)
 {

@@ -283,6 +283,92 @@ __device__ inline void interact_sym(const float4 *__restrict__ rot,
     }
 }
 
+// Two targets per lane (a tile of up to 128): at each step the staged j
+// meets both, its position and index are read once, and the j-side sum
+// takes both terms before the one rotation -- 20.5 vector instructions per
+// pair slot instead of 22.  Same masks and laws as interact_sym.
+template <bool GEN, bool UNI, bool MASKED, bool NEAR, bool CLAMP>
+__device__ inline void interact_sym2(const float4 *__restrict__ rot,
+                                     const uint32_t *__restrict__ rotid, const P32 &pp,
+                                     const float *__restrict__ tab, float xa, float ya, float za,
+                                     float ma, uint32_t iida, float xb, float yb, float zb,
+                                     float mb, uint32_t iidb, float &sxa, float &sya,
+                                     float &sza, float &sxb, float &syb, float &szb,
+                                     float &jx, float &jy, float &jz)
+{
+    const int lane = threadIdx.x & 63;
+    const float4 *rl = rot + lane;
+    const uint32_t *il = rotid + lane;
+    for (int s = 0; s < 64; s += 2) {
+        float dxa[2], dya[2], dza[2], ra[2], fa[2], ta[2];
+        float dxb[2], dyb[2], dzb[2], rb[2], fb[2], tb[2], mj[2];
+        bool keepa[2], keepb[2], ina[2], inb[2];
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const float4 p = rl[s + q];
+            mj[q] = p.w;
+            if (MASKED) {
+                const uint32_t id = il[s + q];
+                keepa[q] = id > iida;
+                keepb[q] = id > iidb;
+            }
+            dxa[q] = p.x - xa;
+            dya[q] = p.y - ya;
+            dza[q] = p.z - za;
+            dxb[q] = p.x - xb;
+            dyb[q] = p.y - yb;
+            dzb[q] = p.z - zb;
+            const float r2a = fmaf(dxa[q], dxa[q], fmaf(dya[q], dya[q], fmaf(dza[q], dza[q], pp.eps)));
+            const float r2b = fmaf(dxb[q], dxb[q], fmaf(dyb[q], dyb[q], fmaf(dzb[q], dzb[q], pp.eps)));
+            const float rinva = __builtin_amdgcn_rsqf(r2a);
+            const float rinvb = __builtin_amdgcn_rsqf(r2b);
+            ra[q] = r2a * rinva;
+            rb[q] = r2b * rinvb;
+            fa[q] = rinva * rinva * rinva;
+            fb[q] = rinvb * rinvb * rinvb;
+            if (GEN) {
+                ina[q] = r2a <= pp.rcut2;
+                inb[q] = r2b <= pp.rcut2;
+            }
+            ta[q] = tab[(CLAMP || GEN) ? min((int)ra[q], kTabSym - 1) : (int)ra[q]];
+            tb[q] = tab[(CLAMP || GEN) ? min((int)rb[q], kTabSym - 1) : (int)rb[q]];
+        }
+        if (NEAR) {
+            const bool close = (ra[0] < pp.h) | (ra[1] < pp.h) | (rb[0] < pp.h) | (rb[1] < pp.h);
+            if (close) {
+#pragma unroll
+                for (int q = 0; q < 2; ++q) {
+                    fa[q] = spline_fac<true>(ra[q], fa[q], 1.0f, pp);
+                    fb[q] = spline_fac<true>(rb[q], fb[q], 1.0f, pp);
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            float f1 = fa[q] * ta[q], f2 = fb[q] * tb[q];
+            if (MASKED) {
+                f1 = keepa[q] ? f1 : 0.0f;
+                f2 = keepb[q] ? f2 : 0.0f;
+            }
+            if (GEN) {
+                f1 = ina[q] ? f1 : 0.0f;
+                f2 = inb[q] ? f2 : 0.0f;
+            }
+            const float fia = UNI ? f1 : f1 * mj[q], fja = UNI ? f1 : f1 * ma;
+            const float fib = UNI ? f2 : f2 * mj[q], fjb = UNI ? f2 : f2 * mb;
+            sxa = fmaf(dxa[q], fia, sxa);
+            sya = fmaf(dya[q], fia, sya);
+            sza = fmaf(dza[q], fia, sza);
+            sxb = fmaf(dxb[q], fib, sxb);
+            syb = fmaf(dyb[q], fib, syb);
+            szb = fmaf(dzb[q], fib, szb);
+            jx = rotate_down(fmaf(-dxb[q], fjb, fmaf(-dxa[q], fja, jx)));
+            jy = rotate_down(fmaf(-dyb[q], fjb, fmaf(-dya[q], fja, jy)));
+            jz = rotate_down(fmaf(-dzb[q], fjb, fmaf(-dza[q], fja, jz)));
+        }
+    }
+}
+
 // True when some staged j (lanes with in set) lies within h of the tile's
 // bounding box; otherwise every pair of the chunk has r >= h and takes the
 // plain law.  Wave-uniform.
@@ -301,8 +387,10 @@ __device__ inline bool chunk_near(float4 q, bool in, float bminx, float bminy, f
 // SYM: each pair once (section above); the tile's own sums go to iacc
 // (sorted order, unscaled) and the j side to jacc (three 64-bit fixed-point
 // planes of n_sorted entries); a second kernel combines them.
-template <bool GENERIC, bool UNIFORM, bool SYM>
-__global__ __launch_bounds__(kThreads, SYM ? 5 : 1) void k_pp32(
+// TPL = 2 (each pair once only): a tile holds up to 128 targets, lane L
+// targets L and 64 + L.
+template <bool GENERIC, bool UNIFORM, bool SYM, int TPL>
+__global__ __launch_bounds__(kThreads, SYM ? (TPL == 2 ? 4 : 5) : 1) void k_pp32(
     const float4 *__restrict__ spos, const uint32_t *__restrict__ skeys,
     const uint32_t *__restrict__ perm, const uint32_t *__restrict__ off,
     BoxView tv, const P32 *__restrict__ p32s, float mass,
@@ -376,13 +464,26 @@ __global__ __launch_bounds__(kThreads, SYM ? 5 : 1) void k_pp32(
         const float xi = pi.x * pp.s + (float)unwrap_delta(cx - c0x, M) * pp.cellf;
         const float yi = pi.y * pp.s + (float)unwrap_delta(cy - c0y, M) * pp.cellf;
         const float zi = pi.z * pp.s + (float)unwrap_delta(cz - c0z, M) * pp.cellf;
+        // The lane's second target (TPL = 2): 64 + L of the tile.
+        const bool valid2 = TPL == 2 && (uint32_t)lane + 64u < cnt;
+        const uint32_t i2 = valid2 ? i0 + 64u + (uint32_t)lane : i;
+        float xi2 = xi, yi2 = yi, zi2 = zi, mi2 = pi.w;
+        if (TPL == 2) {
+            int cx2, cy2, cz2;
+            key_cell(g, skeys[i2] - key_base, cx2, cy2, cz2);
+            const float4 p2 = spos[i2];
+            xi2 = p2.x * pp.s + (float)unwrap_delta(cx2 - c0x, M) * pp.cellf;
+            yi2 = p2.y * pp.s + (float)unwrap_delta(cy2 - c0y, M) * pp.cellf;
+            zi2 = p2.z * pp.s + (float)unwrap_delta(cz2 - c0z, M) * pp.cellf;
+            mi2 = p2.w;
+        }
 
-        const float bminx = uni(wave_min(valid ? xi : inf));
-        const float bminy = uni(wave_min(valid ? yi : inf));
-        const float bminz = uni(wave_min(valid ? zi : inf));
-        const float bmaxx = uni(wave_max(valid ? xi : -inf));
-        const float bmaxy = uni(wave_max(valid ? yi : -inf));
-        const float bmaxz = uni(wave_max(valid ? zi : -inf));
+        const float bminx = uni(wave_min(fminf(valid ? xi : inf, valid2 ? xi2 : inf)));
+        const float bminy = uni(wave_min(fminf(valid ? yi : inf, valid2 ? yi2 : inf)));
+        const float bminz = uni(wave_min(fminf(valid ? zi : inf, valid2 ? zi2 : inf)));
+        const float bmaxx = uni(wave_max(fmaxf(valid ? xi : -inf, valid2 ? xi2 : -inf)));
+        const float bmaxy = uni(wave_max(fmaxf(valid ? yi : -inf, valid2 ? yi2 : -inf)));
+        const float bmaxz = uni(wave_max(fmaxf(valid ? zi : -inf, valid2 ? zi2 : -inf)));
 
         const int lqx = (int)floorf((bminx - pp.rcut) * pp.inv_cellf - 0.01f);
         const int lqy = (int)floorf((bminy - pp.rcut) * pp.inv_cellf - 0.01f);
@@ -416,6 +517,7 @@ __global__ __launch_bounds__(kThreads, SYM ? 5 : 1) void k_pp32(
         const uint32_t total_q = uni(sane ? (uint32_t)nux * nuy * nuz : 0u);
 
         float ax = 0.f, ay = 0.f, az = 0.f, kx = 0.f, ky = 0.f, kz = 0.f;
+        float ax2 = 0.f, ay2 = 0.f, az2 = 0.f, kx2 = 0.f, ky2 = 0.f, kz2 = 0.f;
         // Every candidate lies within sqrt(filter2) of the tile's box, so a
         // pair's distance is at most that plus the box diagonal; below the
         // last bin of the LDS table (with margin for rounding) the bin needs
@@ -425,7 +527,9 @@ __global__ __launch_bounds__(kThreads, SYM ? 5 : 1) void k_pp32(
             !(sqrtf(ex * ex + ey * ey + ez * ez) + sqrtf(pp.filter2) < (float)(kT - 2));
         // SYM: the j side of one staged chunk, added to jacc (fixed point).
         const uint32_t iid = valid ? i : 0xffffffffu;
-        auto eval_sym = [&](int nj, float &sx_, float &sy_, float &sz_) {
+        const uint32_t iid2 = valid2 ? i2 : 0xffffffffu;
+        auto eval_sym = [&](int nj, float &sx_, float &sy_, float &sz_, float &sx2_, float &sy2_,
+                            float &sz2_) {
             float jx = 0.f, jy = 0.f, jz = 0.f;
             // Copy the chunk twice into the rotation buffer; unused slots of a
             // partial chunk get index 0 (always masked) and a position inside
@@ -446,14 +550,22 @@ __global__ __launch_bounds__(kThreads, SYM ? 5 : 1) void k_pp32(
             const bool inside = __ballot(lane < nj && mid < i0 + cnt) != 0ull;
             const bool near = chunk_near(mine, lane < nj, bminx, bminy, bminz, bmaxx, bmaxy,
                                          bmaxz, pp.h);
-            const bool masked = nj < 64 || cnt < 64u || inside;
-#define CUCO_SYM(MK, NR, CL) \
-    interact_sym<GENERIC, UNIFORM, MK, NR, CL>(rot, rotid, pp, s_tab, xi, yi, zi, pi.w, iid, sx_, \
-                                               sy_, sz_, jx, jy, jz)
+            const bool masked = nj < 64 || cnt < 64u * TPL || inside;
+#define CUCO_SYM(MK, NR, CL)                                                                    \
+    if (TPL == 2)                                                                               \
+        interact_sym2<GENERIC, UNIFORM, MK, NR, CL>(rot, rotid, pp, s_tab, xi, yi, zi, pi.w, iid, \
+                                                    xi2, yi2, zi2, mi2, iid2, sx_, sy_, sz_,    \
+                                                    sx2_, sy2_, sz2_, jx, jy, jz);              \
+    else                                                                                        \
+        interact_sym<GENERIC, UNIFORM, MK, NR, CL>(rot, rotid, pp, s_tab, xi, yi, zi, pi.w, iid, \
+                                                   sx_, sy_, sz_, jx, jy, jz)
             if (clamp) {
                 if (masked) {
-                    if (near) CUCO_SYM(true, true, true);
-                    else CUCO_SYM(true, false, true);
+                    if (near) {
+                        CUCO_SYM(true, true, true);
+                    } else {
+                        CUCO_SYM(true, false, true);
+                    }
                 } else if (near) {
                     CUCO_SYM(false, true, true);
                 } else {
@@ -461,8 +573,11 @@ __global__ __launch_bounds__(kThreads, SYM ? 5 : 1) void k_pp32(
                 }
             } else {
                 if (masked) {
-                    if (near) CUCO_SYM(true, true, false);
-                    else CUCO_SYM(true, false, false);
+                    if (near) {
+                        CUCO_SYM(true, true, false);
+                    } else {
+                        CUCO_SYM(true, false, false);
+                    }
                 } else if (near) {
                     CUCO_SYM(false, true, false);
                 } else {
@@ -663,15 +778,20 @@ __global__ __launch_bounds__(kThreads, SYM ? 5 : 1) void k_pp32(
                 jc = uni(jc + (uint32_t)__popcll(bal));
                 wave_sync();
                 if (jc >= 64) {
-                    float sx = 0.f, sy = 0.f, sz = 0.f;
+                    float sx = 0.f, sy = 0.f, sz = 0.f, sx2 = 0.f, sy2 = 0.f, sz2 = 0.f;
                     if (SYM) {
-                        eval_sym(64, sx, sy, sz);
+                        eval_sym(64, sx, sy, sz, sx2, sy2, sz2);
                     } else {
                         eval_default(64, sx, sy, sz);
                     }
                     kahan_add(ax, kx, sx);
                     kahan_add(ay, ky, sy);
                     kahan_add(az, kz, sz);
+                    if (TPL == 2) {
+                        kahan_add(ax2, kx2, sx2);
+                        kahan_add(ay2, ky2, sy2);
+                        kahan_add(az2, kz2, sz2);
+                    }
                     if (cand) ncand += 64ull * cnt;
                     const uint32_t rem = jc - 64;
                     float4 tmp = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -691,21 +811,27 @@ __global__ __launch_bounds__(kThreads, SYM ? 5 : 1) void k_pp32(
             }
         }
         if (jc > 0) {
-            float sx = 0.f, sy = 0.f, sz = 0.f;
+            float sx = 0.f, sy = 0.f, sz = 0.f, sx2 = 0.f, sy2 = 0.f, sz2 = 0.f;
             if (SYM) {
-                eval_sym((int)jc, sx, sy, sz);
+                eval_sym((int)jc, sx, sy, sz, sx2, sy2, sz2);
             } else {
                 eval_default((int)jc, sx, sy, sz);
             }
             kahan_add(ax, kx, sx);
             kahan_add(ay, ky, sy);
             kahan_add(az, kz, sz);
+            if (TPL == 2) {
+                kahan_add(ax2, kx2, sx2);
+                kahan_add(ay2, ky2, sy2);
+                kahan_add(az2, kz2, sz2);
+            }
             if (cand) ncand += (unsigned long long)jc * cnt;
             jc = 0;
         }
         wave_sync();
         if (SYM) {
             if (valid) iacc[i] = make_float4(ax, ay, az, 0.f);
+            if (TPL == 2 && valid2) iacc[i2] = make_float4(ax2, ay2, az2, 0.f);
         } else if (act) {
             acc[perm[i]] = make_float4(ax * pp.out_scale, ay * pp.out_scale,
                                        az * pp.out_scale, 0.f);
@@ -716,13 +842,13 @@ __global__ __launch_bounds__(kThreads, SYM ? 5 : 1) void k_pp32(
 
 // Tiles of one super-block: the 8 Morton blocks (2^shift cells a side) of
 // one block a level up.  Consecutive blocks whose particles add up to at
-// most 64 share a tile (in sparse regions a block alone would leave most of
-// the wave idle); a block with more than 64 particles is cut into runs of
-// 64.  A tile is a contiguous range of sorted particles inside one
-// super-block.  Returns the number of tiles; writes them from t0 on when
-// tstart is given.
+// most run (64, or 128 for two targets per lane) share a tile (in sparse
+// regions a block alone would leave most of the wave idle); a block with
+// more than run particles is cut into runs of that length.  A tile is a
+// contiguous range of sorted particles inside one super-block.  Returns the
+// number of tiles; writes them from t0 on when tstart is given.
 __device__ inline uint32_t super_tiles(const uint32_t *__restrict__ off, uint32_t sb, int shift3,
-                                       uint32_t *__restrict__ tstart,
+                                       uint32_t run, uint32_t *__restrict__ tstart,
                                        uint32_t *__restrict__ tcount, uint32_t t0)
 {
     const size_t base = (size_t)sb << (shift3 + 3);
@@ -738,11 +864,11 @@ __device__ inline uint32_t super_tiles(const uint32_t *__restrict__ off, uint32_
         const uint32_t bs = off[base + ((size_t)k << shift3)];
         const uint32_t be = off[base + ((size_t)(k + 1) << shift3)];
         const uint32_t c = be - bs;
-        if (c > 64u) {
+        if (c > run) {
             if (gn) emit(gs, gn);
             gn = 0;
-            for (uint32_t p = bs; p < be; p += 64u) emit(p, min(64u, be - p));
-        } else if (gn + c > 64u) {
+            for (uint32_t p = bs; p < be; p += run) emit(p, min(run, be - p));
+        } else if (gn + c > run) {
             emit(gs, gn);
             gs = bs;
             gn = c;
@@ -756,19 +882,19 @@ __device__ inline uint32_t super_tiles(const uint32_t *__restrict__ off, uint32_
 }
 
 __global__ void k_block_tiles(const uint32_t *__restrict__ off, uint32_t nsuper, int shift3,
-                              uint32_t *__restrict__ super_count)
+                              uint32_t run, uint32_t *__restrict__ super_count)
 {
     const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b < nsuper) super_count[b] = super_tiles(off, b, shift3, nullptr, nullptr, 0);
+    if (b < nsuper) super_count[b] = super_tiles(off, b, shift3, run, nullptr, nullptr, 0);
 }
 
 __global__ void k_fill_tiles(const uint32_t *__restrict__ off,
                              const uint32_t *__restrict__ tile_off, uint32_t nsuper,
-                             int shift3, uint32_t *__restrict__ tstart,
+                             int shift3, uint32_t run, uint32_t *__restrict__ tstart,
                              uint32_t *__restrict__ tcount)
 {
     const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b < nsuper) (void)super_tiles(off, b, shift3, tstart, tcount, tile_off[b]);
+    if (b < nsuper) (void)super_tiles(off, b, shift3, run, tstart, tcount, tile_off[b]);
 }
 
 // First tile of each box: boxes start on super-block boundaries (a box's
@@ -911,13 +1037,18 @@ int launch_build_tiles(const uint32_t *off, const BoxSet &bs, uint32_t nkeys,
         set_error("tile build: %u Morton super-blocks exceed capacity %u", nsuper, t.nblocks);
         return CUCO_ERR_INVALID_ARG;
     }
-    k_block_tiles<<<grid_for(nsuper, 256), 256, 0, s>>>(off, nsuper, shift3, t.block_tiles);
+    if (t.run != 64u && t.run != 128u) {
+        set_error("tile build: run length %u (64 or 128)", t.run);
+        return CUCO_ERR_INVALID_ARG;
+    }
+    k_block_tiles<<<grid_for(nsuper, 256), 256, 0, s>>>(off, nsuper, shift3, t.run,
+                                                        t.block_tiles);
     CUCO_LAUNCH_CHECK();
     int rc = exclusive_scan_u32(t.block_tiles, t.block_tiles, nsuper, t.scan_scratch,
                                 t.counters, s);
     if (rc) return rc;
     k_fill_tiles<<<grid_for(nsuper, 256), 256, 0, s>>>(off, t.block_tiles, nsuper, shift3,
-                                                       t.tile_start, t.tile_count);
+                                                       t.run, t.tile_start, t.tile_count);
     CUCO_LAUNCH_CHECK();
     k_box_tiles<<<grid_for(bs.nboxes + 1, 64), 64, 0, s>>>(
         t.block_tiles, bs.dev[kKeyMorton], bs.nboxes, shift3, t.counters, bs.tile_map);
@@ -987,15 +1118,23 @@ int launch_pp32(const float4 *spos, const uint32_t *skeys,
         rc = launch_fill_u32(reinterpret_cast<uint32_t *>(sym.jacc), 0u,
                              (size_t)6 * sym.n_sorted, s);
         if (rc) return rc;
-#define CUCO_SYM32(GENERIC, UNIFORM)                                                    \
-    k_pp32<GENERIC, UNIFORM, true><<<nwg, kThreads, 0, s>>>(                            \
+#define CUCO_SYM32(GENERIC, UNIFORM, TPL)                                               \
+    k_pp32<GENERIC, UNIFORM, true, TPL><<<nwg, kThreads, 0, s>>>(                       \
         spos, skeys, perm, off, tv, p32s, mass, table32, t.tile_start, t.tile_count,    \
         t.counters, acc, cand_counter, nullptr, 0, sym.iacc, sym.jacc, sym.n_sorted,   \
         nullptr, t.coarse_cells)
-        if (generic) {
-            if (uniform) CUCO_SYM32(true, true); else CUCO_SYM32(true, false);
+        if (t.run == 128u) {
+            if (generic) {
+                if (uniform) CUCO_SYM32(true, true, 2); else CUCO_SYM32(true, false, 2);
+            } else {
+                if (uniform) CUCO_SYM32(false, true, 2); else CUCO_SYM32(false, false, 2);
+            }
         } else {
-            if (uniform) CUCO_SYM32(false, true); else CUCO_SYM32(false, false);
+            if (generic) {
+                if (uniform) CUCO_SYM32(true, true, 1); else CUCO_SYM32(true, false, 1);
+            } else {
+                if (uniform) CUCO_SYM32(false, true, 1); else CUCO_SYM32(false, false, 1);
+            }
         }
 #undef CUCO_SYM32
         CUCO_LAUNCH_CHECK();
@@ -1007,6 +1146,11 @@ int launch_pp32(const float4 *spos, const uint32_t *skeys,
     // Active set with its own tiles: every listed target is active.
     const bool packed = active_end && at;
     const TileInfo &tt = packed ? at->t : t;
+    // The kernel that visits every ordered pair holds one target per lane.
+    if (tt.run != 64u) {
+        set_error("pair force: tiles of %u targets need the each-pair-once kernel", tt.run);
+        return CUCO_ERR_INVALID_ARG;
+    }
     const BoxView tva{bs.dev[kKeyMorton], packed ? at->tile_map : bs.tile_map, bs.nboxes};
     const int32_t *ae = packed ? nullptr : active_end;
     const uint32_t *tl = packed ? at->list : nullptr;
@@ -1017,7 +1161,7 @@ int launch_pp32(const float4 *spos, const uint32_t *skeys,
         if (rc) return rc;
     }
 #define CUCO_PP32(GENERIC, UNIFORM)                                                     \
-    k_pp32<GENERIC, UNIFORM, false><<<nwg, kThreads, 0, s>>>(                           \
+    k_pp32<GENERIC, UNIFORM, false, 1><<<nwg, kThreads, 0, s>>>(                        \
         spos, skeys, perm, off, tva, p32s, mass, table32, tt.tile_start, tt.tile_count, \
         tt.counters, acc, cand_counter, ae, ti_current, nullptr, nullptr, 0u, tl, 0.f)
     if (generic) {

@@ -200,11 +200,16 @@ __global__ __launch_bounds__(kThreads) void k_pp64(
                     load_range(off, col + wrap_index(z, M), js, je);
                     js = __builtin_amdgcn_readfirstlane(js);
                     je = __builtin_amdgcn_readfirstlane(je);
+                    if (js >= je) continue;
+                    // The next j's position is loaded while this pair is
+                    // evaluated (its load latency is otherwise exposed).
+                    double4 q = spos[js];
                     for (uint32_t j = js; j < je; ++j) {
-                        const double4 q = spos[j];
+                        const double4 qn = spos[min(j + 1, je - 1)];
                         if (in_z && j != i)
                             pair64<WRAP>(pp, tab, p.x, p.y, p.z, q, sx, sy, sz, ax, ay, az,
                                          in_count);
+                        q = qn;
                     }
                 }
             }
@@ -241,10 +246,14 @@ __global__ __launch_bounds__(kThreads) void k_pp64(
                 for (int q = 0; q < nr; ++q) {
                     const uint32_t js = off[col + ra[q]];
                     const uint32_t je = off[col + rb[q] + 1];
+                    if (js >= je) continue;
+                    double4 pj = spos[js];  // loaded one pair ahead, as above
                     for (uint32_t j = js; j < je; ++j) {
-                        if (j == i) continue;
-                        pair64<WRAP>(pp, tab, p.x, p.y, p.z, spos[j], sx, sy, rs[q], ax, ay,
-                                     az, in_count);
+                        const double4 pn = spos[min(j + 1, je - 1)];
+                        if (j != i)
+                            pair64<WRAP>(pp, tab, p.x, p.y, p.z, pj, sx, sy, rs[q], ax, ay,
+                                         az, in_count);
+                        pj = pn;
                     }
                 }
             }

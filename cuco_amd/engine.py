@@ -57,7 +57,8 @@ EXPORTED = [
 class Tuning(C.Structure):
     _fields_ = [("pp_block_shift", C.c_int32), ("pp_workgroups_per_cu", C.c_int32),
                 ("pm_overlap", C.c_int32), ("pp_symmetric", C.c_int32),
-                ("pp_cell_level", C.c_int32), ("fft_split", C.c_int32)]
+                ("pp_cell_level", C.c_int32), ("fft_split", C.c_int32),
+                ("pp_wide_tiles", C.c_int32)]
 
     def as_dict(self) -> dict:
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -595,9 +596,10 @@ class Engine:
 
     def set_tuning(self, pp_block_shift: int, pp_workgroups_per_cu: int,
                    pm_overlap: int = 0, pp_symmetric: int = 1, pp_cell_level: int = 0,
-                   fft_split: int = 0) -> None:
+                   fft_split: int = 0, pp_wide_tiles: int = 0) -> None:
         t = Tuning(pp_block_shift, pp_workgroups_per_cu, int(pm_overlap),
-                   int(pp_symmetric), int(pp_cell_level), int(fft_split))
+                   int(pp_symmetric), int(pp_cell_level), int(fft_split),
+                   int(pp_wide_tiles))
         _check(lib().cuco_set_tuning(self.h, C.byref(t)), "cuco_set_tuning")
 
     def autotune(self):

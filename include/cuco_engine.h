@@ -189,13 +189,18 @@ typedef struct cuco_tuning {
                                      threads per line of length <= 32/128/
                                      512/1024; 1 half as many, each holding
                                      twice the values */
+    int32_t pp_wide_tiles;        /* each-pair-once FP32 kernel: 0 (the
+                                     default) tiles of up to 64 targets, one
+                                     per lane; 1 tiles of up to 128, two per
+                                     lane (each staged particle is read once
+                                     for both) */
 } cuco_tuning;
 
 int cuco_get_tuning(cuco_engine *e, cuco_tuning *t);
 int cuco_set_tuning(cuco_engine *e, const cuco_tuning *t);
 /* Times, on the handle's current particles: the FP32 pair force for every
  * Morton block size, workgroup count and pair variant, then its candidate
- * enumeration level; the FP32 PM chain with each FFT line split; then whole
+ * enumeration level and tile width; the FP32 PM chain with each FFT line split; then whole
  * FP32 evaluations with and without PM/PP overlap.  Keeps the fastest and
  * returns it; *best_ms (optional) receives the best pair-kernel time.
  * Every setting gives the same forces up to FP32 rounding.  Leaves the

@@ -175,6 +175,7 @@ def test_every_tuning_gives_the_same_forces():
     best, ms = eng.autotune()
     assert best["pp_block_shift"] in (2, 3, 4) and ms > 0
     assert best["pp_cell_level"] in (0, 1, 2) and best["fft_split"] in (0, 1)
+    assert best["pp_wide_tiles"] in (0, 1)
     assert eng.tuning() == best
     eng.compute_accelerations(E.FP32, True)
     assert np.abs(eng.accelerations()[0] - ref).max() <= 1e-5 * scale
@@ -184,15 +185,18 @@ def test_every_tuning_gives_the_same_forces():
         eng.set_tuning(3, 8, 0, 1, pp_cell_level=3)
     with pytest.raises(E.CucoError):
         eng.set_tuning(3, 8, 0, 1, fft_split=2)
+    with pytest.raises(E.CucoError):
+        eng.set_tuning(3, 8, 0, 1, pp_wide_tiles=2)
 
 
 @pytest.mark.parametrize("M", [64, 128, 96])
 def test_enumeration_level_and_fft_split_give_the_same_forces(M):
     """The autotuner's candidate-enumeration level of the each-pair-once
     kernel (mesh cells, 2x2x2-cell blocks for wide tiles, blocks for every
-    tile) and the register-FFT line split (96: not a power of two, the
-    Stockham kernels run whatever the split) change only FP32 rounding: PP
-    within 1e-5 and PM within 1e-5 of the force scale."""
+    tile), its tile width (64 or 128 targets) and the register-FFT line
+    split (96: not a power of two, the Stockham kernels run whatever the
+    split) change only FP32 rounding: PP within 1e-5 and PM within 1e-5 of
+    the force scale."""
     box = halo_box(32, M, 100.0, n_halos=20, seed=M)
     pos, vel, mass = ics.generate(box)
     eng = make_engine(box)
@@ -203,15 +207,62 @@ def test_enumeration_level_and_fft_split_give_the_same_forces(M):
     scale = np.linalg.norm(ref_pp + ref_pm, axis=1).max()
     for level in (0, 1, 2):
         for split in (0, 1):
-            eng.set_tuning(3, 8, 0, 1, pp_cell_level=level, fft_split=split)
-            assert eng.tuning()["pp_cell_level"] == level
-            assert eng.tuning()["fft_split"] == split
-            eng.compute_accelerations(E.FP32, True)
-            a, p = eng.accelerations()
-            assert np.abs(a - ref_pp).max() <= 1e-5 * scale, (level, split)
-            assert np.abs(p - ref_pm).max() <= 1e-5 * scale, (level, split)
-            if split == 0:
-                assert np.array_equal(p, ref_pm)
+            for wide in (0, 1):
+                eng.set_tuning(3, 8, 0, 1, pp_cell_level=level, fft_split=split,
+                               pp_wide_tiles=wide)
+                t = eng.tuning()
+                assert (t["pp_cell_level"], t["fft_split"], t["pp_wide_tiles"]) == \
+                    (level, split, wide)
+                eng.compute_accelerations(E.FP32, True)
+                a, p = eng.accelerations()
+                assert np.abs(a - ref_pp).max() <= 1e-5 * scale, (level, split, wide)
+                assert np.abs(p - ref_pm).max() <= 1e-5 * scale, (level, split, wide)
+                if split == 0:
+                    assert np.array_equal(p, ref_pm)
+    eng.close()
+
+
+@pytest.mark.parametrize("case", ["equal", "unequal", "off_edge"])
+def test_wide_tiles(case):
+    """Each pair once with tiles of up to 128 targets, two per lane: the
+    same error bar against the FP64 path as tiles of 64, identical results
+    on every run, the same pairs counted, and the pair force within FP32
+    rounding of the 64-target tiles.  Cases: uniform masses; per-particle
+    masses; rcut = 4 asmth, off the table's bin edge (explicit cut and bin
+    clamp).  Switching to the kernel that visits every ordered pair brings
+    back tiles of 64."""
+    box = halo_box(seed=17)
+    pos, vel, mass = ics.generate(box)
+    if case == "unequal":
+        mass = mass * np.random.default_rng(17).uniform(0.5, 1.5, box.N)
+    kw = box.engine_kwargs()
+    if case == "off_edge":
+        kw["rcut"] = 4.0 * box.asmth
+    eng = E.Engine(max_particles=box.N, **kw)
+    eng.upload(pos, None, mass)
+    eng.compute_accelerations(E.FP64, True)
+    ref_pp, ref_pm = eng.accelerations()
+    tot = ref_pp + ref_pm
+    scale = np.linalg.norm(tot, axis=1).max()
+    eng.set_tuning(3, 8, 0, 1)
+    eng.compute_accelerations(E.FP32, True)
+    narrow = eng.accelerations()[0]
+    pairs64, cand64 = eng.count_pairs()
+    eng.set_tuning(3, 8, 0, 1, pp_wide_tiles=1)
+    runs = []
+    for _ in range(2):
+        eng.compute_accelerations(E.FP32, True)
+        runs.append(eng.accelerations()[0])
+    assert np.array_equal(runs[0], runs[1])
+    err = rel_err(runs[0], ref_pp, tot)
+    print(case, np.median(err), np.percentile(err, 99), err.max())
+    assert np.percentile(err, 99) <= 2e-5 and np.median(err) <= 1e-6
+    assert np.abs(runs[0] - narrow).max() <= 1e-5 * scale
+    pairs128, cand128 = eng.count_pairs()
+    assert pairs128 == pairs64 and 0 < cand128 and cand64 > 0
+    eng.set_tuning(3, 8, 0, 0, pp_wide_tiles=1)
+    eng.compute_accelerations(E.FP32, True)
+    assert np.percentile(rel_err(eng.accelerations()[0], ref_pp, tot), 99) <= 2e-5
     eng.close()
 
 
