@@ -122,8 +122,12 @@ __device__ inline double image_shift(int c, int M, double L)
     return c < 0 ? -L : (c >= M ? L : 0.0);
 }
 
+// Eight waves per SIMD (at most 64 VGPRs, a few spilled): each lane's pair
+// loop is a long dependent FP64 chain (correctly rounded sqrt and divide),
+// so more resident waves hide more of it (441 ms at the register
+// allocation's natural 5 waves, 400 ms at 8, halo256).
 template <bool COUNT, bool WRAP>
-__global__ __launch_bounds__(kThreads) void k_pp64(
+__global__ __launch_bounds__(kThreads, 8) void k_pp64(
     const double4 *__restrict__ spos, const uint32_t *__restrict__ perm,
     const uint32_t *__restrict__ off, uint32_t i0, uint32_t n, PairParams pp,
     const float *__restrict__ table, double4 *__restrict__ acc,
@@ -200,16 +204,11 @@ __global__ __launch_bounds__(kThreads) void k_pp64(
                     load_range(off, col + wrap_index(z, M), js, je);
                     js = __builtin_amdgcn_readfirstlane(js);
                     je = __builtin_amdgcn_readfirstlane(je);
-                    if (js >= je) continue;
-                    // The next j's position is loaded while this pair is
-                    // evaluated (its load latency is otherwise exposed).
-                    double4 q = spos[js];
                     for (uint32_t j = js; j < je; ++j) {
-                        const double4 qn = spos[min(j + 1, je - 1)];
+                        const double4 q = spos[j];
                         if (in_z && j != i)
                             pair64<WRAP>(pp, tab, p.x, p.y, p.z, q, sx, sy, sz, ax, ay, az,
                                          in_count);
-                        q = qn;
                     }
                 }
             }
@@ -246,14 +245,10 @@ __global__ __launch_bounds__(kThreads) void k_pp64(
                 for (int q = 0; q < nr; ++q) {
                     const uint32_t js = off[col + ra[q]];
                     const uint32_t je = off[col + rb[q] + 1];
-                    if (js >= je) continue;
-                    double4 pj = spos[js];  // loaded one pair ahead, as above
                     for (uint32_t j = js; j < je; ++j) {
-                        const double4 pn = spos[min(j + 1, je - 1)];
-                        if (j != i)
-                            pair64<WRAP>(pp, tab, p.x, p.y, p.z, pj, sx, sy, rs[q], ax, ay,
-                                         az, in_count);
-                        pj = pn;
+                        if (j == i) continue;
+                        pair64<WRAP>(pp, tab, p.x, p.y, p.z, spos[j], sx, sy, rs[q], ax, ay,
+                                     az, in_count);
                     }
                 }
             }
