@@ -587,6 +587,10 @@ def main() -> int:
         "traffic": pmc_traffic("pp32", box.name,
                                "traffic_bytes_symmetric" if tuning.get("pp_symmetric")
                                else "traffic_bytes"),
+        "traffic_note": (f"HBM bytes per launch from the committed rocprofv3 "
+                         f"FETCH_SIZE / WRITE_SIZE passes (profiles/pp32_{box.name}_pmc.md: "
+                         f"8^3-cell blocks, 8 workgroups per CU, tiles of 64 targets), "
+                         f"not from this run"),
         "pp_ms": pp_ms,
         "pairs_in": pairs_in,
         "candidates": candidates,
