@@ -117,11 +117,13 @@ struct TileInfo {
 };
 
 // Constants of the FP32 pair kernel for one box.  All lengths are scaled by
-// s = asmthfac (table bins per length unit), so a pair's table bin is
-// simply (int)r.  A pair term in scaled units is dx_s * m * rinv_s^3 * T;
-// the physical acceleration is G * s^2 times the scaled sum.
+// s = 4 asmthfac (quarter table bins per length unit), so a pair's table
+// bin is floor(r / 4), whose byte offset in the LDS table a far pair gets
+// from one multiply-add and a mask.  A pair term in scaled units is
+// dx_s * m * rinv_s^3 * T; the physical acceleration is G * s^2 times the
+// scaled sum.
 struct P32 {
-    float s;                      // asmthfac
+    float s;                      // 4 asmthfac
     float rcut, rcut2, filter2;   // scaled
     float h, h_inv, h3_inv;       // scaled spline softening
     float cellf, inv_cellf;       // scaled cell size
@@ -130,6 +132,7 @@ struct P32 {
     float mass;                   // uniform particle mass of the box
     int M;
     float jscale, jinv;           // fixed-point scale of the each-pair-once j sums
+    float hnear;                  // near-chunk test distance: max(h, 1)
 };
 // mass_max: the box's largest particle mass (bounds the fixed-point j sums).
 P32 make_p32(const PairParams &pp, const GridInfo &g, double uniform_mass,

@@ -111,6 +111,24 @@ __device__ inline float spline_fac(float r, float fac, float mj, const P32 &pp)
     return r < pp.h ? (u < 0.5f ? lo : hi) : fac;
 }
 
+// Byte offset of a pair's table entry, bin floor(r / 4) (lengths are in
+// quarter bins).  Far pairs (r >= 1): 2^23 - 0.5 + r, rounded to an
+// integer by the multiply-add, is 2^23 + floor(r) (an exact integer r can
+// round down by one: the lower bin of an edge), and its mantissa with the
+// two low bits cleared is 4 floor(r / 4).  Near pairs, where r may be
+// below 1, convert r itself.
+__device__ inline uint32_t far_bin_bytes(float r2, float rinv)
+{
+    return __builtin_bit_cast(uint32_t, fmaf(r2, rinv, 8388607.5f)) & 0x7ffffcu;
+}
+
+__device__ inline uint32_t near_bin_bytes(float r) { return (uint32_t)(int)r & ~3u; }
+
+__device__ inline float tab_at(const float *__restrict__ tab, uint32_t bytes)
+{
+    return *reinterpret_cast<const float *>(reinterpret_cast<const char *>(tab) + bytes);
+}
+
 // Pair sum of one lane against cnt staged j particles.
 //
 // GENERIC=false is the reference configuration: rcut sits on a table bin
@@ -145,12 +163,17 @@ __device__ inline void interact(const float4 *__restrict__ sj, int cnt,
             // finite and the term is exactly zero (gravlist_kernel.cu:64-69).
             r2[q] = fmaf(dx[q], dx[q], fmaf(dy[q], dy[q], fmaf(dz[q], dz[q], pp.eps)));
             const float rinv = __builtin_amdgcn_rsqf(r2[q]);
-            r[q] = r2[q] * rinv;
             const float rinv3 = rinv * rinv * rinv;
             fac[q] = UNIFORM ? rinv3 : mj[q] * rinv3;
-            int idx = (int)r[q];
-            if (GENERIC || CLAMP) idx = min(idx, kTab - 1);
-            t[q] = tab[idx];
+            uint32_t b;
+            if (NEAR) {
+                r[q] = r2[q] * rinv;
+                b = near_bin_bytes(r[q]);
+            } else {
+                b = far_bin_bytes(r2[q], rinv);
+            }
+            if (GENERIC || CLAMP) b = min(b, 4u * (kTab - 1));
+            t[q] = tab_at(tab, b);
         }
         if (NEAR) {
             const bool close = (r[0] < pp.h) | (r[1] < pp.h) | (r[2] < pp.h) | (r[3] < pp.h);
@@ -178,9 +201,9 @@ __device__ inline void interact(const float4 *__restrict__ sj, int cnt,
         const float rinv3 = rinv * rinv * rinv;
         float fac = UNIFORM ? rinv3 : p.w * rinv3;
         if (NEAR) fac = spline_fac<UNIFORM>(r, fac, p.w, pp);
-        int idx = (int)r;
-        if (GENERIC || CLAMP) idx = min(idx, kTab - 1);
-        float tv = tab[idx];
+        uint32_t b = NEAR ? near_bin_bytes(r) : far_bin_bytes(r2, rinv);
+        if (GENERIC || CLAMP) b = min(b, 4u * (kTab - 1));
+        float tv = tab_at(tab, b);
         if (GENERIC) tv = r2 <= pp.rcut2 ? tv : 0.0f;
         const float f = fac * tv;
         sx = fmaf(dx, f, sx);
@@ -251,13 +274,20 @@ __device__ inline void interact_sym(const float4 *__restrict__ rot,
             dz[q] = p.z - zi;
             const float r2 = fmaf(dx[q], dx[q], fmaf(dy[q], dy[q], fmaf(dz[q], dz[q], pp.eps)));
             const float rinv = __builtin_amdgcn_rsqf(r2);
-            r[q] = r2 * rinv;
             fac[q] = rinv * rinv * rinv;
             if (GEN) inside[q] = r2 <= pp.rcut2;
             // The table's zero tail from the cut bin on.  CLAMP: bins past
             // the LDS copy read its last (zero) entry; without it the tile's
             // extent keeps every bin inside (the per-tile clamp test).
-            t[q] = tab[(CLAMP || GEN) ? min((int)r[q], kTabSym - 1) : (int)r[q]];
+            uint32_t b;
+            if (NEAR) {
+                r[q] = r2 * rinv;
+                b = near_bin_bytes(r[q]);
+            } else {
+                b = far_bin_bytes(r2, rinv);
+            }
+            if (CLAMP || GEN) b = min(b, 4u * (kTabSym - 1));
+            t[q] = tab_at(tab, b);
         }
         if (NEAR) {
             const bool close = (r[0] < pp.h) | (r[1] < pp.h) | (r[2] < pp.h) | (r[3] < pp.h);
@@ -322,16 +352,28 @@ __device__ inline void interact_sym2(const float4 *__restrict__ rot,
             const float r2b = fmaf(dxb[q], dxb[q], fmaf(dyb[q], dyb[q], fmaf(dzb[q], dzb[q], pp.eps)));
             const float rinva = __builtin_amdgcn_rsqf(r2a);
             const float rinvb = __builtin_amdgcn_rsqf(r2b);
-            ra[q] = r2a * rinva;
-            rb[q] = r2b * rinvb;
             fa[q] = rinva * rinva * rinva;
             fb[q] = rinvb * rinvb * rinvb;
             if (GEN) {
                 ina[q] = r2a <= pp.rcut2;
                 inb[q] = r2b <= pp.rcut2;
             }
-            ta[q] = tab[(CLAMP || GEN) ? min((int)ra[q], kTabSym - 1) : (int)ra[q]];
-            tb[q] = tab[(CLAMP || GEN) ? min((int)rb[q], kTabSym - 1) : (int)rb[q]];
+            uint32_t ba, bb;
+            if (NEAR) {
+                ra[q] = r2a * rinva;
+                rb[q] = r2b * rinvb;
+                ba = near_bin_bytes(ra[q]);
+                bb = near_bin_bytes(rb[q]);
+            } else {
+                ba = far_bin_bytes(r2a, rinva);
+                bb = far_bin_bytes(r2b, rinvb);
+            }
+            if (CLAMP || GEN) {
+                ba = min(ba, 4u * (kTabSym - 1));
+                bb = min(bb, 4u * (kTabSym - 1));
+            }
+            ta[q] = tab_at(tab, ba);
+            tb[q] = tab_at(tab, bb);
         }
         if (NEAR) {
             const bool close = (ra[0] < pp.h) | (ra[1] < pp.h) | (rb[0] < pp.h) | (rb[1] < pp.h);
@@ -371,7 +413,8 @@ __device__ inline void interact_sym2(const float4 *__restrict__ rot,
 
 // True when some staged j (lanes with in set) lies within h of the tile's
 // bounding box; otherwise every pair of the chunk has r >= h and takes the
-// plain law.  Wave-uniform.
+// plain law.  Called with h = max(softening, 1 quarter bin), so a far
+// chunk's pairs also have r >= 1, which far_bin_bytes needs.  Wave-uniform.
 __device__ inline bool chunk_near(float4 q, bool in, float bminx, float bminy, float bminz,
                                   float bmaxx, float bmaxy, float bmaxz, float h)
 {
@@ -524,7 +567,7 @@ __global__ __launch_bounds__(kThreads, SYM ? (TPL == 2 ? 4 : 5) : 1) void k_pp32
         // no clamp.
         const float ex = bmaxx - bminx, ey = bmaxy - bminy, ez = bmaxz - bminz;
         const bool clamp =
-            !(sqrtf(ex * ex + ey * ey + ez * ez) + sqrtf(pp.filter2) < (float)(kT - 2));
+            !(sqrtf(ex * ex + ey * ey + ez * ez) + sqrtf(pp.filter2) + 1.0f < 4.0f * (kT - 2));
         // SYM: the j side of one staged chunk, added to jacc (fixed point).
         const uint32_t iid = valid ? i : 0xffffffffu;
         const uint32_t iid2 = valid2 ? i2 : 0xffffffffu;
@@ -532,9 +575,9 @@ __global__ __launch_bounds__(kThreads, SYM ? (TPL == 2 ? 4 : 5) : 1) void k_pp32
                             float &sz2_) {
             float jx = 0.f, jy = 0.f, jz = 0.f;
             // Copy the chunk twice into the rotation buffer; unused slots of a
-            // partial chunk get index 0 (always masked) and a position inside
-            // the tile (a bounded table bin).
-            float4 mine = make_float4(bminx, bminy, bminz, 0.f);
+            // partial chunk get index 0 (always masked) and a position one
+            // quarter bin below the tile's box (a bounded far-pair bin).
+            float4 mine = make_float4(bminx - 1.0f, bminy, bminz, 0.f);
             uint32_t mid = 0u;
             if (lane < nj) {
                 mine = sj[lane];
@@ -549,7 +592,7 @@ __global__ __launch_bounds__(kThreads, SYM ? (TPL == 2 ? 4 : 5) : 1) void k_pp32
             // mask by index.
             const bool inside = __ballot(lane < nj && mid < i0 + cnt) != 0ull;
             const bool near = chunk_near(mine, lane < nj, bminx, bminy, bminz, bmaxx, bmaxy,
-                                         bmaxz, pp.h);
+                                         bmaxz, pp.hnear);
             const bool masked = nj < 64 || cnt < 64u * TPL || inside;
 #define CUCO_SYM(MK, NR, CL)                                                                    \
     if (TPL == 2)                                                                               \
@@ -595,7 +638,7 @@ __global__ __launch_bounds__(kThreads, SYM ? (TPL == 2 ? 4 : 5) : 1) void k_pp32
         auto eval_default = [&](int nj, float &sx_, float &sy_, float &sz_) {
             const float4 mine = lane < nj ? sj[lane] : make_float4(0.f, 0.f, 0.f, 0.f);
             const bool near =
-                chunk_near(mine, lane < nj, bminx, bminy, bminz, bmaxx, bmaxy, bmaxz, pp.h);
+                chunk_near(mine, lane < nj, bminx, bminy, bminz, bmaxx, bmaxy, bmaxz, pp.hnear);
 #define CUCO_DEF(NR, CL) \
     interact<GENERIC, UNIFORM, NR, CL>(sj, nj, pp, s_tab, xi, yi, zi, sx_, sy_, sz_)
             if (clamp) {
@@ -1059,7 +1102,7 @@ int launch_build_tiles(const uint32_t *off, const BoxSet &bs, uint32_t nkeys,
 P32 make_p32(const PairParams &pp, const GridInfo &g, double uniform_mass,
              double mass_max, uint32_t n_box)
 {
-    const double sc = pp.asmthfac;
+    const double sc = 4.0 * pp.asmthfac;  // quarter bins
     P32 p;
     p.s = (float)sc;
     p.rcut = (float)(pp.rcut * sc);
@@ -1068,6 +1111,7 @@ P32 make_p32(const PairParams &pp, const GridInfo &g, double uniform_mass,
     p.h = (float)(pp.h * sc);
     p.h_inv = (float)(1.0 / (pp.h * sc));
     p.h3_inv = p.h_inv * p.h_inv * p.h_inv;
+    p.hnear = std::max(p.h, 1.0f);
     p.cellf = (float)(g.cell * sc);
     p.inv_cellf = (float)(1.0 / (g.cell * sc));
     p.eps = 1e-12f;
