@@ -77,6 +77,18 @@ __device__ inline uint32_t wave_incl_scan(uint32_t v)
     return v;
 }
 
+// Scaled coordinate of a particle relative to its tile's anchor cell: the
+// FP32 cell-relative coordinate p times s, plus dq cells.  Targets and
+// staged candidates both come through here, as one explicit multiply-add
+// (this file is compiled with contraction on, and the two sides written as
+// separate expressions were once contracted differently): a target and its
+// own copy among the candidates then have bit-identical coordinates, so
+// the self pair's separation is exactly zero and its term vanishes.
+__device__ inline float rel_coord(float p, int dq, float s, float cellf)
+{
+    return fmaf(p, s, (float)dq * cellf);
+}
+
 __device__ inline uint32_t pack_q(int qx, int qy, int qz)
 {
     return ((uint32_t)(qx + 512) << 20) | ((uint32_t)(qy + 512) << 10) |
@@ -504,9 +516,9 @@ __global__ __launch_bounds__(kThreads, SYM ? (TPL == 2 ? 4 : 5) : 1) void k_pp32
         key_cell(g, skeys[ia] - key_base, c0x, c0y, c0z);
         key_cell(g, skeys[i] - key_base, cx, cy, cz);
         const float4 pi = spos[i];
-        const float xi = pi.x * pp.s + (float)unwrap_delta(cx - c0x, M) * pp.cellf;
-        const float yi = pi.y * pp.s + (float)unwrap_delta(cy - c0y, M) * pp.cellf;
-        const float zi = pi.z * pp.s + (float)unwrap_delta(cz - c0z, M) * pp.cellf;
+        const float xi = rel_coord(pi.x, unwrap_delta(cx - c0x, M), pp.s, pp.cellf);
+        const float yi = rel_coord(pi.y, unwrap_delta(cy - c0y, M), pp.s, pp.cellf);
+        const float zi = rel_coord(pi.z, unwrap_delta(cz - c0z, M), pp.s, pp.cellf);
         // The lane's second target (TPL = 2): 64 + L of the tile.
         const bool valid2 = TPL == 2 && (uint32_t)lane + 64u < cnt;
         const uint32_t i2 = valid2 ? i0 + 64u + (uint32_t)lane : i;
@@ -515,9 +527,9 @@ __global__ __launch_bounds__(kThreads, SYM ? (TPL == 2 ? 4 : 5) : 1) void k_pp32
             int cx2, cy2, cz2;
             key_cell(g, skeys[i2] - key_base, cx2, cy2, cz2);
             const float4 p2 = spos[i2];
-            xi2 = p2.x * pp.s + (float)unwrap_delta(cx2 - c0x, M) * pp.cellf;
-            yi2 = p2.y * pp.s + (float)unwrap_delta(cy2 - c0y, M) * pp.cellf;
-            zi2 = p2.z * pp.s + (float)unwrap_delta(cz2 - c0z, M) * pp.cellf;
+            xi2 = rel_coord(p2.x, unwrap_delta(cx2 - c0x, M), pp.s, pp.cellf);
+            yi2 = rel_coord(p2.y, unwrap_delta(cy2 - c0y, M), pp.s, pp.cellf);
+            zi2 = rel_coord(p2.z, unwrap_delta(cz2 - c0z, M), pp.s, pp.cellf);
             mi2 = p2.w;
         }
 
@@ -545,11 +557,10 @@ __global__ __launch_bounds__(kThreads, SYM ? (TPL == 2 ? 4 : 5) : 1) void k_pp32
         // staging takes each candidate's cell from the low bits of its key.
         // Blocks need an even mesh (they wrap onto blocks) and a reach of at
         // most M cells per axis; coarse_cells is the tile extent (in cells)
-        // from which they are used.  Only the each-pair-once variant uses
-        // them (it runs the halo workloads; the default variant keeps cells).
+        // from which they are used (both variants).
         const float ext = fmaxf(bmaxx - bminx, fmaxf(bmaxy - bminy, bmaxz - bminz));
         const bool coarse =
-            SYM && uni((uint32_t)(sane && (M & 1) == 0 && nqx <= M && nqy <= M && nqz <= M &&
+            uni((uint32_t)(sane && (M & 1) == 0 && nqx <= M && nqy <= M && nqz <= M &&
                                   ext >= coarse_cells * pp.cellf)) != 0u;
         const int lv = coarse ? 1 : 0;
         // Units (absolute, unwrapped) covering cells c0 + lq .. c0 + lq + nq - 1.
@@ -804,9 +815,9 @@ __global__ __launch_bounds__(kThreads, SYM ? (TPL == 2 ? 4 : 5) : 1) void k_pp32
                         qz += (int)(kj & 1u);
                     }
                     const float4 pj = spos[j];
-                    qj = make_float4(pj.x * pp.s + qx * pp.cellf,
-                                     pj.y * pp.s + qy * pp.cellf,
-                                     pj.z * pp.s + qz * pp.cellf, pj.w);
+                    qj = make_float4(rel_coord(pj.x, qx, pp.s, pp.cellf),
+                                     rel_coord(pj.y, qy, pp.s, pp.cellf),
+                                     rel_coord(pj.z, qz, pp.s, pp.cellf), pj.w);
                     const float gx = fmaxf(0.f, fmaxf(bminx - qj.x, qj.x - bmaxx));
                     const float gy = fmaxf(0.f, fmaxf(bminy - qj.y, qj.y - bmaxy));
                     const float gz = fmaxf(0.f, fmaxf(bminz - qj.z, qj.z - bmaxz));
@@ -1207,7 +1218,7 @@ int launch_pp32(const float4 *spos, const uint32_t *skeys,
 #define CUCO_PP32(GENERIC, UNIFORM)                                                     \
     k_pp32<GENERIC, UNIFORM, false, 1><<<nwg, kThreads, 0, s>>>(                        \
         spos, skeys, perm, off, tva, p32s, mass, table32, tt.tile_start, tt.tile_count, \
-        tt.counters, acc, cand_counter, ae, ti_current, nullptr, nullptr, 0u, tl, 0.f)
+        tt.counters, acc, cand_counter, ae, ti_current, nullptr, nullptr, 0u, tl, t.coarse_cells)
     if (generic) {
         if (uniform) CUCO_PP32(true, true); else CUCO_PP32(true, false);
     } else {

@@ -178,8 +178,8 @@ typedef struct cuco_tuning {
                                      particles (full evaluations; active sets
                                      of the integer timeline visit every
                                      ordered pair); 0: every ordered pair */
-    int32_t pp_cell_level;        /* candidate enumeration of the each-pair-
-                                     once FP32 kernel: 0 (the default) mesh
+    int32_t pp_cell_level;        /* candidate enumeration of the FP32
+                                     pair kernel: 0 (the default) mesh
                                      cells for tiles narrower than 4 cells,
                                      2x2x2-cell blocks for wider ones; 1 mesh
                                      cells for every tile; 2 two-cell blocks

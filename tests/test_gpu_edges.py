@@ -91,3 +91,33 @@ def test_edge_cases(orc, case):
         assert np.abs(m32 - ref_pm).max() <= 2e-5 * unit, (sym, "pm")
         if n == 1:
             assert not a32.any()
+
+
+def test_isolated_particles_feel_no_short_range_force():
+    """512 particles on a grid 8 cells apart (rcut is 5.6 cells), each at a
+    random point of its cell: a particle's only pair inside rcut is itself,
+    whose term must vanish exactly.  The kernel that visits every ordered
+    pair stages each target among its own candidates, so this holds only if
+    a target's coordinates and its candidate copy's are bit-identical (a
+    build that contracted the two coordinate expressions differently gave
+    every self pair a one-ulp separation and a spurious softened force).
+    Every pair-kernel tuning; FP64 as well."""
+    box = ics.small_box("isolated", 8, 64, 80.0, 41)
+    rng = np.random.default_rng(41)
+    cell = box.L / box.M
+    g = np.stack(np.meshgrid(*[np.arange(8)] * 3, indexing="ij"), -1).reshape(-1, 3)
+    pos = (g * 8 + rng.uniform(0.0, 1.0, g.shape)) * cell
+    mass = np.full(len(pos), ics.generate(box)[2][0])
+    eng = make_engine(box, capacity=len(pos))
+    eng.upload(pos, None, mass)
+    assert eng.count_pairs()[0] == 0
+    eng.compute_accelerations(E.FP64, True)
+    assert not eng.accelerations()[0].any()
+    for sym in (0, 1):
+        for level in (0, 1, 2):
+            for wide in ((0, 1) if sym else (0,)):
+                eng.set_tuning(3, 8, 0, sym, pp_cell_level=level, pp_wide_tiles=wide)
+                eng.compute_accelerations(E.FP32, True)
+                a32 = eng.accelerations()[0]
+                assert not a32.any(), (sym, level, wide, np.abs(a32).max())
+    eng.close()
