@@ -229,27 +229,20 @@ __global__ __launch_bounds__(kThreads, 8) void k_pp64(
                 if (za > zb) continue;
                 const uint32_t col = ((uint32_t)iix * M + iiy) * (uint32_t)M;
                 // Up to two contiguous runs in reference order (wrapped part
-                // first when the range starts below 0).
-                int ra[2], rb[2], nr = 0;
-                double rs[2];  // image shift of each run
-                if (za < 0 && zb >= 0) {
-                    ra[0] = za + M; rb[0] = M - 1; rs[0] = -pp.L;
-                    ra[1] = 0; rb[1] = zb; rs[1] = 0.0; nr = 2;
-                } else if (za < M && zb >= M) {
-                    ra[0] = za; rb[0] = M - 1; rs[0] = 0.0;
-                    ra[1] = 0; rb[1] = zb - M; rs[1] = pp.L; nr = 2;
-                } else {
-                    ra[0] = wrap_index(za, M); rb[0] = wrap_index(zb, M);
-                    rs[0] = image_shift(za, M, pp.L); nr = 1;
-                }
-                for (int q = 0; q < nr; ++q) {
-                    const uint32_t js = off[col + ra[q]];
-                    const uint32_t je = off[col + rb[q] + 1];
+                // first when the range starts below 0): z cells za..zb, the
+                // second run starting at the wrap.
+                for (int z0 = za; z0 <= zb;) {
+                    const int zw = wrap_index(z0, M);
+                    const int z1 = min(zb, z0 + (M - 1 - zw));  // last z before a wrap
+                    const double sz = image_shift(z0, M, pp.L);
+                    const uint32_t js = off[col + zw];
+                    const uint32_t je = off[col + zw + (z1 - z0) + 1];
                     for (uint32_t j = js; j < je; ++j) {
                         if (j == i) continue;
-                        pair64<WRAP>(pp, tab, p.x, p.y, p.z, spos[j], sx, sy, rs[q], ax, ay,
+                        pair64<WRAP>(pp, tab, p.x, p.y, p.z, spos[j], sx, sy, sz, ax, ay,
                                      az, in_count);
                     }
+                    z0 = z1 + 1;
                 }
             }
         }
