@@ -17,7 +17,7 @@ from helpers import halo_box, make_engine, oracle_params
 pytestmark = pytest.mark.gpu
 
 
-def setup(orc, seed=9):
+def setup(orc, seed=9, wide=0):
     box = halo_box(12, 24, 40.0, n_halos=4, seed=seed)
     pos, vel, mass = ics.generate(box)
     tl = dict(time_begin=0.2, time_max=0.3, omega0=0.3, omega_lambda=0.7,
@@ -30,6 +30,8 @@ def setup(orc, seed=9):
                                 tl["omega_lambda"], tl["hubble"])
     ref = orc.Timeline(oracle_params(box), tl, d, k, pos, vel, mass)
     eng = make_engine(box)
+    if wide:
+        eng.set_tuning(**dict(eng.tuning(), pp_wide_tiles=1))
     eng.upload(pos, vel, mass)
     eng.timeline_init(**tl)
     return box, tl, ref, eng
@@ -95,16 +97,19 @@ def test_active_set_force_equals_full_evaluation(orc):
             assert np.abs(app[active] - fa[active]).max() <= 1e-5 * scale
 
 
-def test_mostly_active_step_is_a_whole_evaluation(orc):
+@pytest.mark.parametrize("wide", [0, 1])
+def test_mostly_active_step_is_a_whole_evaluation(orc, wide):
     """A step whose active particles carry at least 4/5 of the estimated
     pair work (the first: all particles active) evaluates the FP32
     short-range force of every particle with the each-pair-once kernel:
-    bit-identical to a full evaluation."""
-    box, tl, ref, eng = setup(orc, seed=11)
+    bit-identical to a full evaluation (tiles of 64 and of 128 targets)."""
+    box, tl, ref, eng = setup(orc, seed=11, wide=wide)
     st = eng.timeline_run(E.FP32, max_steps=1)
     assert st["active"] == box.N
     pos, _ = eng.download_particles()
     full = make_engine(box)
+    if wide:
+        full.set_tuning(**dict(full.tuning(), pp_wide_tiles=1))
     full.upload(pos, None, ics.generate(box)[2])
     full.set_softening(eng_soft(tl, st["time"]))
     full.gravity_linklist(E.FP32)
@@ -168,14 +173,17 @@ def test_force_path_follows_the_pair_work_rule(orc):
     assert False in seen
 
 
-def test_fp32_timeline_repeats_bit_for_bit(orc):
+@pytest.mark.parametrize("wide", [0, 1])
+def test_fp32_timeline_repeats_bit_for_bit(orc, wide):
     """Every FP32 force on the timeline is deterministic (fixed-point j sums,
     fixed per-target order, fixed-point mesh deposit) and the choice between
     active-set tiles and a whole evaluation depends on the particles only:
-    two handles given the same particles agree bit for bit."""
+    two handles given the same particles agree bit for bit (whole
+    evaluations with tiles of 64 or of 128 targets; the active-set tiles
+    always hold 64)."""
     runs = []
     for _ in range(2):
-        box, tl, ref, eng = setup(orc, seed=12)
+        box, tl, ref, eng = setup(orc, seed=12, wide=wide)
         st = eng.timeline_run(E.FP32, max_steps=10)
         pos, vel = eng.download_particles()
         runs.append((st, pos, vel, *eng.timesteps()))
