@@ -23,6 +23,40 @@ namespace {
 
 constexpr int kThreads = 256;
 
+// sqrt(x) and n / d for the ranges pair terms take, as the operation
+// sequences of the compiler's correctly rounded f64 sqrt and divide on
+// gfx950 without their extreme-exponent scaling and special-value selects
+// (inactive there: the same operations on the same values, so the same
+// bits).  sqrt_pair: x = 0 or 2^-767 <= x < 2^1023.  div_pair: normal n and
+// d whose quotient and reciprocal stay normal (masses over cubed
+// distances).
+__device__ inline double sqrt_pair(double x)
+{
+    double y = __builtin_amdgcn_rsq(x);
+    double g = x * y;
+    double h = y * 0.5;
+    const double e = fma(-h, g, 0.5);
+    g = fma(g, e, g);
+    h = fma(h, e, h);
+    double d = fma(-g, g, x);
+    g = fma(d, h, g);
+    d = fma(-g, g, x);
+    g = fma(d, h, g);
+    return x == 0.0 ? x : g;
+}
+
+__device__ inline double div_pair(double n, double d)
+{
+    double y = __builtin_amdgcn_rcp(d);
+    double e = fma(-d, y, 1.0);
+    y = fma(y, e, y);
+    e = fma(-d, y, 1.0);
+    y = fma(y, e, y);
+    const double q = n * y;
+    const double r = fma(-d, q, n);
+    return fma(r, y, q);
+}
+
 // Minimum image.  WRAP: the reference's per-pair tests.  Otherwise the
 // caller passes the image shift of j's cell (sx, sy, sz: -L for a cell
 // index below 0, +L above M-1, else 0): when rcut plus one cell stays below
@@ -55,10 +89,10 @@ __device__ inline void pair64(const PairParams &pp, const double *__restrict__ t
     double r2 = dx * dx + dy * dy + dz * dz;
     if (r2 > pp.rcut2) return;
     ++in_count;
-    double r = sqrt(r2);
+    double r = sqrt_pair(r2);
     double fac;
     if (r >= pp.h) {
-        fac = q.w / (r2 * r);
+        fac = div_pair(q.w, r2 * r);
     } else {
         double u = r * pp.h_inv;
         if (u < 0.5)
