@@ -29,7 +29,7 @@ constexpr int kThreads = 256;
 // (inactive there: the same operations on the same values, so the same
 // bits).  sqrt_pair: x = 0 or 2^-767 <= x < 2^1023.  div_pair: normal n and
 // d whose quotient and reciprocal stay normal (masses over cubed
-// distances).
+// distances, the spline's 0.06667 / u^3 with u >= 0.5).
 __device__ inline double sqrt_pair(double x)
 {
     double y = __builtin_amdgcn_rsq(x);
@@ -100,7 +100,7 @@ __device__ inline void pair64(const PairParams &pp, const double *__restrict__ t
         else
             fac = q.w * pp.h3_inv *
                   (21.33333 - 48.0 * u + 38.4 * u * u - 10.66667 * u * u * u -
-                   0.06667 / (u * u * u));
+                   div_pair(0.06667, u * u * u));  // u >= 0.5
     }
     int tabindex = (int)(pp.asmthfac * r);
     if (tabindex < kNTab) {
