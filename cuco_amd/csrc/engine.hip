@@ -1514,14 +1514,10 @@ int timeline_step(cuco_engine *e, int precision, int32_t ti_stop, bool &stopped)
     return CUCO_OK;
 }
 
-}  // namespace
-
-extern "C" {
-
-int cuco_timeline_init(cuco_engine *e, const cuco_timeline *t)
-try {
-    CHECK_ENGINE(e);
-    TRY(check_ready(e));
+// The parts of init that do not touch the particles' timesteps: parameter
+// checks, buffers, the drift and kick tables and the timebase.
+int timeline_setup(cuco_engine *e, const cuco_timeline *t)
+{
     if (!t || !(t->time_begin > 0) || !(t->time_max > t->time_begin) ||
         !(t->hubble > 0) || !(t->err_tol_int_accuracy > 0) ||
         !(t->max_size_timestep > 0) || !(t->min_size_timestep >= 0) ||
@@ -1568,6 +1564,18 @@ try {
     e->tl_log_begin = log(t->time_begin);
     e->tl_log_max = log(t->time_max);
     e->tl_interval = (log(t->time_max) - log(t->time_begin)) / kTimebase;  // init.c:23
+    return CUCO_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cuco_timeline_init(cuco_engine *e, const cuco_timeline *t)
+try {
+    CHECK_ENGINE(e);
+    TRY(check_ready(e));
+    TRY(timeline_setup(e, t));
     TRY(launch_fill_u32(reinterpret_cast<uint32_t *>(e->ti_beg), 0u, e->n, e->stream));
     TRY(launch_fill_u32(reinterpret_cast<uint32_t *>(e->ti_end), 0u, e->n, e->stream));
     TRY(launch_fill_u32(reinterpret_cast<uint32_t *>(e->ti_scalars), 0u, 4, e->stream));
@@ -1629,6 +1637,208 @@ int cuco_get_timesteps(cuco_engine *e, int32_t *ti_begstep, int32_t *ti_endstep)
     CUCO_HIP(hipStreamSynchronize(e->stream));
     return CUCO_OK;
 }
+
+namespace {
+
+struct CkFile {
+    FILE *f = nullptr;
+    ~CkFile()
+    {
+        if (f) fclose(f);
+    }
+};
+
+size_t accel_bytes(int32_t precision)
+{
+    return precision < 0 ? 0 : (precision == CUCO_FP64 ? sizeof(double4) : sizeof(float4));
+}
+
+void *accel_buffer(cuco_engine *e, bool pp, int32_t precision)
+{
+    if (precision == CUCO_FP64) return pp ? (void *)e->acc_pp64 : (void *)e->acc_pm64;
+    return pp ? (void *)e->acc_pp32 : (void *)e->acc_pm32;
+}
+
+}  // namespace
+
+int cuco_checkpoint_write(cuco_engine *e, const char *path)
+try {
+    CHECK_ENGINE(e);
+    TRY(check_ready(e));
+    if (!path) {
+        set_error("null path");
+        return CUCO_ERR_INVALID_ARG;
+    }
+    const size_t n = e->n;
+    cuco_checkpoint_header h{};
+    memcpy(h.magic, "CUCOCKPT", 8);
+    h.version = 1;
+    h.nboxes = (int32_t)e->box.size();
+    h.n = (int64_t)n;
+    h.timeline = e->tl_init ? 1 : 0;
+    h.pp_precision = e->pp_prec;
+    h.pm_precision = e->pm_prec;
+    if (e->tl_init) {
+        h.params = e->tl;
+        h.state = e->tls;
+    }
+    std::vector<cuco_params> prm;
+    std::vector<int64_t> counts;
+    for (const HostBox &b : e->box) {
+        prm.push_back(b.prm);
+        counts.push_back(e->ensemble ? (int64_t)b.cap : (int64_t)n);
+    }
+    std::vector<double4> hp(n), hv(n);
+    std::vector<int32_t> tb(e->tl_init ? n : 0), te(e->tl_init ? n : 0);
+    std::vector<char> app(accel_bytes(e->pp_prec) * n), apm(accel_bytes(e->pm_prec) * n);
+    CUCO_HIP(hipMemcpyAsync(hp.data(), e->pos, sizeof(double4) * n, hipMemcpyDeviceToHost,
+                            e->stream));
+    CUCO_HIP(hipMemcpyAsync(hv.data(), e->vel, sizeof(double4) * n, hipMemcpyDeviceToHost,
+                            e->stream));
+    if (e->tl_init) {
+        CUCO_HIP(hipMemcpyAsync(tb.data(), e->ti_beg, sizeof(int32_t) * n,
+                                hipMemcpyDeviceToHost, e->stream));
+        CUCO_HIP(hipMemcpyAsync(te.data(), e->ti_end, sizeof(int32_t) * n,
+                                hipMemcpyDeviceToHost, e->stream));
+    }
+    if (!app.empty())
+        CUCO_HIP(hipMemcpyAsync(app.data(), accel_buffer(e, true, e->pp_prec), app.size(),
+                                hipMemcpyDeviceToHost, e->stream));
+    if (!apm.empty())
+        CUCO_HIP(hipMemcpyAsync(apm.data(), accel_buffer(e, false, e->pm_prec), apm.size(),
+                                hipMemcpyDeviceToHost, e->stream));
+    CUCO_HIP(hipStreamSynchronize(e->stream));
+
+    CkFile fp;
+    fp.f = fopen(path, "wb");
+    if (!fp.f) {
+        set_error("cannot open checkpoint file %s for writing", path);
+        return CUCO_ERR_INVALID_ARG;
+    }
+    auto put = [&](const void *p, size_t bytes) {
+        return bytes == 0 || fwrite(p, 1, bytes, fp.f) == bytes;
+    };
+    if (!put(&h, sizeof(h)) || !put(prm.data(), sizeof(cuco_params) * prm.size()) ||
+        !put(counts.data(), sizeof(int64_t) * counts.size()) ||
+        !put(hp.data(), sizeof(double4) * n) || !put(hv.data(), sizeof(double4) * n) ||
+        !put(tb.data(), sizeof(int32_t) * tb.size()) ||
+        !put(te.data(), sizeof(int32_t) * te.size()) || !put(app.data(), app.size()) ||
+        !put(apm.data(), apm.size())) {
+        set_error("short write to %s", path);
+        return CUCO_ERR_INVALID_ARG;
+    }
+    const int closed = fclose(fp.f);
+    fp.f = nullptr;
+    if (closed != 0) {
+        set_error("closing %s failed", path);
+        return CUCO_ERR_INVALID_ARG;
+    }
+    return CUCO_OK;
+} CUCO_CATCH
+
+int cuco_checkpoint_read(cuco_engine *e, const char *path)
+try {
+    CHECK_ENGINE(e);
+    cuco_checkpoint_header h;
+    TRY(cuco_checkpoint_info(path, &h));
+    if (h.nboxes != (int32_t)e->box.size()) {
+        set_error("%s holds %d boxes, the handle %zu", path, h.nboxes, e->box.size());
+        return CUCO_ERR_INVALID_ARG;
+    }
+    TRY(check_count(e, h.n));
+    const size_t n = (size_t)h.n, nb = (size_t)h.nboxes;
+    CkFile fp;
+    fp.f = fopen(path, "rb");
+    if (!fp.f) {
+        set_error("cannot open checkpoint file %s", path);
+        return CUCO_ERR_INVALID_ARG;
+    }
+    auto get = [&](void *p, size_t bytes) {
+        return bytes == 0 || fread(p, 1, bytes, fp.f) == bytes;
+    };
+    std::vector<cuco_params> prm(nb);
+    std::vector<int64_t> counts(nb);
+    std::vector<double4> hp(n), hv(n);
+    std::vector<int32_t> tb(h.timeline ? n : 0), te(h.timeline ? n : 0);
+    std::vector<char> app(accel_bytes(h.pp_precision) * n), apm(accel_bytes(h.pm_precision) * n);
+    if (fseek(fp.f, (long)sizeof(h), SEEK_SET) != 0 ||
+        !get(prm.data(), sizeof(cuco_params) * nb) || !get(counts.data(), sizeof(int64_t) * nb) ||
+        !get(hp.data(), sizeof(double4) * n) || !get(hv.data(), sizeof(double4) * n) ||
+        !get(tb.data(), sizeof(int32_t) * tb.size()) ||
+        !get(te.data(), sizeof(int32_t) * te.size()) || !get(app.data(), app.size()) ||
+        !get(apm.data(), apm.size())) {
+        set_error("%s: short read", path);
+        return CUCO_ERR_INVALID_ARG;
+    }
+    // Everything is checked before the handle changes.
+    int64_t i = 0;
+    for (size_t b = 0; b < nb; ++b) {
+        const cuco_params &a = prm[b], &c = e->box[b].prm;
+        const int64_t want = e->ensemble ? (int64_t)e->box[b].cap : h.n;
+        if (a.pm_grid != c.pm_grid || a.box_size != c.box_size || a.G != c.G ||
+            a.asmth != c.asmth || a.rcut != c.rcut || counts[b] != want || !(a.soft_h > 0)) {
+            set_error("%s: box %zu was written with other parameters or particle count", path, b);
+            return CUCO_ERR_INVALID_ARG;
+        }
+        for (int64_t k = 0; k < counts[b]; ++k, ++i) {
+            const double4 q = hp[i];
+            if (!(q.x >= 0 && q.x < c.box_size && q.y >= 0 && q.y < c.box_size && q.z >= 0 &&
+                  q.z < c.box_size)) {
+                set_error("%s: particle %lld lies outside box %zu", path, (long long)i, b);
+                return CUCO_ERR_INVALID_ARG;
+            }
+        }
+    }
+    if (h.timeline) {
+        const cuco_timeline_state &st = h.state;
+        bool ok = st.ti_current >= 0 && st.ti_current <= kTimebase && st.pm_ti_begstep >= 0 &&
+                  st.pm_ti_begstep <= st.pm_ti_endstep && st.pm_ti_endstep <= kTimebase;
+        for (size_t k = 0; ok && k < n; ++k)
+            ok = tb[k] >= 0 && tb[k] <= te[k] && te[k] <= kTimebase;
+        if (!ok) {
+            set_error("%s: integer-timeline state out of range", path);
+            return CUCO_ERR_INVALID_ARG;
+        }
+    }
+
+    std::vector<double> pos3(3 * n), vel3(3 * n), mass(n);
+    for (size_t k = 0; k < n; ++k) {
+        pos3[3 * k] = hp[k].x;
+        pos3[3 * k + 1] = hp[k].y;
+        pos3[3 * k + 2] = hp[k].z;
+        mass[k] = hp[k].w;
+        vel3[3 * k] = hv[k].x;
+        vel3[3 * k + 1] = hv[k].y;
+        vel3[3 * k + 2] = hv[k].z;
+    }
+    for (size_t b = 0; b < nb; ++b) {
+        e->box[b].prm.soft_h = prm[b].soft_h;
+        set_pair_params(e->box[b]);
+    }
+    TRY(upload_impl(e, h.n, pos3.data(), vel3.data(), mass.data(), 0.0, false));
+    destroy_graph(e);
+    TRY(update_pair_constants(e));
+    if (h.timeline) {
+        TRY(timeline_setup(e, &h.params));
+        CUCO_HIP(hipMemcpyAsync(e->ti_beg, tb.data(), sizeof(int32_t) * n,
+                                hipMemcpyHostToDevice, e->stream));
+        CUCO_HIP(hipMemcpyAsync(e->ti_end, te.data(), sizeof(int32_t) * n,
+                                hipMemcpyHostToDevice, e->stream));
+        TRY(launch_fill_u32(reinterpret_cast<uint32_t *>(e->ti_scalars), 0u, 4, e->stream));
+        e->tls = h.state;
+        e->tl_init = true;
+    }
+    if (!app.empty())
+        CUCO_HIP(hipMemcpyAsync(accel_buffer(e, true, h.pp_precision), app.data(), app.size(),
+                                hipMemcpyHostToDevice, e->stream));
+    if (!apm.empty())
+        CUCO_HIP(hipMemcpyAsync(accel_buffer(e, false, h.pm_precision), apm.data(), apm.size(),
+                                hipMemcpyHostToDevice, e->stream));
+    CUCO_HIP(hipStreamSynchronize(e->stream));
+    e->pp_prec = h.pp_precision;
+    e->pm_prec = h.pm_precision;
+    return CUCO_OK;
+} CUCO_CATCH
 
 int cuco_download_records(cuco_engine *e, cuco_particle *P)
 try {

@@ -12,8 +12,13 @@
 // and the list is read into doubles (the reference reads "%g" into float;
 // the values are rounded to float here as well, so the integer output
 // times agree).
+//
+// Checkpoint files (no counterpart in the reference, which cannot restart):
+// their header is checked here, host only; the handle's state is written
+// and restored by the engine.
 #include <cmath>
 #include <cstdarg>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 
@@ -27,6 +32,7 @@ using cuco::set_error;
 
 static_assert(sizeof(cuco_header) == 32, "HEADER is 32 bytes");
 static_assert(sizeof(cuco_particle) == 60, "PARTICLE is 60 bytes");
+static_assert(sizeof(cuco_checkpoint_header) == 168, "checkpoint header is 168 bytes");
 
 namespace {
 
@@ -174,6 +180,60 @@ int cuco_init_records(cuco_particle *P, int64_t n, double time_begin)
         P[i].Ti_endstep = 0;
         P[i].Ti_begstep = 0;
     }
+    return CUCO_OK;
+}
+
+int64_t cuco_checkpoint_bytes(const cuco_checkpoint_header *h)
+{
+    auto prec_ok = [](int32_t p) { return p == -1 || p == CUCO_FP64 || p == CUCO_FP32; };
+    if (!h || h->nboxes < 1 || h->nboxes > (1 << 20) || h->n < 1 || h->n > (int64_t)UINT32_MAX ||
+        (h->timeline != 0 && h->timeline != 1) || !prec_ok(h->pp_precision) ||
+        !prec_ok(h->pm_precision))
+        return -1;
+    auto acc = [](int32_t p) -> int64_t { return p < 0 ? 0 : (p == CUCO_FP64 ? 32 : 16); };
+    const int64_t per_particle = 2 * 32 + (h->timeline ? 8 : 0) + acc(h->pp_precision) +
+                                 acc(h->pm_precision);
+    return (int64_t)sizeof(cuco_checkpoint_header) +
+           (int64_t)h->nboxes * (int64_t)(sizeof(cuco_params) + sizeof(int64_t)) +
+           h->n * per_particle;
+}
+
+int cuco_checkpoint_info(const char *path, cuco_checkpoint_header *header)
+{
+    if (!path || !header) {
+        set_error("null argument");
+        return CUCO_ERR_INVALID_ARG;
+    }
+    File fp;
+    fp.f = fopen(path, "rb");
+    if (!fp.f) {
+        set_error("cannot open checkpoint file %s", path);
+        return CUCO_ERR_INVALID_ARG;
+    }
+    cuco_checkpoint_header h;
+    if (fread(&h, sizeof(h), 1, fp.f) != 1 || memcmp(h.magic, "CUCOCKPT", 8) != 0) {
+        set_error("%s: not a checkpoint file", path);
+        return CUCO_ERR_INVALID_ARG;
+    }
+    if (h.version != 1 || h.reserved != 0) {
+        set_error("%s: checkpoint version %d is not supported", path, h.version);
+        return CUCO_ERR_INVALID_ARG;
+    }
+    const int64_t want = cuco_checkpoint_bytes(&h);
+    if (want < 0) {
+        set_error("%s: header fields out of range", path);
+        return CUCO_ERR_INVALID_ARG;
+    }
+    if (fseek(fp.f, 0, SEEK_END) != 0) {
+        set_error("%s: cannot seek", path);
+        return CUCO_ERR_INVALID_ARG;
+    }
+    const long size = ftell(fp.f);
+    if ((int64_t)size != want) {
+        set_error("%s: %ld bytes, but its header needs %lld", path, size, (long long)want);
+        return CUCO_ERR_INVALID_ARG;
+    }
+    *header = h;
     return CUCO_OK;
 }
 

@@ -50,7 +50,8 @@ EXPORTED = [
     "cuco_read_snapshot", "cuco_read_output_list", "cuco_find_next_output_time",
     "cuco_download_records", "cuco_upload_records", "cuco_timeline_init",
     "cuco_timeline_run", "cuco_get_timesteps", "cuco_init_records",
-    "cuco_radix_sort_noalloc", "cuco_device_drift_table",
+    "cuco_radix_sort_noalloc", "cuco_device_drift_table", "cuco_checkpoint_bytes",
+    "cuco_checkpoint_info", "cuco_checkpoint_write", "cuco_checkpoint_read",
 ]
 
 
@@ -123,6 +124,14 @@ class TimelineState(C.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class CheckpointHeader(C.Structure):
+    """Header of a checkpoint file (cuco_checkpoint_header, 168 bytes)."""
+    _fields_ = [("magic", C.c_char * 8), ("version", C.c_int32), ("nboxes", C.c_int32),
+                ("n", C.c_int64), ("timeline", C.c_int32), ("pp_precision", C.c_int32),
+                ("pm_precision", C.c_int32), ("reserved", C.c_int32),
+                ("params", Timeline), ("state", TimelineState)]
+
+
 PARTICLE_DTYPE = np.dtype([
     ("Pos", "<f4", 3), ("Mass", "<f4"), ("Vel", "<f4", 3),
     ("GravAccel", "<f4", 3), ("GravPM", "<f4", 3),
@@ -164,6 +173,10 @@ def lib():
         "cuco_get_timesteps": (C.c_int, [vp, C.POINTER(i32), C.POINTER(i32)]),
         "cuco_upload_records": (C.c_int, [vp, vp, i64]),
         "cuco_init_records": (C.c_int, [vp, i64, C.c_double]),
+        "cuco_checkpoint_bytes": (i64, [C.POINTER(CheckpointHeader)]),
+        "cuco_checkpoint_info": (C.c_int, [C.c_char_p, C.POINTER(CheckpointHeader)]),
+        "cuco_checkpoint_write": (C.c_int, [vp, C.c_char_p]),
+        "cuco_checkpoint_read": (C.c_int, [vp, C.c_char_p]),
         "cuco_upload_particles": (C.c_int, [vp, i64, dp, dp, dp, C.c_double]),
         "cuco_download_particles": (C.c_int, [vp, dp, dp]),
         "cuco_download_accel": (C.c_int, [vp, dp, dp]),
@@ -291,6 +304,18 @@ def read_snapshot(path: str, header_only: bool = False):
                                     P.ctypes.data_as(C.c_void_p), P.shape[0]),
            "cuco_read_snapshot")
     return h, P
+
+
+def checkpoint_info(path: str) -> dict:
+    """The checked header of a checkpoint file (host only)."""
+    h = CheckpointHeader()
+    _check(lib().cuco_checkpoint_info(os.fsencode(path), C.byref(h)), "cuco_checkpoint_info")
+    out = {f: getattr(h, f) for f, _ in CheckpointHeader._fields_
+           if f not in ("params", "state")}
+    out["params"] = {f: getattr(h.params, f) for f, _ in Timeline._fields_}
+    out["state"] = h.state.as_dict()
+    out["bytes"] = int(lib().cuco_checkpoint_bytes(C.byref(h)))
+    return out
 
 
 def init_records(P: np.ndarray, time_begin: float) -> None:
@@ -527,6 +552,15 @@ class Engine:
         _check(lib().cuco_timeline_run(self.h, precision, max_steps, ti_stop,
                                        C.byref(s)), "cuco_timeline_run")
         return s.as_dict()
+
+    def checkpoint_write(self, path: str) -> None:
+        """Particles, velocities, last accelerations and integer timeline."""
+        _check(lib().cuco_checkpoint_write(self.h, os.fsencode(path)), "cuco_checkpoint_write")
+
+    def checkpoint_read(self, path: str) -> None:
+        """Restores a checkpoint written by a handle with the same parameters."""
+        _check(lib().cuco_checkpoint_read(self.h, os.fsencode(path)), "cuco_checkpoint_read")
+        self.n = int(checkpoint_info(path)["n"])
 
     def timesteps(self):
         b = np.empty(self.n, np.int32)

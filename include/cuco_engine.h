@@ -373,6 +373,54 @@ int cuco_upload_records(cuco_engine *e, const cuco_particle *P, int64_t n);
 int cuco_init_records(cuco_particle *P, int64_t n, double time_begin);
 
 /* ---------------------------------------------------------------- */
+/* Checkpoints (restart)                                             */
+/* ---------------------------------------------------------------- */
+
+/* The reference has no restart: savepositions (io.c:9-40) keeps neither
+ * the run's internal velocities nor its integer timeline, and read_ic
+ * treats a snapshot as new initial conditions (init.c:27-36 rescales Vel
+ * and zeroes every Ti).  A checkpoint holds the whole state a handle needs
+ * to continue: cuco_checkpoint_read into a handle created with the same
+ * parameters (and, for bit-identical FP32 results, the same tuning), then
+ * cuco_timeline_run continues exactly as the handle that wrote it would.
+ *
+ * File layout, native byte order: this header; nboxes cuco_params (the
+ * boxes' parameters, soft_h the softening in force); nboxes int64 particle
+ * counts; then per particle in original order n double4 (x, y, z, mass),
+ * n double4 (internal velocity, 0); when timeline = 1, n int32 Ti_begstep
+ * and n int32 Ti_endstep; when pp_precision >= 0, n accelerations of the
+ * short-range force (double4 for CUCO_FP64, float4 for CUCO_FP32, the
+ * fourth component unused); then likewise the PM force. */
+typedef struct cuco_checkpoint_header {
+    char magic[8];              /* "CUCOCKPT" */
+    int32_t version;            /* 1 */
+    int32_t nboxes;             /* 1, or the ensemble's box count */
+    int64_t n;                  /* particles */
+    int32_t timeline;           /* 1: params, state and the Ti arrays are held */
+    int32_t pp_precision;       /* precision of the stored PP accelerations, -1: none */
+    int32_t pm_precision;       /* likewise for the PM accelerations */
+    int32_t reserved;           /* 0 */
+    cuco_timeline params;       /* integrator parameters (timeline = 1) */
+    cuco_timeline_state state;  /* integer-timeline state (timeline = 1) */
+} cuco_checkpoint_header;
+
+/* Size in bytes of the checkpoint file a header describes (-1 when its
+ * fields are out of range). */
+int64_t cuco_checkpoint_bytes(const cuco_checkpoint_header *header);
+/* Reads and checks a checkpoint's header (magic, version, field ranges,
+ * file size), host only. */
+int cuco_checkpoint_info(const char *path, cuco_checkpoint_header *header);
+/* Writes the handle's particles, velocities, last accelerations and, after
+ * cuco_timeline_init, its integer timeline to path. */
+int cuco_checkpoint_write(cuco_engine *e, const char *path);
+/* Restores a checkpoint into a handle: the box count, per-box particle
+ * counts and parameters (except the softening, which is restored) must
+ * match the handle's; otherwise CUCO_ERR_INVALID_ARG and the handle is
+ * unchanged.  Replaces the particles; the integer timeline is restored
+ * when the file holds one, else the handle has none. */
+int cuco_checkpoint_read(cuco_engine *e, const char *path);
+
+/* ---------------------------------------------------------------- */
 /* Device-pointer API (mirrors proto.cuh:26-58)                      */
 /* ---------------------------------------------------------------- */
 
