@@ -14,7 +14,15 @@
  *              <max_dt> [max_steps]
  *       the reference's run(): individual timesteps on the integer timeline,
  *       a snapshot <outbase>_<k>.cuco at every listed output time
- *       (find_next_sync_point_and_drift, run.c:47-56) and at the end.
+ *       (find_next_sync_point_and_drift, run.c:47-56) and at the end, and
+ *       beside each listed one a checkpoint <outbase>_<k>.ckpt, or
+ *
+ *   driver resume <ic> <outbase> <pmgrid> <fp64|fp32> <outputlist> <ckpt>
+ *                 <k> [max_steps]
+ *       continues a run from its checkpoint (which the reference cannot:
+ *       it has no restart), numbering the next snapshot <k>; the snapshots
+ *       are those the uninterrupted run writes.  <ic> gives the box
+ *       (HEADER) and the particle count.
  *
  * Units and cosmology are the reference's (begrun.c:33-70): kpc/h,
  * 1e10 Msun/h, km/s, Omega0 = 0.3, OmegaLambda = 0.7, h = 0.7.  The IC
@@ -125,9 +133,45 @@ static void savepositions(const char *fname) /* io.c:9-40 */
     check(cuco_write_snapshot(fname, &h, P), "savepositions");
 }
 
+/* The output loop of run() (run.c:9-26) from the handle's current state:
+ * snapshots <outbase>_<count>.cuco (and checkpoints .ckpt) at the listed
+ * times, a last snapshot at the end. */
+static int advance(const cuco_timeline *t, const double *times, int32_t ntimes,
+                   int32_t ti_next, int count, int steps, int max_steps, const char *outbase)
+{
+    const double interval = (log(t->time_max) - log(t->time_begin)) / CUCO_TIMEBASE;
+    char fname[512];
+    for (;;) {
+        cuco_timeline_state s;
+        const int budget = max_steps - steps;
+        if (budget <= 0) break;
+        check(cuco_timeline_run(engine, precision, budget,
+                                ti_next < 2 * CUCO_TIMEBASE ? ti_next : -1, &s),
+              "run");
+        steps = s.steps;
+        All.Time = s.time;
+        if (s.at_stop) {
+            check(cuco_download_records(engine, P), "download");
+            snprintf(fname, sizeof(fname), "%s_%d.cuco", outbase, count);
+            savepositions(fname);
+            printf("output %s at a = %g after %d steps\n", fname, s.time, s.steps);
+            snprintf(fname, sizeof(fname), "%s_%d.ckpt", outbase, count++);
+            check(cuco_checkpoint_write(engine, fname), "checkpoint");
+            ti_next = cuco_find_next_output_time(times, ntimes, t->time_begin,
+                                                 t->time_max, interval, ti_next + 1);
+            continue;
+        }
+        if (s.finished) break;
+    }
+    check(cuco_download_records(engine, P), "download");
+    snprintf(fname, sizeof(fname), "%s_%d.cuco", outbase, count++);
+    savepositions(fname); /* run.c:27 */
+    printf("final %s at a = %g after %d steps\n", fname, All.Time, steps);
+    return 0;
+}
+
 static int run(int argc, char **argv)
 {
-    const char *outbase = argv[3];
     double times[500];
     int32_t ntimes = 0;
     check(cuco_read_output_list(argv[6], times, 500, &ntimes), "read_outputlist");
@@ -151,45 +195,47 @@ static int run(int argc, char **argv)
     check(cuco_upload_records(engine, P, NumPart), "upload");
     check(cuco_timeline_init(engine, &t), "init");
     const double interval = (log(t.time_max) - log(t.time_begin)) / CUCO_TIMEBASE;
-    int32_t ti_next = cuco_find_next_output_time(times, ntimes, t.time_begin, t.time_max,
-                                                 interval, 0);
-    int count = 0, steps = 0;
-    char fname[512];
-    for (;;) {
-        cuco_timeline_state s;
-        const int budget = max_steps - steps;
-        if (budget <= 0) break;
-        check(cuco_timeline_run(engine, precision, budget,
-                                ti_next < 2 * CUCO_TIMEBASE ? ti_next : -1, &s),
-              "run");
-        steps = s.steps;
-        All.Time = s.time;
-        if (s.at_stop) {
-            check(cuco_download_records(engine, P), "download");
-            snprintf(fname, sizeof(fname), "%s_%d.cuco", outbase, count++);
-            savepositions(fname);
-            printf("output %s at a = %g after %d steps\n", fname, s.time, s.steps);
-            ti_next = cuco_find_next_output_time(times, ntimes, t.time_begin,
-                                                 t.time_max, interval, ti_next + 1);
-            continue;
-        }
-        if (s.finished) break;
+    const int32_t ti_next =
+        cuco_find_next_output_time(times, ntimes, t.time_begin, t.time_max, interval, 0);
+    return advance(&t, times, ntimes, ti_next, 0, 0, max_steps, argv[3]);
+}
+
+static int resume(int argc, char **argv)
+{
+    double times[500];
+    int32_t ntimes = 0;
+    check(cuco_read_output_list(argv[6], times, 500, &ntimes), "read_outputlist");
+    cuco_checkpoint_header ck;
+    check(cuco_checkpoint_info(argv[7], &ck), "checkpoint_info");
+    if (!ck.timeline || ck.n != NumPart) {
+        fprintf(stderr, "%s: no integer timeline, or not %d particles\n", argv[7], NumPart);
+        return 1;
     }
-    check(cuco_download_records(engine, P), "download");
-    snprintf(fname, sizeof(fname), "%s_%d.cuco", outbase, count++);
-    savepositions(fname); /* run.c:27 */
-    printf("final %s at a = %g after %d steps\n", fname, All.Time, steps);
-    return 0;
+    const cuco_timeline t = ck.params;
+    const int max_steps = argc > 9 ? atoi(argv[9]) : 1 << 30;
+    All.SofteningHalo = t.softening_halo;
+    All.SofteningHaloMaxPhys = t.softening_halo_max_phys;
+    All.Time = ck.state.time;
+    linklist_init();
+    check(cuco_checkpoint_read(engine, argv[7]), "checkpoint_read");
+    const double interval = (log(t.time_max) - log(t.time_begin)) / CUCO_TIMEBASE;
+    const int32_t ti_next = cuco_find_next_output_time(times, ntimes, t.time_begin, t.time_max,
+                                                       interval, ck.state.ti_current + 1);
+    return advance(&t, times, ntimes, ti_next, atoi(argv[8]), ck.state.steps, max_steps,
+                   argv[3]);
 }
 
 int main(int argc, char **argv)
 {
-    if (argc < 7 || (strcmp(argv[1], "forces") && strcmp(argv[1], "run")) ||
-        (!strcmp(argv[1], "run") && argc < 13)) {
+    if (argc < 7 ||
+        (strcmp(argv[1], "forces") && strcmp(argv[1], "run") && strcmp(argv[1], "resume")) ||
+        (!strcmp(argv[1], "run") && argc < 13) || (!strcmp(argv[1], "resume") && argc < 9)) {
         fprintf(stderr, "usage: driver forces <ic> <out> <pmgrid> <fp64|fp32> <softening>\n"
                         "       driver run <ic> <outbase> <pmgrid> <fp64|fp32> <outputlist>"
                         " <a_begin> <a_max> <soft> <soft_max_phys> <err_tol> <max_dt>"
-                        " [max_steps]\n");
+                        " [max_steps]\n"
+                        "       driver resume <ic> <outbase> <pmgrid> <fp64|fp32> <outputlist>"
+                        " <ckpt> <k> [max_steps]\n");
         return 1;
     }
     set_units();
@@ -198,6 +244,7 @@ int main(int argc, char **argv)
     precision = strcmp(argv[5], "fp32") ? CUCO_FP64 : CUCO_FP32;
     pm_init_periodic();
     if (!strcmp(argv[1], "run")) return run(argc, argv);
+    if (!strcmp(argv[1], "resume")) return resume(argc, argv);
     All.Time = header.Time;
     All.SofteningHalo = atof(argv[6]);
     All.SofteningHaloMaxPhys = 1e30;

@@ -113,3 +113,32 @@ def test_run_mode_writes_the_scheduled_snapshots(tmp_path):
     assert k == 2
     h, Q = E.read_snapshot(str(tmp_path / f"snap_{k}.cuco"))
     assert Q.tobytes() == eng.download_records().tobytes()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_resume_from_a_checkpoint_writes_the_same_snapshots(tmp_path, prec):
+    box = halo_box(12, 24, 40.0, n_halos=4, seed=9)
+    P = ic_records(box)
+    ic = str(tmp_path / "ic.cuco")
+    E.write_snapshot(ic, E.Header(box.N, float(P["Mass"][0]), 0.2, 4.0, box.L,
+                                  0.3, 0.7, 0.7), P)
+    lst = str(tmp_path / "list.txt")
+    with open(lst, "w") as f:
+        f.write("0.201\n0.203\n0.206\n")
+    args = [str(box.M), prec, lst]
+    run_driver(["run", ic, str(tmp_path / "full"), *args, "0.2", "0.3",
+                repr(box.softening), repr(box.softening * 0.25), "0.025",
+                repr(math.log(1.5) / 8), "60"])
+    ckpt = str(tmp_path / "full_0.ckpt")
+    info = E.checkpoint_info(ckpt)
+    assert info["timeline"] == 1 and info["n"] == box.N and info["state"]["at_stop"] == 1
+    run_driver(["resume", ic, str(tmp_path / "part"), *args, ckpt, "1", "60"])
+    n = 0
+    for k in range(1, 5):
+        full, part = tmp_path / f"full_{k}.cuco", tmp_path / f"part_{k}.cuco"
+        assert full.exists() == part.exists(), k
+        if full.exists():
+            assert full.read_bytes() == part.read_bytes(), k
+            n += 1
+    assert n >= 2                                      # a listed output and the end
+    assert not (tmp_path / "part_0.cuco").exists()
