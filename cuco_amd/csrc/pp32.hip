@@ -273,7 +273,7 @@ __device__ inline void interact_sym(const float4 *__restrict__ rot,
     const int lane = threadIdx.x & 63;
     const float4 *rl = rot + lane;
     const uint32_t *il = rotid + lane;
-    for (int s = 0; s < 64; s += 4) {
+    auto group = [&](int s) {
         float dx[4], dy[4], dz[4], r[4], fac[4], t[4], mj[4];
         bool keep[4], inside[4];
 #pragma unroll
@@ -322,6 +322,15 @@ __device__ inline void interact_sym(const float4 *__restrict__ rot,
             jy = rotate_down(fmaf(-dy[q], fj, jy));
             jz = rotate_down(fmaf(-dz[q], fj, jz));
         }
+    };
+    // The common case (no masks, far, no clamp) is unrolled whole: the LDS
+    // reads then take their offsets from the instruction and the loop keeps
+    // no counter (1-3% at halo256).
+    if (!MASKED && !NEAR && !CLAMP) {
+#pragma unroll
+        for (int s = 0; s < 64; s += 4) group(s);
+    } else {
+        for (int s = 0; s < 64; s += 4) group(s);
     }
 }
 
@@ -341,7 +350,7 @@ __device__ inline void interact_sym2(const float4 *__restrict__ rot,
     const int lane = threadIdx.x & 63;
     const float4 *rl = rot + lane;
     const uint32_t *il = rotid + lane;
-    for (int s = 0; s < 64; s += 2) {
+    auto group = [&](int s) {
         float dxa[2], dya[2], dza[2], ra[2], fa[2], ta[2];
         float dxb[2], dyb[2], dzb[2], rb[2], fb[2], tb[2], mj[2];
         bool keepa[2], keepb[2], ina[2], inb[2];
@@ -420,6 +429,12 @@ __device__ inline void interact_sym2(const float4 *__restrict__ rot,
             jy = rotate_down(fmaf(-dyb[q], fjb, fmaf(-dya[q], fja, jy)));
             jz = rotate_down(fmaf(-dzb[q], fjb, fmaf(-dza[q], fja, jz)));
         }
+    };
+    if (!MASKED && !NEAR && !CLAMP) {
+#pragma unroll
+        for (int s = 0; s < 64; s += 2) group(s);
+    } else {
+        for (int s = 0; s < 64; s += 2) group(s);
     }
 }
 
