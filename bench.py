@@ -127,6 +127,23 @@ def ensemble_bench(E, ics, nboxes, steps, tune=True):
     else:
         tuning = ens.tuning()
 
+    # FP32 against the FP64 path on the initial state (every box; the
+    # denominator is each particle's FP64 total acceleration).
+    ens.upload(pos, vel, mass)
+    ens.compute_accelerations(E.FP64, True)
+    ref_pp, ref_pm = ens.accelerations()
+    ens.compute_accelerations(E.FP32, True)
+    a_pp, a_pm = ens.accelerations()
+    den = np.linalg.norm(ref_pp + ref_pm, axis=1)
+    den = np.where(den > 0, den, 1.0)
+    errors = {
+        "pp": percentiles(np.linalg.norm(a_pp - ref_pp, axis=1) / den),
+        "pm": percentiles(np.linalg.norm(a_pm - ref_pm, axis=1) / den),
+        "total": percentiles(np.linalg.norm(a_pp + a_pm - ref_pp - ref_pm, axis=1) / den),
+    }
+    del ref_pp, ref_pm, a_pp, a_pm, den
+    log(f"[bench] ensemble FP32 vs FP64 {errors['total']}")
+
     def run_batched():
         ens.upload(pos, vel, mass)
         ens.compute_accelerations(E.FP32, True)
@@ -174,6 +191,7 @@ def ensemble_bench(E, ics, nboxes, steps, tune=True):
         "batched_over_looped": t_looped / t_batched,
         "batched_evals_per_s": n_total * steps / t_batched,
         "bit_identical_to_looped": identical,
+        "error_vs_fp64": errors,
     }
 
 
