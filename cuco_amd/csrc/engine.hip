@@ -1514,14 +1514,19 @@ int timeline_step(cuco_engine *e, int precision, int32_t ti_stop, bool &stopped)
     return CUCO_OK;
 }
 
+bool timeline_params_ok(const cuco_timeline *t)
+{
+    return t && t->time_begin > 0 && t->time_max > t->time_begin && t->hubble > 0 &&
+           t->err_tol_int_accuracy > 0 && t->max_size_timestep > 0 &&
+           t->min_size_timestep >= 0 && t->softening_halo > 0 &&
+           t->softening_halo_max_phys > 0;
+}
+
 // The parts of init that do not touch the particles' timesteps: parameter
 // checks, buffers, the drift and kick tables and the timebase.
 int timeline_setup(cuco_engine *e, const cuco_timeline *t)
 {
-    if (!t || !(t->time_begin > 0) || !(t->time_max > t->time_begin) ||
-        !(t->hubble > 0) || !(t->err_tol_int_accuracy > 0) ||
-        !(t->max_size_timestep > 0) || !(t->min_size_timestep >= 0) ||
-        !(t->softening_halo > 0) || !(t->softening_halo_max_phys > 0)) {
+    if (!timeline_params_ok(t)) {
         set_error("timeline parameters out of range");
         return CUCO_ERR_INVALID_ARG;
     }
@@ -1791,12 +1796,13 @@ try {
     }
     if (h.timeline) {
         const cuco_timeline_state &st = h.state;
-        bool ok = st.ti_current >= 0 && st.ti_current <= kTimebase && st.pm_ti_begstep >= 0 &&
+        bool ok = timeline_params_ok(&h.params) && st.ti_current >= 0 &&
+                  st.ti_current <= kTimebase && st.pm_ti_begstep >= 0 &&
                   st.pm_ti_begstep <= st.pm_ti_endstep && st.pm_ti_endstep <= kTimebase;
         for (size_t k = 0; ok && k < n; ++k)
             ok = tb[k] >= 0 && tb[k] <= te[k] && te[k] <= kTimebase;
         if (!ok) {
-            set_error("%s: integer-timeline state out of range", path);
+            set_error("%s: integer-timeline parameters or state out of range", path);
             return CUCO_ERR_INVALID_ARG;
         }
     }

@@ -136,3 +136,29 @@ def test_other_handles_refuse_and_stay_unchanged(tmp_path):
     short.write_bytes(open(path, "rb").read()[:-8])
     with pytest.raises(E.CucoError):
         make_engine(box).checkpoint_read(str(short))
+
+
+def test_out_of_range_timeline_is_refused_before_any_change(tmp_path):
+    box = halo_box(10, 20, 30.0, n_halos=3, seed=8)
+    pos, vel, mass = ics.generate(box)
+    a = make_engine(box)
+    a.upload(pos, vel, mass)
+    a.timeline_init(**timeline_params(box))
+    a.timeline_run(E.FP64, max_steps=2)
+    path = tmp_path / "t.ckpt"
+    a.checkpoint_write(str(path))
+    data = bytearray(path.read_bytes())
+    off = E.CheckpointHeader.params.offset + E.Timeline.time_max.offset
+    data[off:off + 8] = np.float64(0.0).tobytes()       # time_max <= time_begin
+    bad = tmp_path / "bad.ckpt"
+    bad.write_bytes(bytes(data))
+    assert E.checkpoint_info(str(bad))["params"]["time_max"] == 0.0
+
+    b = make_engine(box)
+    b.upload(pos[::-1].copy(), vel, mass)
+    before = b.download_particles()
+    with pytest.raises(E.CucoError) as err:
+        b.checkpoint_read(str(bad))
+    assert err.value.status == 1
+    for x, y in zip(before, b.download_particles()):
+        assert np.array_equal(x, y)
