@@ -205,14 +205,17 @@ __global__ __launch_bounds__(kThreads) void k_rs_downsweep(
 
 // Each i in [0, n] owns the keys (key[i-1], key[i]] and writes off[c] = i
 // for them (key[-1] = -1, key[n] = nkeys): every entry is written exactly
-// once.  The key gaps are uneven (voids), so a wave concatenates the ranges
-// of its 64 lanes and fills them cooperatively.
+// once.  A wave's 64 ranges follow one another, so together they are the
+// keys lo_0 .. lo_0 + total - 1, which the wave writes 64 at a time
+// (coalesced).  An entry's owner is the last non-empty range starting at
+// or before it: each window marks the starts that fall inside it in LDS,
+// and a lane takes the highest marked slot at or below it (a ballot and a
+// leading-zero count), or the owner carried from the window before.
 __global__ __launch_bounds__(256) void k_cell_offsets(
     const uint32_t *__restrict__ skeys, uint32_t n, uint32_t nkeys,
     uint32_t *__restrict__ off)
 {
-    __shared__ uint32_t s_pre[4][65];
-    __shared__ uint32_t s_lo[4][64];
+    __shared__ uint32_t s_own[4][64];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     uint32_t lo = 0, len = 0;
@@ -222,20 +225,27 @@ __global__ __launch_bounds__(256) void k_cell_offsets(
         lo = (uint32_t)(prev + 1);
         len = cur > prev ? (uint32_t)(cur - prev) : 0u;
     }
-    uint32_t inc = wave_incl_scan(len);
-    s_pre[w][lane] = inc - len;
-    s_lo[w][lane] = lo;
-    if (lane == 63) s_pre[w][64] = inc;
-    __syncthreads();
-    const uint32_t total = s_pre[w][64];
-    const uint32_t first = (blockIdx.x * 256u) + (uint32_t)(w * 64);
-    for (uint32_t f = lane; f < total; f += 64) {
-        uint32_t a = 0, b = 64;   // s_pre[w][a] <= f < s_pre[w][b]
-        while (b - a > 1) {
-            const uint32_t mid = (a + b) >> 1;
-            if (s_pre[w][mid] <= f) a = mid; else b = mid;
-        }
-        off[s_lo[w][a] + (f - s_pre[w][a])] = first + a;
+    const uint32_t inc = wave_incl_scan(len);
+    const uint32_t start = inc - len;  // my range's place in the wave's run
+    const uint32_t total = __builtin_amdgcn_readlane(inc, 63);
+    // Lane 0 holds the run's first key (a wave past n has total = 0).
+    const uint32_t key0 = __builtin_amdgcn_readfirstlane(lo);
+    const uint32_t first = blockIdx.x * 256u + (uint32_t)(w * 64);
+    uint32_t *own = s_own[w];
+    uint32_t carry = 0;  // owner of the entry before the window
+    for (uint32_t f0 = 0; f0 < total; f0 += 64) {
+        own[lane] = 0xffffffffu;
+        wave_sync();
+        if (len > 0 && start >= f0 && start - f0 < 64u) own[start - f0] = (uint32_t)lane;
+        wave_sync();
+        const uint64_t marks = __ballot(own[lane] != 0xffffffffu);
+        const uint64_t upto = marks & ((2ull << lane) - 1ull);  // slots 0..lane
+        const int q = upto ? 63 - (int)__clzll((long long)upto) : 0;
+        const uint32_t o = upto ? own[q] : carry;
+        const uint32_t f = f0 + (uint32_t)lane;
+        if (f < total) off[key0 + f] = first + o;
+        carry = __builtin_amdgcn_readlane(o, 63);
+        wave_sync();
     }
 }
 
