@@ -56,3 +56,32 @@ def test_sort_keys_only_and_ignores_high_bits():
     assert vs is None
     rk, _ = stable(keys, 16)
     assert np.array_equal(ks, rk)
+
+
+@pytest.mark.parametrize("M", [512, 320])
+def test_engine_sort_with_25_to_27_bit_keys(M, orc):
+    """The engine's own sort on a 512^3 (27-bit keys) or 320^3 (25-bit)
+    mesh with a dense clump of repeated keys: the order must be the
+    oracle's (x-major keys) and a stable order of the Morton keys."""
+    L, n = 100.0, 20000
+    rng = np.random.default_rng(M)
+    pos = rng.random((n, 3)) * L
+    pos[:4000] = L * 0.5 + rng.random((4000, 3)) * (0.02 * L)  # a clump: repeated keys
+    mass = np.ones(n)
+    asmth = 1.25 * L / M
+    kw = dict(pm_grid=M, box_size=L, G=1.0, asmth=asmth, rcut=4.5 * asmth, soft_h=0.1 * L / M)
+    eng = E.Engine(max_particles=n, **kw)
+    eng.upload(pos, None, mass)
+    eng.gravity_linklist(E.FP64)
+    keys, idx = eng.sorted_order()
+    p = orc.params(M, L, 1.0, asmth, 4.5 * asmth, 0.1 * L / M)
+    ok, oi = orc.calc_keys(p, pos)
+    ok, oi = orc.sort_pairs(ok, oi)
+    assert int(keys.max()) >= 1 << 24
+    assert np.array_equal(keys, ok) and np.array_equal(idx, oi)
+    eng.gravity_linklist(E.FP32)
+    k32, i32 = eng.sorted_order()
+    dk = np.diff(k32.astype(np.int64))
+    assert (dk >= 0).all()
+    assert (np.diff(i32.astype(np.int64))[dk == 0] > 0).all()
+    assert np.array_equal(np.sort(i32), np.arange(n, dtype=np.uint32))
