@@ -117,6 +117,9 @@ struct cuco_engine {
     SortScratch sc;
     uint32_t *skeys = nullptr, *perm = nullptr;
     uint32_t *off = nullptr;
+    uint4 *runs = nullptr;     // long empty key ranges of the cell table
+    uint32_t *nruns = nullptr;
+    uint32_t run_cap = 0;
     int sorted_mode = -1;
     double4 *spos64 = nullptr;
     float4 *spos32 = nullptr;
@@ -475,7 +478,8 @@ int sort_particles(cuco_engine *e, int mode)
                          e->stream, &e->skeys, &e->perm));
     TRY(stage_stop(e, CUCO_STAGE_SORT));
     TRY(stage_start(e, CUCO_STAGE_CELLS));
-    TRY(launch_cell_offsets(e->skeys, e->n, e->nkeys[mode], e->off, e->stream));
+    TRY(launch_cell_offsets(e->skeys, e->n, e->nkeys[mode], e->off, e->runs,
+                            e->run_cap, e->nruns, e->stream));
     if (mode == kKeyXMajor) {
         TRY(launch_reorder64(e->pos, e->perm, e->n, e->spos64, e->stream));
     } else {
@@ -857,6 +861,9 @@ int create_engine(const cuco_params *params, const uint32_t *caps, int nboxes,
     e->sc.capacity = e->capacity;
     const size_t offw = (size_t)std::max(e->nkeys[0], e->nkeys[1]) + 1;
     ALLOC(e->off, sizeof(uint32_t) * offw);
+    e->run_cap = cell_run_capacity((uint32_t)(offw - 1));
+    ALLOC(e->runs, sizeof(uint4) * e->run_cap);
+    ALLOC(e->nruns, sizeof(uint32_t));
     ALLOC(e->spos64, sizeof(double4) * cap);
     ALLOC(e->spos32, sizeof(float4) * cap);
     ALLOC(e->table, sizeof(float) * kNTab);
@@ -990,7 +997,8 @@ int cuco_destroy(cuco_engine *e)
     free_ptr(e->keys); free_ptr(e->vals);
     free_ptr(e->sc.keys_tmp); free_ptr(e->sc.vals_tmp);
     free_ptr(e->sc.counts); free_ptr(e->sc.totals);
-    free_ptr(e->off); free_ptr(e->spos64); free_ptr(e->spos32);
+    free_ptr(e->off); free_ptr(e->runs); free_ptr(e->nruns);
+    free_ptr(e->spos64); free_ptr(e->spos32);
     free_ptr(e->table); free_ptr(e->table32); free_ptr(e->dcount);
     free_ptr(e->p32);
     free_ptr(e->psym.iacc); free_ptr(e->psym.jacc);

@@ -72,8 +72,11 @@ uint32_t static_sort_capacity();
 int radix_sort_static(uint32_t *keys, uint32_t *vals, uint32_t n, int key_bits,
                       hipStream_t s);
 // off[c] = first sorted index with key >= c, c in [0, nkeys]; off[nkeys] = n.
+// runs (cell_run_capacity(nkeys) entries) and nruns (one word) are scratch.
 int launch_cell_offsets(const uint32_t *skeys, uint32_t n, uint32_t nkeys,
-                        uint32_t *off, hipStream_t s);
+                        uint32_t *off, uint4 *runs, uint32_t run_cap, uint32_t *nruns,
+                        hipStream_t s);
+uint32_t cell_run_capacity(uint32_t nkeys);
 int launch_reorder64(const double4 *pos, const uint32_t *perm, uint32_t n,
                      double4 *spos, hipStream_t s);
 int launch_reorder32(const double4 *pos, const uint32_t *perm,

@@ -205,17 +205,27 @@ __global__ __launch_bounds__(kThreads) void k_rs_downsweep(
 
 // Each i in [0, n] owns the keys (key[i-1], key[i]] and writes off[c] = i
 // for them (key[-1] = -1, key[n] = nkeys): every entry is written exactly
-// once.  A wave's 64 ranges follow one another, so together they are the
-// keys lo_0 .. lo_0 + total - 1, which the wave writes 64 at a time
-// (coalesced).  An entry's owner is the last non-empty range starting at
-// or before it: each window marks the starts that fall inside it in LDS,
-// and a lane takes the highest marked slot at or below it (a ballot and a
-// leading-zero count), or the owner carried from the window before.
+// once.  A wave's ranges follow one another in key order and are written
+// 64 entries at a time (coalesced); an entry's owner is the last non-empty
+// range starting at or before it: each window marks the starts that fall
+// inside it in LDS, and a lane takes the highest marked slot at or below
+// it (a ballot and a leading-zero count), or the owner carried from the
+// window before.  A range longer than kLongRun (a void, or the padding of
+// a Morton key space around a mesh that is not a power of two: runs of
+// millions of empty keys) would keep its wave filling alone, so it goes
+// to a list of pieces of at most kRunPiece entries that k_fill_runs spreads
+// over the whole chip.
+constexpr uint32_t kLongRun = 2048;
+constexpr uint32_t kRunPiece = 65536;
+
 __global__ __launch_bounds__(256) void k_cell_offsets(
     const uint32_t *__restrict__ skeys, uint32_t n, uint32_t nkeys,
-    uint32_t *__restrict__ off)
+    uint32_t *__restrict__ off, uint4 *__restrict__ runs, uint32_t run_cap,
+    uint32_t *__restrict__ nruns)
 {
     __shared__ uint32_t s_own[4][64];
+    __shared__ uint32_t s_lo[4][64];
+    __shared__ uint32_t s_start[4][64];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     uint32_t lo = 0, len = 0;
@@ -225,13 +235,20 @@ __global__ __launch_bounds__(256) void k_cell_offsets(
         lo = (uint32_t)(prev + 1);
         len = cur > prev ? (uint32_t)(cur - prev) : 0u;
     }
+    if (len > kLongRun) {
+        const uint32_t pieces = (len + kRunPiece - 1) / kRunPiece;
+        const uint32_t slot = atomicAdd(nruns, pieces);
+        for (uint32_t k = 0; k < pieces && slot + k < run_cap; ++k)
+            runs[slot + k] = make_uint4(lo + k * kRunPiece, min(kRunPiece, len - k * kRunPiece), i, 0u);
+        len = 0;
+    }
     const uint32_t inc = wave_incl_scan(len);
-    const uint32_t start = inc - len;  // my range's place in the wave's run
+    const uint32_t start = inc - len;  // my range's place in the wave's entries
     const uint32_t total = __builtin_amdgcn_readlane(inc, 63);
-    // Lane 0 holds the run's first key (a wave past n has total = 0).
-    const uint32_t key0 = __builtin_amdgcn_readfirstlane(lo);
     const uint32_t first = blockIdx.x * 256u + (uint32_t)(w * 64);
     uint32_t *own = s_own[w];
+    s_lo[w][lane] = lo;
+    s_start[w][lane] = start;
     uint32_t carry = 0;  // owner of the entry before the window
     for (uint32_t f0 = 0; f0 < total; f0 += 64) {
         own[lane] = 0xffffffffu;
@@ -243,9 +260,21 @@ __global__ __launch_bounds__(256) void k_cell_offsets(
         const int q = upto ? 63 - (int)__clzll((long long)upto) : 0;
         const uint32_t o = upto ? own[q] : carry;
         const uint32_t f = f0 + (uint32_t)lane;
-        if (f < total) off[key0 + f] = first + o;
+        if (f < total) off[s_lo[w][o] + (f - s_start[w][o])] = first + o;
         carry = __builtin_amdgcn_readlane(o, 63);
         wave_sync();
+    }
+}
+
+// The long ranges k_cell_offsets left: one workgroup per piece.
+__global__ __launch_bounds__(256) void k_fill_runs(const uint4 *__restrict__ runs,
+                                                   const uint32_t *__restrict__ nruns,
+                                                   uint32_t run_cap, uint32_t *__restrict__ off)
+{
+    const uint32_t nr = min(nruns[0], run_cap);
+    for (uint32_t r = blockIdx.x; r < nr; r += gridDim.x) {
+        const uint4 run = runs[r];
+        for (uint32_t c = threadIdx.x; c < run.y; c += 256) off[run.x + c] = run.z;
     }
 }
 
@@ -501,14 +530,22 @@ int radix_sort_pairs(uint32_t *keys, uint32_t *vals, bool vals_identity,
 }
 
 int launch_cell_offsets(const uint32_t *skeys, uint32_t n, uint32_t nkeys,
-                        uint32_t *off, hipStream_t s)
+                        uint32_t *off, uint4 *runs, uint32_t run_cap, uint32_t *nruns,
+                        hipStream_t s)
 {
-    k_cell_offsets<<<grid_for((int64_t)n + 1, 256), 256, 0, s>>>(skeys, n,
-                                                                  nkeys, off);
-    // (n + 1 owners; the last block's lanes past n own nothing.)
+    const int rc = launch_fill_u32(nruns, 0u, 1, s);
+    if (rc != CUCO_OK) return rc;
+    k_cell_offsets<<<grid_for((int64_t)n + 1, 256), 256, 0, s>>>(skeys, n, nkeys, off, runs,
+                                                                 run_cap, nruns);
+    CUCO_LAUNCH_CHECK();
+    k_fill_runs<<<2048, 256, 0, s>>>(runs, nruns, run_cap, off);
     CUCO_LAUNCH_CHECK();
     return CUCO_OK;
 }
+
+// Every range cut into pieces is longer than kLongRun and gives at most
+// len / kLongRun pieces; the ranges cover the nkeys + 1 entries once.
+uint32_t cell_run_capacity(uint32_t nkeys) { return (nkeys + 1u) / kLongRun + 2u; }
 
 int launch_cell_start_end(const uint32_t *off, uint32_t ncells,
                           uint32_t *start, uint32_t *end, hipStream_t s)
