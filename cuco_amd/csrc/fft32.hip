@@ -1,4 +1,5 @@
-// FP32 3D FFTs of power-of-two meshes (M = 32..1024) held in registers:
+// FP32 3D FFTs held in registers, for power-of-two meshes (M = 32..1024) and
+// meshes of 3 or 5 times a power of two (M = 96, 160, 192):
 // the transforms of pm_periodic.c:140 (forward r2c), :146-197 (Green's
 // function on the x lines) and :201 (c2r), for the FP32 path.
 //
@@ -14,7 +15,8 @@
 // exchange per pass (the Stockham kernels need one per radix stage), no
 // twiddle loads inside the small DFTs (their constants are compile-time).
 //
-// Blocks of 256 threads hold 256/N1 lines.  In the strided passes (y and x
+// Blocks of 256 threads (128 where 256 would need over 64 KB of LDS for the
+// exchange) hold threads/N1 lines.  In the strided passes (y and x
 // lines) consecutive lanes take consecutive lines, so each load and store of
 // a wave covers 128-byte runs of the mesh (16 lines x 8 bytes at N1 = 16);
 // in the z-row passes consecutive lanes take consecutive elements of a row.
@@ -115,10 +117,38 @@ __device__ __forceinline__ C twiddle(C x)
 
 // ------------------------------------------------------------ register DFTs
 // Length-R DFT of v[0..R) in place, natural order in and out (radix-2
-// decimation in time; every index and twiddle is a compile-time constant,
-// so v stays in registers).
+// decimation in time down to lengths 2, 4, 3 or 5; every index and twiddle
+// is a compile-time constant, so v stays in registers).
 template <int R, bool INV>
 struct RegDft;
+
+// Odd lengths (3, 5) by the definition X[k] = sum_j v[j] W_R^(j k).
+template <int R, bool INV>
+struct DirectDft {
+    template <int K, int J>
+    __device__ __forceinline__ static C term(const C *u)
+    {
+        if constexpr (J + 1 == R)
+            return twiddle<(J * K) % R, R, INV>(u[J]);
+        else
+            return c_add(twiddle<(J * K) % R, R, INV>(u[J]), term<K, J + 1>(u));
+    }
+    template <int K>
+    __device__ __forceinline__ static void out(C *v, const C *u)
+    {
+        if constexpr (K < R) {
+            v[K] = term<K, 0>(u);
+            out<K + 1>(v, u);
+        }
+    }
+    __device__ __forceinline__ static void run(C *v)
+    {
+        C u[R];
+#pragma unroll
+        for (int i = 0; i < R; ++i) u[i] = v[i];
+        out<0>(v, u);
+    }
+};
 
 template <int K, int R, bool INV>
 struct Combine {
@@ -160,6 +190,12 @@ struct RegDft<2, INV> {
 };
 
 template <bool INV>
+struct RegDft<3, INV> : DirectDft<3, INV> {};
+
+template <bool INV>
+struct RegDft<5, INV> : DirectDft<5, INV> {};
+
+template <bool INV>
 struct RegDft<4, INV> {
     __device__ __forceinline__ static void run(C *v)
     {
@@ -173,19 +209,40 @@ struct RegDft<4, INV> {
 };
 
 // ------------------------------------------------------------ shapes
-// Threads per line N1 (the rest, N2 = N/N1 >= N1, in registers).
-// S = 1: half as many threads per line (more values each, more lines per
-// block).
+constexpr bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+
+// Threads per line N1, a power of two (the rest, N2 = N/N1, a multiple of
+// N1, in registers).  Power-of-two lines: S = 1 takes half as many threads
+// per line (more values each, more lines per block).  Lines of 3 or 5 times
+// a power of two (48, 80, 96, 160, 192): 8 threads where N/8 is a multiple
+// of 8, else 4; S does not apply.
+constexpr int reg_n1(int N, int S)
+{
+    return is_pow2(N) ? (N <= 32 ? 4 : N <= 128 ? 8 : N <= 512 ? 16 : 32) >> S
+                      : ((N / 8) % 8 == 0 ? 8 : 4);
+}
+
+// Threads per block: 256, or 128 for a line length that is not a power of
+// two whose 256/N1 lines would need more than 64 KB of exchange slots.
+constexpr int reg_threads(int N, int S)
+{
+    return !is_pow2(N) && sizeof(Cx<float>) * (size_t)(kRegThreads / reg_n1(N, S)) * N > 65536
+               ? kRegThreads / 2
+               : kRegThreads;
+}
+
 template <int N, int S = 0>
 struct Shape {
-    static constexpr int N1 = (N <= 32 ? 4 : N <= 128 ? 8 : N <= 512 ? 16 : 32) >> S;
+    static constexpr int N1 = reg_n1(N, S);
     static constexpr int N2 = N / N1;
-    static constexpr int B = kRegThreads / N1;  // lines per block
+    static constexpr int T = reg_threads(N, S);  // threads per block
+    static constexpr int B = T / N1;             // lines per block
     // LDS exchange swizzle: slot k2*N1 + (g ^ ((k2 + Q*l) mod N1)) of line l
     // spreads every wave's accesses evenly over the banks.
     static constexpr int Q = N1 >= 32 ? 4 : 1;
     static constexpr size_t lds = sizeof(C) * (size_t)B * N;
     static_assert(N2 % N1 == 0, "N2 must be a multiple of N1");
+    static_assert(N1 * N2 == N && (is_pow2(N) || lds <= 65536), "register FFT shape");
 };
 
 template <int N, int S>
@@ -209,7 +266,7 @@ __device__ __forceinline__ void step_a(C *x, C *lds, int l, int g,
     RegDft<N2, INV>::run(x);
 #pragma unroll
     for (int k2 = 1; k2 < N2; ++k2) {
-        const C w = conj_if(tw[((g * k2) & (N - 1)) * ts], INV);
+        const C w = conj_if(tw[g * k2 * ts], INV);  // g * k2 < N
         x[k2] = {x[k2].x * w.x - x[k2].y * w.y, x[k2].x * w.y + x[k2].y * w.x};
     }
 #pragma unroll
@@ -248,7 +305,7 @@ __device__ __forceinline__ C *lds_area()
 // Real rows of length M = 2N -> N+1 complex, forward.  Rows mapping: lane
 // group of N1 threads per row.
 template <int N, int S>
-__global__ __launch_bounds__(kRegThreads) void k_rrows_r2c(const float *__restrict__ in_all,
+__global__ __launch_bounds__((Shape<N, S>::T)) void k_rrows_r2c(const float *__restrict__ in_all,
                                                            C *__restrict__ out_all, BoxView v,
                                                            const C *__restrict__ tw_all)
 {
@@ -279,12 +336,12 @@ __global__ __launch_bounds__(kRegThreads) void k_rrows_r2c(const float *__restri
 #pragma unroll
         for (int k1 = 0; k1 < N1; ++k1) rl[g + N1 * u + N2 * k1] = y[u * N1 + k1];
     __syncthreads();
-    for (int e = threadIdx.x; e < Zc * B; e += kRegThreads) {
+    for (int e = threadIdx.x; e < Zc * B; e += Shape<N, S>::T) {
         const int ll = e / Zc, k = e - ll * Zc;
         if (row0 + ll >= nrows) break;
         const C *r = lds + ll * N;
-        const C zk = r[k & (N - 1)];
-        const C zm = r[(N - k) & (N - 1)];
+        const C zk = r[k == N ? 0 : k];
+        const C zm = r[k == 0 ? 0 : N - k];
         const C s = {zk.x + zm.x, zk.y - zm.y}, d = {zk.x - zm.x, zk.y + zm.y};
         const C w = tw[k];  // W_M^k (Z_k - conj Z_{N-k})
         const C wd = {w.x * d.x - w.y * d.y, w.x * d.y + w.y * d.x};
@@ -294,7 +351,7 @@ __global__ __launch_bounds__(kRegThreads) void k_rrows_r2c(const float *__restri
 
 // N+1 complex -> real rows of length 2N, inverse (unnormalized).
 template <int N, int S>
-__global__ __launch_bounds__(kRegThreads) void k_rrows_c2r(const C *__restrict__ in_all,
+__global__ __launch_bounds__((Shape<N, S>::T)) void k_rrows_c2r(const C *__restrict__ in_all,
                                                            float *__restrict__ out_all, BoxView v,
                                                            const C *__restrict__ tw_all)
 {
@@ -352,7 +409,7 @@ __device__ __forceinline__ size_t rline_base(int L, int M, int Zc)
 }
 
 template <int N, bool INV, bool XAXIS, int S>
-__global__ __launch_bounds__(kRegThreads) void k_rlines(C *__restrict__ data_all, BoxView v,
+__global__ __launch_bounds__((Shape<N, S>::T)) void k_rlines(C *__restrict__ data_all, BoxView v,
                                                         const C *__restrict__ tw_all)
 {
     constexpr int N1 = Shape<N, S>::N1, N2 = Shape<N, S>::N2, B = Shape<N, S>::B;
@@ -391,7 +448,7 @@ __global__ __launch_bounds__(kRegThreads) void k_rlines(C *__restrict__ data_all
 // slots it read, so no other thread's slots are touched -- and after one
 // more barrier reads its own g back for the inverse length-N2 DFTs.
 template <int N, int S>
-__global__ __launch_bounds__(kRegThreads, (N >= 1024 || (S && N >= 512)) ? 1 : 2) void k_rgreen(C *__restrict__ data_all, BoxView v,
+__global__ __launch_bounds__((Shape<N, S>::T), (N >= 1024 || (S && N >= 512)) ? 1 : 2) void k_rgreen(C *__restrict__ data_all, BoxView v,
                                                         const C *__restrict__ tw_all,
                                                         const float *__restrict__ gexp_all,
                                                         const float *__restrict__ gsinc_all,
@@ -445,7 +502,7 @@ __global__ __launch_bounds__(kRegThreads, (N >= 1024 || (S && N >= 512)) ? 1 : 2
         RegDft<N1, true>::run(q);
 #pragma unroll
         for (int gp = 0; gp < N1; ++gp) {
-            const C w = conj_if(tw[(gp * k2) & (N - 1)], true);
+            const C w = conj_if(tw[gp * k2], true);  // gp * k2 < N
             lds[xslot<N, S>(l, gp, k2)] = {q[gp].x * w.x - q[gp].y * w.y,
                                         q[gp].x * w.y + q[gp].y * w.x};
         }
@@ -460,11 +517,29 @@ __global__ __launch_bounds__(kRegThreads, (N >= 1024 || (S && N >= 512)) ? 1 : 2
 }
 
 // ------------------------------------------------------------ launchers
+// The mesh sizes with register FFTs; bit i of BoxSet::fft32_mask, and the
+// i-th row and line maps, stand for kRegMeshes[i].
+constexpr int kRegMeshes[] = {32, 64, 128, 256, 512, 1024, 96, 160, 192};
+constexpr int kNRegMeshes = (int)(sizeof(kRegMeshes) / sizeof(kRegMeshes[0]));
+static_assert(kNRegMeshes == kRegFftSizes, "one block map per register-FFT size");
+
+constexpr int reg_index(int M)
+{
+    for (int i = 0; i < kNRegMeshes; ++i)
+        if (kRegMeshes[i] == M) return i;
+    return -1;
+}
+template <int M>
+constexpr int rows_map = kMapRowsR + reg_index(M);
+template <int M>
+constexpr int lines_map = kMapLinesR + reg_index(M);
+
 template <typename K, typename... Args>
-int launch_reg(K kernel, size_t lds, unsigned blocks, hipStream_t s, Args... args)
+int launch_reg(K kernel, size_t lds, int threads, unsigned blocks, hipStream_t s,
+               Args... args)
 {
     if (blocks == 0) return CUCO_OK;
-    kernel<<<xcd_grid(blocks), kRegThreads, lds, s>>>(args...);
+    kernel<<<xcd_grid(blocks), threads, lds, s>>>(args...);
     CUCO_LAUNCH_CHECK();
     return CUCO_OK;
 }
@@ -472,28 +547,28 @@ int launch_reg(K kernel, size_t lds, unsigned blocks, hipStream_t s, Args... arg
 template <int M, int S>
 int forward_m(const float *rho, C *c, const BoxSet &bs, const C *tw, hipStream_t s)
 {
-    int rc = launch_reg(k_rrows_r2c<M / 2, S>, Shape<M / 2, S>::lds, bs.total[kMapRowsR], s,
-                        rho, c, bs.view(0, kMapRowsR), tw);
+    int rc = launch_reg(k_rrows_r2c<M / 2, S>, Shape<M / 2, S>::lds, Shape<M / 2, S>::T, bs.total[rows_map<M>], s,
+                        rho, c, bs.view(0, rows_map<M>), tw);
     if (rc) return rc;
-    return launch_reg(k_rlines<M, false, false, S>, Shape<M, S>::lds, bs.total[kMapLinesR], s,
-                      c, bs.view(0, kMapLinesR), tw);
+    return launch_reg(k_rlines<M, false, false, S>, Shape<M, S>::lds, Shape<M, S>::T, bs.total[lines_map<M>], s,
+                      c, bs.view(0, lines_map<M>), tw);
 }
 
 template <int M, int S>
 int inverse_m(C *c, float *phi, const BoxSet &bs, const C *tw, hipStream_t s)
 {
-    int rc = launch_reg(k_rlines<M, true, false, S>, Shape<M, S>::lds, bs.total[kMapLinesR], s,
-                        c, bs.view(0, kMapLinesR), tw);
+    int rc = launch_reg(k_rlines<M, true, false, S>, Shape<M, S>::lds, Shape<M, S>::T, bs.total[lines_map<M>], s,
+                        c, bs.view(0, lines_map<M>), tw);
     if (rc) return rc;
-    return launch_reg(k_rrows_c2r<M / 2, S>, Shape<M / 2, S>::lds, bs.total[kMapRowsR], s,
-                      (const C *)c, phi, bs.view(0, kMapRowsR), tw);
+    return launch_reg(k_rrows_c2r<M / 2, S>, Shape<M / 2, S>::lds, Shape<M / 2, S>::T, bs.total[rows_map<M>], s,
+                      (const C *)c, phi, bs.view(0, rows_map<M>), tw);
 }
 
 template <int M, int S>
 int green_m(C *c, const BoxSet &bs, const PmTables &tb, hipStream_t s)
 {
-    return launch_reg(k_rgreen<M, S>, Shape<M, S>::lds, bs.total[kMapLinesR], s, c,
-                      bs.view(0, kMapLinesR), static_cast<const C *>(tb.tw),
+    return launch_reg(k_rgreen<M, S>, Shape<M, S>::lds, Shape<M, S>::T, bs.total[lines_map<M>], s, c,
+                      bs.view(0, lines_map<M>), static_cast<const C *>(tb.tw),
                       static_cast<const float *>(tb.gexp), static_cast<const float *>(tb.gsinc),
                       static_cast<const float *>(tb.gprod));
 }
@@ -502,23 +577,28 @@ template <int M, int S>
 int x_m(C *c, const BoxSet &bs, bool inverse, const C *tw, hipStream_t s)
 {
     if (inverse)
-        return launch_reg(k_rlines<M, true, true, S>, Shape<M, S>::lds, bs.total[kMapLinesR], s,
-                          c, bs.view(0, kMapLinesR), tw);
-    return launch_reg(k_rlines<M, false, true, S>, Shape<M, S>::lds, bs.total[kMapLinesR], s, c,
-                      bs.view(0, kMapLinesR), tw);
+        return launch_reg(k_rlines<M, true, true, S>, Shape<M, S>::lds, Shape<M, S>::T, bs.total[lines_map<M>], s,
+                          c, bs.view(0, lines_map<M>), tw);
+    return launch_reg(k_rlines<M, false, true, S>, Shape<M, S>::lds, Shape<M, S>::T, bs.total[lines_map<M>], s, c,
+                      bs.view(0, lines_map<M>), tw);
 }
 
 // One launch per mesh size present in the set.
 template <typename F>
 int each_mesh(const BoxSet &bs, F &&f)
 {
-    for (int i = 0; i < 6; ++i)
+    for (int i = 0; i < kNRegMeshes; ++i)
         if (bs.fft32_mask & (1u << i)) {
-            const int rc = f(32 << i);
+            const int rc = f(kRegMeshes[i]);
             if (rc) return rc;
         }
     return CUCO_OK;
 }
+
+// The split variant (S = 1) of a mesh size: its own shape for powers of
+// two, the S = 0 shape otherwise.
+template <int M>
+constexpr int kSplitS = is_pow2(M) ? 1 : 0;
 
 #define CUCO_FFT32_CASES(M, CALL)              \
     switch (M) {                               \
@@ -528,6 +608,9 @@ int each_mesh(const BoxSet &bs, F &&f)
     case 256: return CALL(256);                \
     case 512: return CALL(512);                \
     case 1024: return CALL(1024);              \
+    case 96: return CALL(96);                  \
+    case 160: return CALL(160);                \
+    case 192: return CALL(192);                \
     default: return (int)CUCO_ERR_UNSUPPORTED; \
     }
 
@@ -535,23 +618,22 @@ int each_mesh(const BoxSet &bs, F &&f)
 
 bool fft32_supported(int M)
 {
-    return M == 32 || M == 64 || M == 128 || M == 256 || M == 512 || M == 1024;
+    return fft32_mask_bit(M) != 0;
 }
 
 uint32_t fft32_mask_bit(int M)
 {
-    if (!fft32_supported(M)) return 0;
-    uint32_t i = 0;
-    while ((32 << i) != M) ++i;
-    return 1u << i;
+    for (int i = 0; i < kNRegMeshes; ++i)
+        if (kRegMeshes[i] == M) return 1u << i;
+    return 0;
 }
 
 uint32_t fft32_blocks(int M, bool rows, int split)
 {
     if (!fft32_supported(M)) return 0;
     const int N = rows ? M / 2 : M;
-    const int N1 = (N <= 32 ? 4 : N <= 128 ? 8 : N <= 512 ? 16 : 32) >> (split ? 1 : 0);
-    const uint32_t B = (uint32_t)(kRegThreads / N1);
+    const int S = split && is_pow2(M) ? 1 : 0;
+    const uint32_t B = (uint32_t)(reg_threads(N, S) / reg_n1(N, S));
     const uint32_t lines = rows ? (uint32_t)M * M : (uint32_t)M * (M / 2 + 1);
     return (lines + B - 1) / B;
 }
@@ -561,7 +643,7 @@ int fft32_forward(const float *rho, Cx<float> *c, const BoxSet &bs, const PmTabl
 {
     const C *tw = static_cast<const C *>(tb.tw);
     return each_mesh(bs, [&](int M) -> int {
-#define CALL(m) (bs.fft_split ? forward_m<m, 1>(rho, c, bs, tw, s) : forward_m<m, 0>(rho, c, bs, tw, s))
+#define CALL(m) (bs.fft_split ? forward_m<m, kSplitS<m>>(rho, c, bs, tw, s) : forward_m<m, 0>(rho, c, bs, tw, s))
         CUCO_FFT32_CASES(M, CALL)
 #undef CALL
     });
@@ -572,7 +654,7 @@ int fft32_inverse(Cx<float> *c, float *phi, const BoxSet &bs, const PmTables &tb
 {
     const C *tw = static_cast<const C *>(tb.tw);
     return each_mesh(bs, [&](int M) -> int {
-#define CALL(m) (bs.fft_split ? inverse_m<m, 1>(c, phi, bs, tw, s) : inverse_m<m, 0>(c, phi, bs, tw, s))
+#define CALL(m) (bs.fft_split ? inverse_m<m, kSplitS<m>>(c, phi, bs, tw, s) : inverse_m<m, 0>(c, phi, bs, tw, s))
         CUCO_FFT32_CASES(M, CALL)
 #undef CALL
     });
@@ -581,7 +663,7 @@ int fft32_inverse(Cx<float> *c, float *phi, const BoxSet &bs, const PmTables &tb
 int fft32_green_x(Cx<float> *c, const BoxSet &bs, const PmTables &tb, hipStream_t s)
 {
     return each_mesh(bs, [&](int M) -> int {
-#define CALL(m) (bs.fft_split ? green_m<m, 1>(c, bs, tb, s) : green_m<m, 0>(c, bs, tb, s))
+#define CALL(m) (bs.fft_split ? green_m<m, kSplitS<m>>(c, bs, tb, s) : green_m<m, 0>(c, bs, tb, s))
         CUCO_FFT32_CASES(M, CALL)
 #undef CALL
     });
@@ -591,7 +673,7 @@ int fft32_x(Cx<float> *c, const BoxSet &bs, bool inverse, const PmTables &tb, hi
 {
     const C *tw = static_cast<const C *>(tb.tw);
     return each_mesh(bs, [&](int M) -> int {
-#define CALL(m) (bs.fft_split ? x_m<m, 1>(c, bs, inverse, tw, s) : x_m<m, 0>(c, bs, inverse, tw, s))
+#define CALL(m) (bs.fft_split ? x_m<m, kSplitS<m>>(c, bs, inverse, tw, s) : x_m<m, 0>(c, bs, inverse, tw, s))
         CUCO_FFT32_CASES(M, CALL)
 #undef CALL
     });

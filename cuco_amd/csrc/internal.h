@@ -7,6 +7,9 @@
 namespace cuco {
 
 // ------------------------------------------------------------ boxes
+// Mesh sizes with register FFTs (fft32_mask_bit).
+constexpr int kRegFftSizes = 9;
+
 // Work units counted by the block maps (one prefix over boxes each).
 enum BoxMap : int {
     kMapPart = 0,   // 256 particles per block
@@ -15,9 +18,12 @@ enum BoxMap : int {
     kMapRowsF,      // FFT z-row blocks, FP32
     kMapLinesD,     // FFT y/x-line blocks, FP64
     kMapLinesF,     // FFT y/x-line blocks, FP32
-    kMapRowsR,      // register-FFT z-row blocks (FP32, power-of-two meshes)
-    kMapLinesR,     // register-FFT y/x-line blocks
-    kMapBrick,      // 16^3-cell Morton bricks (FP32 deposit, interpolation)
+    // Register-FFT z-row and y/x-line blocks, one map per mesh size
+    // (fft32_mask_bit order): a launch for one size covers only the boxes
+    // of that size.
+    kMapRowsR,
+    kMapLinesR = kMapRowsR + kRegFftSizes,
+    kMapBrick = kMapLinesR + kRegFftSizes,  // 16^3-cell Morton bricks (FP32 deposit, interpolation)
     kMapCount
 };
 
@@ -31,14 +37,14 @@ struct BoxSet {
     uint32_t *maps = nullptr;        // kMapCount x (nboxes+1)
     uint32_t total[kMapCount] = {};  // blocks per launch
     uint32_t *tile_map = nullptr;    // nboxes+1: first pair-kernel tile per box
-    uint32_t fft32_mask = 0;         // bit i: a box has mesh 32 << i (register FFTs)
+    uint32_t fft32_mask = 0;         // register-FFT mesh sizes present (fft32_mask_bit)
     int fft_split = 0;               // register-FFT line split (fft32_blocks)
     BoxView view(int mode, int map) const
     {
         return BoxView{dev[mode], maps + (size_t)map * (nboxes + 1), nboxes};
     }
 };
-// Blocks of one box of mesh size M for a PM map (kMapGatherX..kMapLinesR);
+// Blocks of one box of mesh size M for a PM map (kMapGatherX..kMapBrick);
 // fft_split selects the register-FFT line split.
 uint32_t pm_blocks(int M, int map, int fft_split = 0);
 
@@ -214,13 +220,16 @@ int pm_fft_x(Cx<T> *c, const BoxSet &bs, bool inverse, const PmTables &tb,
              hipStream_t s);
 bool fft_size_supported(int M);
 // FP32 register-resident FFTs of the boxes whose mesh is a power of two in
-// 32..1024 (fft32_supported); the generic FP32 maps count no blocks for
+// 32..1024, or 96, 160 or 192 (fft32_supported); the generic FP32 maps count
+// no blocks for
 // those boxes, so every box takes the same kernels whatever set it is in.
 // One launch per mesh size present (bs.fft32_mask).
 bool fft32_supported(int M);
 uint32_t fft32_mask_bit(int M);
 // split 0: N1 = 4/8/16/32 threads per line of length N <= 32/128/512/1024;
 // split 1: half as many threads per line, each holding twice the values.
+// Lines of 3 or 5 times a power of two: 8 threads where N/8 is a multiple
+// of 8, else 4, whatever the split.
 uint32_t fft32_blocks(int M, bool rows, int split = 0);
 int fft32_forward(const float *rho, Cx<float> *c, const BoxSet &bs, const PmTables &tb,
                   hipStream_t s);

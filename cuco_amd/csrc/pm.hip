@@ -11,8 +11,8 @@
 //           FP32: per 16^3-cell Morton brick a node tile summed in 32-bit
 //           fixed point in LDS, brick faces added in a fixed order.
 //  FFT      hand-written Stockham autosort FFT in LDS, radix 4/2/3/5 (FP64,
-//           and FP32 meshes that are not powers of two; the FP32
-//           power-of-two meshes use the register FFT of fft32.hip):
+//           and the FP32 meshes fft32.hip's register FFT does not take:
+//           it takes 32..1024 in powers of two, 96, 160 and 192):
 //           z rows: real length-M r2c through a length-M/2 complex FFT and the
 //                   split post-process; c2r the mirror image;
 //           y lines: strided batched complex FFT, lines contiguous in z so
@@ -1242,6 +1242,11 @@ size_t pm_deposit_scratch_bytes(const BoxSet &bs, int fp64)
 
 uint32_t pm_blocks(int M, int map, int fft_split)
 {
+    if (map >= kMapRowsR && map < kMapBrick) {  // a register-FFT size's own map
+        const bool rows = map < kMapLinesR;
+        const int size = map - (rows ? kMapRowsR : kMapLinesR);
+        return fft32_mask_bit(M) == (1u << size) ? fft32_blocks(M, rows, fft_split) : 0u;
+    }
     const uint32_t Zc = (uint32_t)(M / 2 + 1);
     switch (map) {
     case kMapGatherX: return gather_blocks(M);
@@ -1262,8 +1267,6 @@ uint32_t pm_blocks(int M, int map, int fft_split)
         while ((1 << bits) < M) ++bits;
         return bits >= 4 ? 1u << (3 * (bits - 4)) : 1u;
     }
-    case kMapRowsR: return fft32_blocks(M, true, fft_split);
-    case kMapLinesR: return fft32_blocks(M, false, fft_split);
     default: return 0;
     }
 }

@@ -89,10 +89,11 @@ def test_halo128_pair_force_error_both_variants(sym):
     assert np.median(err) <= 1e-6
 
 
-@pytest.mark.parametrize("M", [40, 48, 96])
+@pytest.mark.parametrize("M", [40, 48, 96, 160, 192])
 def test_error_percentiles_other_meshes(orc, M):
-    """Meshes that are not powers of two (LDS Stockham FFT) or not multiples
-    of the 16-cell deposit and interpolation bricks (the last brick wraps)."""
+    """Meshes that are not powers of two (40, 48: LDS Stockham FFT; 96, 160,
+    192: register FFT with radix-3 and radix-5 steps) or not multiples of
+    the 16-cell deposit and interpolation bricks (the last brick wraps)."""
     box = ics.small_box(f"mesh{M}", 24, M, 60.0, 30 + M, "halos", n_halos=8)
     pos, vel, mass = ics.generate(box)
     eng, errs = fp32_errors(box, pos, mass, orc)
@@ -194,7 +195,7 @@ def test_enumeration_level_and_fft_split_give_the_same_forces(M):
     """The autotuner's candidate-enumeration level of the each-pair-once
     kernel (mesh cells, 2x2x2-cell blocks for wide tiles, blocks for every
     tile), its tile width (64 or 128 targets) and the register-FFT line
-    split (96: not a power of two, the Stockham kernels run whatever the
+    split (96: not a power of two, one register-FFT shape whatever the
     split) change only FP32 rounding: PP within 1e-5 and PM within 1e-5 of
     the force scale."""
     box = halo_box(32, M, 100.0, n_halos=20, seed=M)
