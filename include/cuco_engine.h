@@ -184,11 +184,12 @@ typedef struct cuco_tuning {
                                      2x2x2-cell blocks for wider ones; 1 mesh
                                      cells for every tile; 2 two-cell blocks
                                      for every tile (even meshes) */
-    int32_t fft_split;            /* FP32 register FFTs (power-of-two
-                                     meshes): 0 (the default) 4/8/16/32
+    int32_t fft_split;            /* FP32 register FFTs of power-of-two
+                                     meshes: 0 (the default) 4/8/16/32
                                      threads per line of length <= 32/128/
                                      512/1024; 1 half as many, each holding
-                                     twice the values */
+                                     twice the values (meshes 96, 160, 192:
+                                     one shape either way) */
     int32_t pp_wide_tiles;        /* each-pair-once FP32 kernel: 0 (the
                                      default) tiles of up to 64 targets, one
                                      per lane; 1 tiles of up to 128, two per
