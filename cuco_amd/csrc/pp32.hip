@@ -922,6 +922,10 @@ __device__ inline uint32_t super_tiles(const uint32_t *__restrict__ off, uint32_
 {
     const size_t base = (size_t)sb << (shift3 + 3);
     uint32_t t = t0, gs = off[base], gn = 0;
+    // An empty super-block (voids, and the padding of a Morton key space
+    // around a mesh that is not a power of two) has no tiles: two reads
+    // instead of nine.
+    if (off[base + ((size_t)8 << shift3)] == gs) return 0;
     auto emit = [&](uint32_t start, uint32_t n) {
         if (tstart) {
             tstart[t] = start;
