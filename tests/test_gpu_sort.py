@@ -85,3 +85,31 @@ def test_engine_sort_with_25_to_27_bit_keys(M, orc):
     assert (dk >= 0).all()
     assert (np.diff(i32.astype(np.int64))[dk == 0] > 0).all()
     assert np.array_equal(np.sort(i32), np.arange(n, dtype=np.uint32))
+
+
+def test_cell_table_with_long_empty_ranges(orc):
+    """Cell table of a 256^3-cell box holding 20000 particles in two small
+    clumps: the empty key ranges between them run to millions of cells (the
+    first and the last range included), so they go to the long-range list
+    and are filled in pieces of 65,536 entries.  Every cell's start and end
+    must be the oracle's.  A second particle set with the clumps elsewhere,
+    on the same handle, shows that no entry keeps the first table's value."""
+    M, L, n = 256, 100.0, 20000
+    asmth = 1.25 * L / M
+    kw = dict(pm_grid=M, box_size=L, G=1.0, asmth=asmth, rcut=4.5 * asmth, soft_h=0.1 * L / M)
+    p = orc.params(M, L, 1.0, asmth, 4.5 * asmth, 0.1 * L / M)
+    eng = E.Engine(max_particles=n, **kw)
+    rng = np.random.default_rng(77)
+    for centres in ((0.3, 0.7), (0.12, 0.55)):
+        pos = np.empty((n, 3))
+        for h, c in enumerate(centres):
+            pos[h * n // 2:(h + 1) * n // 2] = L * c + rng.random((n // 2, 3)) * (0.02 * L)
+        eng.upload(pos, None, np.ones(n))
+        eng.gravity_linklist(E.FP64)
+        ok, _ = orc.sort_pairs(*orc.calc_keys(p, pos))
+        s, e = eng.cell_table()
+        os_, oe = orc.cell_table(ok, M ** 3)
+        gaps = np.diff(np.concatenate([[-1], np.unique(ok).astype(np.int64), [M ** 3]])) - 1
+        assert gaps.max() > 65536 and gaps[0] > 2048 and gaps[-1] > 2048
+        assert np.array_equal(s, os_) and np.array_equal(e, oe), centres
+    eng.close()
