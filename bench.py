@@ -155,6 +155,18 @@ def ensemble_bench(E, ics, nboxes, steps, tune=True):
     run_batched()                       # captures the step graph
     t_batched = run_batched()
     bpos, _ = ens.download_particles()
+    # Stage times of eager FP32 evaluations on the final state (outside the
+    # timed region; the first evaluation re-sorts after the last drift).
+    ens.compute_accelerations(E.FP32, True)
+    ens.synchronize()
+    ens.set_timing(True)
+    ens.stage_times(reset=True)
+    for _ in range(3):
+        ens.compute_accelerations(E.FP32, True)
+    ens.synchronize()
+    st, nev = ens.stage_times(reset=True)
+    ens.set_timing(False)
+    stage_ms = {k: v / max(nev, 1) for k, v in st.items()}
     ens.close()
     log(f"[bench] ensemble batched {t_batched * 1e3 / steps:.2f} ms/step")
 
@@ -192,6 +204,7 @@ def ensemble_bench(E, ics, nboxes, steps, tune=True):
         "batched_evals_per_s": n_total * steps / t_batched,
         "bit_identical_to_looped": identical,
         "error_vs_fp64": errors,
+        "stage_ms_fp32": stage_ms,
     }
 
 
