@@ -97,7 +97,7 @@ def cpu_baseline(box, pos, mass, n_targets, seed=5006, gpu_fp64=None):
     }
 
 
-def ensemble_bench(E, ics, nboxes, steps, tune=True):
+def ensemble_bench(E, ics, nboxes, steps, tune=True, looped=True):
     """The 64-box ensemble: FP32 KDK steps (each one hipGraph launch) of all
     boxes batched in one handle against the same boxes run one after
     another through single-box handles.  The ensemble handle is autotuned
@@ -171,7 +171,7 @@ def ensemble_bench(E, ics, nboxes, steps, tune=True):
     log(f"[bench] ensemble batched {t_batched * 1e3 / steps:.2f} ms/step")
 
     t_looped, identical = 0.0, True
-    for b, (box, (p, v, m)) in enumerate(zip(boxes, data)):
+    for b, (box, (p, v, m)) in enumerate(zip(boxes, data) if looped else ()):
         eng = E.Engine(max_particles=box.N, **box.engine_kwargs())
         eng.set_tuning(**tuning)
         for rep in range(2):            # the first run captures the graph
@@ -186,8 +186,9 @@ def ensemble_bench(E, ics, nboxes, steps, tune=True):
         o = sum(x.N for x in boxes[:b])
         identical = identical and bool(np.array_equal(bpos[o:o + box.N], sp))
         eng.close()
-    log(f"[bench] ensemble looped {t_looped * 1e3 / steps:.2f} ms/step, "
-        f"identical={identical}")
+    if looped:
+        log(f"[bench] ensemble looped {t_looped * 1e3 / steps:.2f} ms/step, "
+            f"identical={identical}")
     n_total = sum(b.N for b in boxes)
     sides = sorted({b.n_side for b in boxes})
     return {
@@ -199,10 +200,10 @@ def ensemble_bench(E, ics, nboxes, steps, tune=True):
         "tuning": tuning,
         "steps": steps,
         "batched_ms_per_step": t_batched * 1e3 / steps,
-        "looped_ms_per_step": t_looped * 1e3 / steps,
-        "batched_over_looped": t_looped / t_batched,
+        "looped_ms_per_step": t_looped * 1e3 / steps if looped else None,
+        "batched_over_looped": t_looped / t_batched if looped else None,
         "batched_evals_per_s": n_total * steps / t_batched,
-        "bit_identical_to_looped": identical,
+        "bit_identical_to_looped": identical if looped else None,
         "error_vs_fp64": errors,
         "stage_ms_fp32": stage_ms,
     }
@@ -481,6 +482,11 @@ def main() -> int:
     ap.add_argument("--pp-symmetric", choices=["auto", "0", "1"], default="auto",
                     help="FP32 pair force evaluating each pair once: the "
                          "autotuner's choice, or forced off / on")
+    ap.add_argument("--ensemble-only", action="store_true",
+                    help="run only the ensemble leg and print its JSON object "
+                         "(a run for a kernel trace of the batched ensemble)")
+    ap.add_argument("--ensemble-looped", type=int, choices=[0, 1], default=1,
+                    help="time the ensemble's boxes one by one as well (0: batched only)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -491,6 +497,13 @@ def main() -> int:
 
     from cuco_amd import engine as E
     from cuco_amd import ics
+
+    if args.ensemble_only:
+        print(json.dumps({"ensemble": ensemble_bench(E, ics, args.ensemble_boxes,
+                                                     args.ensemble_steps,
+                                                     looped=bool(args.ensemble_looped))}),
+              flush=True)
+        return 0
 
     box = ics.WORKLOADS[args.workload]
     t0 = time.perf_counter()
@@ -673,7 +686,8 @@ def main() -> int:
         out["timeline"] = timeline_bench(E, ics, box, pos, vel, mass, args.timeline_steps)
     if args.ensemble_boxes > 0:
         out["ensemble"] = ensemble_bench(E, ics, args.ensemble_boxes,
-                                         args.ensemble_steps)
+                                         args.ensemble_steps,
+                                         looped=bool(args.ensemble_looped))
     if args.halo512_steps > 0 and box.name != "halo512":
         out["halo512"] = halo512_bench(E, ics, tuning, args.halo512_steps,
                                        args.halo512_oracle,
